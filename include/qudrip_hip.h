@@ -1,0 +1,235 @@
+/*
+ * qudrip_hip.h -- C ABI of the MI355X (gfx950) sparse-Hamiltonian Krylov path.
+ *
+ * This is the boundary a QuDrip caller binds to.  The reference library is
+ * header-only C++ (no FFI); each entry point below replaces one call of its
+ * template API, cited as reference file:line.  The C++ drop-in headers under
+ * include/qudrip/ re-create that template API on top of these functions.
+ *
+ * Conventions
+ *  - Every function returns an int status (QD_OK == 0); qd_last_error(ctx)
+ *    returns a message for the last failure on that context.  No C++
+ *    exception ever crosses this boundary.
+ *  - Complex numbers are interleaved (re, im) doubles, the memory layout of
+ *    std::complex<double>.
+ *  - Arguments named d_* are DEVICE pointers (allocated with qd_malloc, or by
+ *    any HIP allocator on the context's device).  Arguments named h_* or
+ *    documented as host buffers are caller-owned host memory read or written
+ *    only during the call.
+ *  - A context owns one HIP stream.  Calls on one context must be serialised
+ *    by the caller; distinct contexts are independent.
+ *  - Handles (basis, termprog, csr, linop, krylov) own their device memory and
+ *    must be destroyed before their context.
+ */
+#ifndef QUDRIP_HIP_H
+#define QUDRIP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+  QD_OK = 0,
+  QD_E_ARG = 1,         /* invalid argument                                   */
+  QD_E_OOM = 2,         /* device allocation failed                           */
+  QD_E_HIP = 3,         /* a HIP runtime call failed                          */
+  QD_E_BREAKDOWN = 4,   /* Krylov breakdown (|w| < 1e-10) before any vector   */
+  QD_E_EIGEN = 5,       /* small dense eigensolver / expm failed (NaN input)  */
+  QD_E_CAPACITY = 6,    /* a row exceeds the kernel's per-row capacity        */
+  QD_E_UNSUPPORTED = 7, /* sector or operator shape not supported             */
+  QD_E_BUDGET = 8,      /* request does not fit the HBM budget                */
+  QD_E_STATE = 9        /* call not valid in the handle's current state       */
+};
+
+/* ------------------------------------------------------------------ handles */
+typedef struct qd_ctx qd_ctx;
+typedef struct qd_basis qd_basis;
+typedef struct qd_termprog qd_termprog;
+typedef struct qd_csr qd_csr;
+typedef struct qd_linop qd_linop;
+typedef struct qd_krylov qd_krylov;
+
+/* ------------------------------------------------------------ data formats */
+
+/* One conserved quantity: popcount(x & mask) == count.
+ * Replaces Nset (conservation.hpp:89-115) with `cq = value` (:78-81):
+ * getNset masks bits 0..N (:233-236), getNup bits 0..N-1 (:238-241),
+ * getNdo bits N..2N-1 (:243-246). */
+typedef struct {
+  uint64_t mask;
+  int64_t count;
+} qd_constraint;
+
+/* One elementary gate, as stored by the reference after `gate << U` and
+ * `gate(site)` / `gate(site2, site1)`.
+ *   kind == QD_OP_BIT : bitOp (operators/bit.hpp:6-84).  site_a = site.
+ *       U holds U_[i*2+j] = U(i,j) (operator.hpp:24-29) as 4 interleaved
+ *       complex numbers; the child with output bit o of input bit b gets
+ *       (jw * U(b,o)) * val (bit.hpp:57-63).
+ *   kind == QD_OP_HOP : hopBitOp (operators/hopbit.hpp:6-84), gate(site2,
+ *       site1) = c+_{site2} c_{site1}: site_a = site2, site_b = site1.
+ *       U is unused.
+ *   string = +1 (getBoseGate / getHopBoseGate) or -1 (fermionic, Jordan-
+ *       Wigner sign from the bits above the site). */
+enum { QD_OP_BIT = 0, QD_OP_HOP = 1 };
+typedef struct {
+  int32_t kind;
+  int32_t site_a;
+  int32_t site_b;
+  int32_t string;
+  double U[8];
+} qd_op;
+
+/* One term of an OperatorSum (operator_abel.hpp:9-128): coefficient and a
+ * product of ops ops[op_begin .. op_begin+n_ops-1] in Operator::ops_ order
+ * (leftmost factor first; applied right-to-left, operator.hpp:151). */
+typedef struct {
+  double coeff[2];
+  int32_t op_begin;
+  int32_t n_ops;
+} qd_term;
+
+#define QD_MAX_OPS_PER_TERM 8
+#define QD_MAX_BITS 64
+
+/* ------------------------------------------------------------------ context */
+int qd_ctx_create(int device, qd_ctx** out);
+/* Same, but the context issues its work on an existing hipStream_t owned by
+ * the caller (e.g. a framework's current stream), so the two orders agree. */
+int qd_ctx_create_on_stream(int device, void* stream, qd_ctx** out);
+int qd_ctx_destroy(qd_ctx* ctx);
+const char* qd_last_error(const qd_ctx* ctx);
+int qd_ctx_sync(qd_ctx* ctx);
+/* The context's hipStream_t, for callers that interleave their own work. */
+void* qd_ctx_stream(qd_ctx* ctx);
+/* Number of ABI functions; lets a loader check it bound the right library. */
+int qd_abi_version(void);
+
+/* ------------------------------------------------------- device memory */
+int qd_malloc(qd_ctx* ctx, size_t bytes, void** d_ptr);
+int qd_free(qd_ctx* ctx, void* d_ptr);
+int qd_memcpy_h2d(qd_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int qd_memcpy_d2h(qd_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+/* ------------------------------------------------------------------- basis */
+/* Constrain(QbitsIndex(nbits), cq = v, ...) (conservation.hpp:212-231):
+ * all x in [0, 2^nbits) meeting every constraint, in ascending x.  The
+ * states are produced on the GPU by combinatorial-number-system unranking
+ * (disjoint contiguous masks) or by a predicate scan (any other masks). */
+int qd_basis_create(qd_ctx* ctx, int nbits, const qd_constraint* cons, int ncons,
+                    qd_basis** out);
+int qd_basis_destroy(qd_basis* basis);
+/* SubIndex::range() (index.hpp:357). */
+int64_t qd_basis_size(const qd_basis* basis);
+/* SubIndex::mapping, i.e. the unrank table (index.hpp:242-248); host buffer
+ * of qd_basis_size() entries. */
+int qd_basis_download(qd_basis* basis, uint64_t* h_states);
+/* Device pointer to the same table (valid while the basis lives). */
+const uint64_t* qd_basis_device_states(qd_basis* basis);
+/* SubIndex::operator idxv_type() (index.hpp:331-337): rank of each state;
+ * a state outside the sector ranks to qd_basis_size().  Device arrays. */
+int qd_basis_rank(qd_basis* basis, const uint64_t* d_states, int64_t count,
+                  int64_t* d_ranks);
+
+/* -------------------------------------------------------------- term program */
+/* An OperatorSum flattened by the drop-in headers.  nbits is the width of
+ * the QbitsIndex the gates act on; every site must be < nbits. */
+int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, int nterms,
+                       const qd_op* ops, int nops, qd_termprog** out);
+int qd_termprog_destroy(qd_termprog* prog);
+
+/* --------------------------------------------------------------- CSR build */
+/* `ham >> index` (operator_abel.hpp:25-34 over operator.hpp:235-267):
+ * two-pass count/fill on the GPU.  The arrays are the reference's Eigen
+ * ColMajor storage: outer[n+1] indexed by SOURCE state (column), inner[nnz]
+ * target rows ascending, values complex128 -- bit-identical to the
+ * reference's summation order (duplicates of one term in leaf order, then
+ * acc = (+0,+0); acc += c_k * v_k in term order; explicit zeros kept). */
+int qd_csr_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_csr** out);
+int qd_csr_destroy(qd_csr* csr);
+int qd_csr_info(const qd_csr* csr, int64_t* n, int64_t* nnz, int* hermitian);
+/* Host buffers of n+1, nnz and 2*nnz entries. */
+int qd_csr_download(qd_csr* csr, int32_t* h_outer, int32_t* h_inner, double* h_vals);
+/* Device pointers to the same arrays (valid while the csr lives). */
+int qd_csr_device_arrays(qd_csr* csr, const int32_t** d_outer, const int32_t** d_inner,
+                         const double** d_vals);
+
+/* ------------------------------------------------------- linear operators */
+/* y = H x through the stored matrix. */
+int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);
+/* y = H x evaluated on the fly from the term program (no stored matrix);
+ * the same H as `prog >> basis` (operator.hpp:215-232 is the lazy form). */
+int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out);
+int qd_linop_destroy(qd_linop* op);
+int64_t qd_linop_dim(const qd_linop* op);
+/* Lanes per row of the stored-matrix product (2, 4, ..., 64); chosen from
+ * nnz/n at creation, overridable by a tuner. */
+int qd_linop_set_lanes(qd_linop* op, int lanes);
+
+/* ------------------------------------------------------- vector kernels */
+/* d_y = H d_x (complex128, n entries). */
+int qd_spmv(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y);
+/* <psi|H|psi>, unnormalised (State::eval, state.hpp:33-44).  out: host[2]. */
+int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* h_out);
+/* Re(psi^H psi) (Hermitian::norm, sparse_algorithm.hpp:7-16). */
+int qd_norm2(qd_ctx* ctx, int64_t n, const double* d_psi, double* h_out);
+/* Seeded replacement of shuffle_state (utils.hpp:129-140): re, im of entry
+ * i from splitmix64((seed << 32) ^ (2i + c)) mapped to U(-1, 1), then
+ * normalised. */
+int qd_seeded_state(qd_ctx* ctx, int64_t n, uint64_t seed, double* d_psi);
+
+/* ------------------------------------------------------------ Krylov solver */
+/* Arnoldi::getSolver (herm = 0) / getLanczosSolver (herm = 1)
+ * (sparse_algorithm.hpp:596-604).  Keeps up to kry_dim+1 basis vectors on
+ * the device, full (CGS2) reorthogonalisation, FP64 throughout. */
+int qd_krylov_create(qd_ctx* ctx, int64_t n, int kry_dim, int herm, qd_krylov** out);
+int qd_krylov_destroy(qd_krylov* kr);
+/* ArnoldiSolver::getGroundState(H, psi, tstp, add_dim) (:564-585): starts
+ * from d_psi if no basis is held yet, extends the basis by add_dim vectors
+ * (-1: up to kry_dim), solves the tridiagonal Ritz problem in FP64 and
+ * overwrites d_psi with the lowest Ritz vector (not renormalised).
+ * E0 (host, may be NULL) receives the lowest Ritz value. */
+int qd_krylov_ground_state(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim,
+                           double* h_E0);
+/* ArnoldiSolver::evolve(H, psi, tstp, h, kry_dim_rt) (:548-562):
+ * d_psi_out = V_m expm(-i h K_m) e_1 with V_m[:,0] = psi_in / |psi_in| and
+ * m = kry_dim_rt - 1 (kry_dim_rt = 0: kry_dim); like the reference the
+ * input norm is not carried and the result is not renormalised.
+ * d_psi_out may equal d_psi_in. */
+int qd_krylov_evolve(qd_krylov* kr, qd_linop* op, const double* d_psi_in, double* d_psi_out,
+                     double h, int kry_dim_rt);
+typedef struct {
+  int act_dim;      /* ArnoldiSolver::act_dim() */
+  int breakdown;    /* 1 if the last expansion stopped on |w| < 1e-10 */
+  double error;     /* ArnoldiSolver::error() */
+  double beta_last; /* trailing |w| of the last expansion */
+  int64_t n_spmv;   /* operator applications so far */
+} qd_krylov_info;
+int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info);
+/* Ritz values of the last ground-state solve, ascending (storeE, :587-593).
+ * Writes min(cap, act_dim) values; returns their count through *count. */
+int qd_krylov_eigenvalues(qd_krylov* kr, double* h_evals, int cap, int* count);
+/* ArnoldiSolver::convergeEV(N, iv) (:494-500). */
+int qd_krylov_converge_ev(qd_krylov* kr, int N, double iv_re, double iv_im);
+/* Forget the held basis (act_dim = 0). */
+int qd_krylov_reset(qd_krylov* kr);
+
+/* ------------------------------------------------- FP64 host numerics */
+/* The small dense problems of the solvers, exposed for direct checks.
+ * Symmetric tridiagonal eigenpairs (d[m], e[m-1]) -> w[m] ascending and,
+ * if z != NULL, column-major eigenvectors z[m*m] (SelfAdjointEigenSolver on
+ * the Krylov matrix, sparse_algorithm.hpp:571). */
+int qd_host_tridiag_eig(int m, const double* d, const double* e, double* w, double* z);
+/* E = expm(A), m x m complex column-major interleaved (MatrixExp,
+ * matrix.hpp:20; used at sparse_algorithm.hpp:559). */
+int qd_host_expm(int m, const double* A, double* E);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUDRIP_HIP_H */

@@ -1,0 +1,224 @@
+"""CPU oracle: plain-C restatement of the reference algorithms (oracle.c).
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline; the
+product (qudrip_amd/, include/) never imports, links or calls it.
+
+Each wrapper names the reference function it restates; oracle.c documents
+how each one is pinned independently (tests/test_oracle.py).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = ctypes.CDLL(LIB_PATH)
+    L.orc_basis_scan.restype = ctypes.c_int64
+    L.orc_basis_scan.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int64]
+    L.orc_rank.restype = ctypes.c_int64
+    L.orc_rank.argtypes = [_VP, ctypes.c_int64, ctypes.c_uint64]
+    L.orc_binom.restype = ctypes.c_uint64
+    L.orc_binom.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.orc_expand_term.restype = ctypes.c_int
+    L.orc_expand_term.argtypes = [_VP, ctypes.c_int, ctypes.c_uint64, _VP, _VP, _VP, ctypes.c_int]
+    L.orc_build_csc.restype = ctypes.c_int64
+    L.orc_build_csc.argtypes = [_VP, ctypes.c_int64, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, ctypes.c_int64]
+    L.orc_map_acc.restype = None
+    L.orc_map_acc.argtypes = [_VP, ctypes.c_int64, _VP, ctypes.c_int, _VP, _VP, _VP, _VP]
+    L.orc_spmv_csc.restype = None
+    L.orc_spmv_csc.argtypes = [ctypes.c_int64, _VP, _VP, _VP, _VP, _VP]
+    L.orc_norm2.restype = ctypes.c_double
+    L.orc_norm2.argtypes = [ctypes.c_int64, _VP]
+    L.orc_eval.restype = None
+    L.orc_eval.argtypes = [ctypes.c_int64, _VP, _VP, _VP, _VP, _VP]
+    L.orc_uniform.restype = ctypes.c_double
+    L.orc_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.orc_seeded_state.restype = None
+    L.orc_seeded_state.argtypes = [ctypes.c_int64, ctypes.c_uint64, _VP, ctypes.c_int]
+    L.orc_tridiag_eigvals.restype = ctypes.c_int
+    L.orc_tridiag_eigvals.argtypes = [ctypes.c_int, _VP, _VP, _VP]
+    L.orc_tridiag_eigvec.restype = ctypes.c_int
+    L.orc_tridiag_eigvec.argtypes = [ctypes.c_int, _VP, _VP, ctypes.c_double, _VP]
+    L.orc_lanczos_gs.restype = ctypes.c_int
+    L.orc_lanczos_gs.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP, _VP]
+    L.orc_expm.restype = ctypes.c_int
+    L.orc_expm.argtypes = [ctypes.c_int, _VP, _VP]
+    L.orc_arnoldi_evolve.restype = ctypes.c_int
+    L.orc_arnoldi_evolve.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, _VP]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def basis_scan(sector):
+    """Constrain / LooseConstrain (conservation.hpp:212-231): ascending scan."""
+    from qudrip_amd.abi import qd_constraint
+    cons = (qd_constraint * max(1, len(sector.constraints)))()
+    for i, c in enumerate(sector.constraints):
+        cons[i].mask = c.mask
+        cons[i].count = c.count
+    n = lib().orc_basis_scan(sector.nbits, cons, len(sector.constraints), None, 0)
+    out = np.empty(max(n, 1), dtype=np.uint64)
+    lib().orc_basis_scan(sector.nbits, cons, len(sector.constraints), _p(out), n)
+    return out[:n]
+
+
+def rank(basis, x):
+    """SubIndex -> inv_mapping lookup (index.hpp:331-337); miss -> len(basis)."""
+    return lib().orc_rank(_p(basis), len(basis), int(x))
+
+
+def binom(n, k):
+    return lib().orc_binom(n, k)
+
+
+def build_csc(basis, opsum):
+    """`ham >> index` (operator_abel.hpp:25-34, operator.hpp:235-267)."""
+    terms, nt, ops, no = opsum.program()
+    n = len(basis)
+    colptr = np.empty(n + 1, dtype=np.int32)
+    nnz = lib().orc_build_csc(_p(basis), n, terms, nt, ops, _p(colptr), None, None, 0)
+    if nnz < 0:
+        raise RuntimeError("oracle csc build failed")
+    rowidx = np.empty(max(nnz, 1), dtype=np.int32)
+    vals = np.empty(2 * max(nnz, 1), dtype=np.float64)
+    lib().orc_build_csc(_p(basis), n, terms, nt, ops, _p(colptr), _p(rowidx), _p(vals), nnz)
+    return colptr, rowidx[:nnz], vals[:2 * nnz]
+
+
+def expand_term(opsum, k, state, init=(1.0, 0.0)):
+    terms, nt, ops, no = opsum.program()
+    t = terms[k]
+    cap = 1 << 8
+    idx = np.zeros(cap, dtype=np.uint64)
+    val = np.zeros(2 * cap, dtype=np.float64)
+    ini = np.array(init, dtype=np.float64)
+    base = ctypes.addressof(ops) + t.op_begin * ctypes.sizeof(ops[0])
+    nl = lib().orc_expand_term(ctypes.c_void_p(base), t.n_ops, int(state), _p(ini), _p(idx), _p(val), cap)
+    return idx[:nl], val[:2 * nl].view(np.complex128)
+
+
+def map_acc(basis, opsum, frm, to, coeff=1.0 + 0j):
+    """OperatorSum::map_acc (operator_abel.hpp:36-42; operator.hpp:215-232)."""
+    terms, nt, ops, no = opsum.program()
+    c = np.array([coeff.real, coeff.imag], dtype=np.float64)
+    frm = np.ascontiguousarray(frm, dtype=np.complex128)
+    assert to.dtype == np.complex128 and to.flags.c_contiguous
+    lib().orc_map_acc(_p(basis), len(basis), terms, nt, ops, _p(frm), _p(to), _p(c))
+    return to
+
+
+def spmv(csc, x):
+    """Eigen ColMajor sparse * dense (sparse_algorithm.hpp:378,404)."""
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty(n, dtype=np.complex128)
+    lib().orc_spmv_csc(n, _p(colptr), _p(rowidx), _p(vals), _p(x), _p(y))
+    return y
+
+
+def eval_expect(csc, psi):
+    """State::eval (state.hpp:33-44)."""
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.empty(2, dtype=np.float64)
+    lib().orc_eval(n, _p(colptr), _p(rowidx), _p(vals), _p(psi), _p(out))
+    return complex(out[0], out[1])
+
+
+def seeded_state(n, seed, normalise=True):
+    """Seeded replacement of shuffle_state (utils.hpp:129-140)."""
+    out = np.empty(n, dtype=np.complex128)
+    lib().orc_seeded_state(n, seed, _p(out), 1 if normalise else 0)
+    return out
+
+
+def tridiag_eigvals(d, e):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    w = np.empty(len(d))
+    if lib().orc_tridiag_eigvals(len(d), _p(d), _p(e), _p(w)) != 0:
+        raise RuntimeError("tridiag eigvals failed")
+    return w
+
+
+def tridiag_eigvec(d, e, lam):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    z = np.empty(len(d))
+    if lib().orc_tridiag_eigvec(len(d), _p(d), _p(e), lam, _p(z)) != 0:
+        raise RuntimeError("tridiag eigvec failed")
+    return z
+
+
+def lanczos_gs(csc, k, psi0):
+    """getLanczosSolver(idx,k).getGroundState (sparse_algorithm.hpp:395-473,564-585)."""
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    out = np.empty(n, dtype=np.complex128)
+    e0 = np.zeros(1)
+    ev = np.zeros(k + 1)
+    al = np.zeros(k + 1)
+    be = np.zeros(k + 1)
+    act = lib().orc_lanczos_gs(n, _p(colptr), _p(rowidx), _p(vals), k, _p(psi0), _p(out), _p(e0), _p(ev),
+                               _p(al), _p(be))
+    if act < 0:
+        raise RuntimeError("oracle lanczos failed")
+    return {"E0": e0[0], "psi": out, "evals": ev[:act], "alpha": al[:act], "beta": be[:act], "act_dim": act}
+
+
+def expm(A):
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m = A.shape[0]
+    E = np.empty((m, m), dtype=np.complex128, order="F")
+    if lib().orc_expm(m, _p(A), _p(E)) != 0:
+        raise RuntimeError("oracle expm failed")
+    return E
+
+
+def arnoldi_evolve(csc, k_rt, h, psi):
+    """getSolver(idx,k).evolve (sparse_algorithm.hpp:372-393,548-562)."""
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.empty(n, dtype=np.complex128)
+    act = lib().orc_arnoldi_evolve(n, _p(colptr), _p(rowidx), _p(vals), k_rt, h, _p(psi), _p(out))
+    if act < 0:
+        raise RuntimeError("oracle arnoldi failed")
+    return out
+
+
+def csc_to_dense(csc):
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    v = vals.view(np.complex128)
+    M = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        for e in range(colptr[j], colptr[j + 1]):
+            M[rowidx[e], j] += v[e]
+    return M

@@ -1,0 +1,249 @@
+// ctx.hip -- context, error reporting, device memory helpers, prefix scan.
+#include <cstdarg>
+#include <cstdio>
+
+#include "qd_internal.hpp"
+
+int qd_set_error(qd_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+extern "C" int qd_abi_version(void) { return 1; }
+
+extern "C" int qd_ctx_create(int device, qd_ctx** out) {
+  if (!out) return QD_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return QD_E_HIP;
+  if (device < 0 || device >= ndev) return QD_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return QD_E_HIP;
+  qd_ctx* c = new qd_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return QD_E_HIP;
+  }
+  c->own_stream = true;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return QD_OK;
+}
+
+// Bind a context to an existing stream (e.g. the caller's framework stream),
+// so that work on both is ordered without extra synchronisation.
+extern "C" int qd_ctx_create_on_stream(int device, void* stream, qd_ctx** out) {
+  int rc = qd_ctx_create(device, out);
+  if (rc != QD_OK) return rc;
+  qd_ctx* c = *out;
+  hipStreamDestroy(c->stream);
+  c->stream = (hipStream_t)stream;
+  c->own_stream = false;
+  return QD_OK;
+}
+
+extern "C" int qd_ctx_destroy(qd_ctx* ctx) {
+  if (!ctx) return QD_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return QD_OK;
+}
+
+extern "C" const char* qd_last_error(const qd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int qd_ctx_sync(qd_ctx* ctx) {
+  if (!ctx) return QD_E_ARG;
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QD_OK;
+}
+
+extern "C" void* qd_ctx_stream(qd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int qd_malloc(qd_ctx* ctx, size_t bytes, void** d_ptr) {
+  if (!ctx || !d_ptr) return QD_E_ARG;
+  *d_ptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipSetDevice(ctx->device);
+  hipError_t e = hipMalloc(d_ptr, bytes);
+  if (e != hipSuccess) return qd_set_error(ctx, QD_E_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return QD_OK;
+}
+
+extern "C" int qd_free(qd_ctx* ctx, void* d_ptr) {
+  if (!ctx) return QD_E_ARG;
+  if (!d_ptr) return QD_OK;
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  QD_HIP(ctx, hipFree(d_ptr));
+  return QD_OK;
+}
+
+extern "C" int qd_memcpy_h2d(qd_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+  if (!ctx || (!d_dst && bytes) || (!h_src && bytes)) return QD_E_ARG;
+  QD_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QD_OK;
+}
+
+extern "C" int qd_memcpy_d2h(qd_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (!d_src && bytes) || (!h_dst && bytes)) return QD_E_ARG;
+  QD_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QD_OK;
+}
+
+int qd_ctx_scratch(qd_ctx* ctx, size_t bytes, void** p) {
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) {
+      QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      hipFree(ctx->scratch);
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes;
+    hipError_t e = hipMalloc(&ctx->scratch, nb);
+    if (e != hipSuccess) return qd_set_error(ctx, QD_E_OOM, "scratch hipMalloc(%zu): %s", nb, hipGetErrorString(e));
+    ctx->scratch_bytes = nb;
+  }
+  *p = ctx->scratch;
+  return QD_OK;
+}
+
+int qd_ctx_pinned(qd_ctx* ctx, size_t count, double** p) {
+  if (count > ctx->pinned_count) {
+    if (ctx->pinned) {
+      QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      hipHostFree(ctx->pinned);
+      ctx->pinned = nullptr;
+      ctx->pinned_count = 0;
+    }
+    size_t nc = count < 4096 ? 4096 : count;
+    QD_HIP(ctx, hipHostMalloc((void**)&ctx->pinned, nc * sizeof(double), hipHostMallocDefault));
+    ctx->pinned_count = nc;
+  }
+  *p = ctx->pinned;
+  return QD_OK;
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of int32 counts into int64 offsets[n+1]: per-block sums,
+// a single-block scan of the block sums, then a per-block local scan.
+#define SCAN_BLOCK 1024
+#define SCAN_ITEMS 4  // items per thread
+
+__global__ void scan_block_sums(const int32_t* __restrict__ counts, int64_t n, int64_t* __restrict__ bsum) {
+  __shared__ int64_t red[SCAN_BLOCK / 64];
+  int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+  int64_t s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + (int64_t)k * SCAN_BLOCK + threadIdx.x;
+    if (i < n) s += counts[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) t += red[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// single block: exclusive scan of nb block sums in place, total to out_total
+__global__ void scan_bsums(int64_t* bsum, int64_t nb, int64_t* out_total) {
+  __shared__ int64_t carry;
+  __shared__ int64_t wsum[SCAN_BLOCK / 64];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nb; base += SCAN_BLOCK) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = (i < nb) ? bsum[i] : 0;
+    // inclusive wave scan
+    int lane = threadIdx.x & 63;
+    int64_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t acc = 0;
+      for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        int64_t t = wsum[w];
+        wsum[w] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+    int64_t excl = carry + wsum[threadIdx.x >> 6] + x - v;
+    if (i < nb) bsum[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ void scan_local(const int32_t* __restrict__ counts, int64_t n, const int64_t* __restrict__ bsum,
+                           int64_t* __restrict__ offsets) {
+  __shared__ int64_t wsum[SCAN_BLOCK / 64];
+  __shared__ int64_t carry;
+  int64_t base = (int64_t)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+  if (threadIdx.x == 0) carry = bsum[blockIdx.x];
+  __syncthreads();
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + (int64_t)k * SCAN_BLOCK + threadIdx.x;
+    int64_t v = (i < n) ? counts[i] : 0;
+    int lane = threadIdx.x & 63;
+    int64_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t acc = 0;
+      for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        int64_t t = wsum[w];
+        wsum[w] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+    int64_t excl = carry + wsum[threadIdx.x >> 6] + x - v;
+    if (i < n) offsets[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) carry = excl + v;
+    __syncthreads();
+  }
+}
+
+int qd_scan_counts(qd_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets) {
+  int64_t per = (int64_t)SCAN_BLOCK * SCAN_ITEMS;
+  int64_t nb = (n + per - 1) / per;
+  if (nb < 1) nb = 1;
+  void* sp = nullptr;
+  int rc = qd_ctx_scratch(ctx, sizeof(int64_t) * (nb + 1), &sp);
+  if (rc) return rc;
+  int64_t* bsum = (int64_t*)sp;
+  scan_block_sums<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum);
+  QD_CHECK_LAUNCH(ctx);
+  scan_bsums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(bsum, nb, d_offsets + n);
+  QD_CHECK_LAUNCH(ctx);
+  scan_local<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum, d_offsets);
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}

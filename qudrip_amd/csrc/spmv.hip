@@ -1,0 +1,366 @@
+// spmv.hip -- K5 (stored-matrix product), K6 (matrix-free product), and the
+// deterministic vector reductions used by eval / norm / Krylov (K10).
+//
+// Layout.  The stored matrix is the reference's ColMajor arrays (outer =
+// source column).  For a Hermitian H, column r conjugated is row r, so
+//     y_r = sum_{e in column r} conj(val_e) * x_{inner_e}
+// is a row-parallel "pull" product with no atomics and a fixed summation
+// order.  Non-Hermitian sums use the columns of the adjoint program instead
+// (same formula), built once by qd_linop_from_csr.
+#include "qd_internal.hpp"
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------- K5
+// G lanes cooperate on one row: each lane strides the row's entries by G,
+// then a butterfly over the G lanes (fixed tree) combines them.  A wave of 64
+// lanes covers 64/G consecutive rows, whose entries are contiguous in memory,
+// so value/column loads are coalesced.  The matrix streams through with
+// non-temporal loads to leave L2/MALL to the gathered x.
+//
+// Block-to-XCD remap: the hardware deals workgroups round-robin over the 8
+// XCDs; logical block = (b % 8) * per + b / 8 gives each XCD a contiguous
+// range of rows, so neighbouring rows (which gather neighbouring x entries)
+// share one L2.
+template <int G>
+__global__ void __launch_bounds__(256)
+    spmv_pull_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                     const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
+                     const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
+                     cplx shift) {
+  constexpr int RPB = 256 / G;
+  const unsigned nb = gridDim.x;
+  const unsigned per = nb >> 3;  // grid is a multiple of 8
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x % G;
+  const int64_t row = (int64_t)lb * RPB + g;
+  int32_t beg = 0, end = 0;
+  if (row < nrows) {
+    beg = outer[row];
+    end = outer[row + 1];
+  }
+  double sx = 0.0, sy = 0.0;
+  const dbl2* vp = reinterpret_cast<const dbl2*>(vals);
+  for (int32_t e = beg + l; e < end; e += G) {
+    dbl2 v = __builtin_nontemporal_load(vp + e);
+    int32_t c = __builtin_nontemporal_load(inner + e);
+    cplx xv = x[c];
+    sx += v.x * xv.x + v.y * xv.y;
+    sy += v.x * xv.y - v.y * xv.x;
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    sx += __shfl_xor(sx, o, G);
+    sy += __shfl_xor(sy, o, G);
+  }
+  if (l == 0 && row < nrows) {
+    double s = xscale ? xscale[xscale_idx] : 1.0;
+    cplx r = cmake(sx * s, sy * s);
+    if (diag || shift.x != 0.0 || shift.y != 0.0) {
+      cplx xr = x[row];
+      cplx d = diag ? cadd(diag[row], shift) : shift;
+      r = cadd(r, crmul(s, cmul(d, xr)));
+    }
+    y[row] = r;
+  }
+}
+
+// ---------------------------------------------------------------- K6
+// Matrix-free pull product from the adjoint program: expanding column r of
+// A^dagger = sum conj(c_k) P_k^dagger from state r gives leaves (t, w) with
+// w = P_k^dagger[t, r] = conj(P_k[r, t]), so
+//     y_r = sum_k c_k sum_leaves conj(w) x_t = sum conj(conj(c_k) w) x_t.
+// One thread per row; the term loop is uniform across the wave, so term and
+// op records are fetched through the scalar cache.
+__global__ void __launch_bounds__(256)
+    spmv_matfree_kernel(SectorDesc sec, const uint64_t* __restrict__ states, int64_t n,
+                        const DevTerm* __restrict__ aterms, int nterms, const DevOp* __restrict__ aops,
+                        const cplx* __restrict__ x, cplx* __restrict__ y, const double* __restrict__ xscale,
+                        int xscale_idx, const cplx* __restrict__ diag, cplx shift) {
+  const unsigned nb = gridDim.x;
+  const unsigned per = nb >> 3;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  int64_t r = (int64_t)lb * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint64_t s = states[r];
+  double sx = 0.0, sy = 0.0;
+  for (int k = 0; k < nterms; ++k) {
+    const DevTerm t = aterms[k];
+    const DevOp* tops = aops + t.op_begin;
+    int nbitops = 0;
+    for (int q = 0; q < t.n_ops; ++q) nbitops += (tops[q].kind == QD_OP_BIT);
+    for (int L = 0; L < t.n_leaves; ++L) {
+      uint64_t xt;
+      cplx w;
+      if (!eval_leaf(tops, t.n_ops, nbitops, s, L, &xt, &w)) continue;
+      if (!(hypot(w.x, w.y) > QD_LEAF_EPS)) continue;
+      uint64_t ti = qd_rank(sec, states, xt);
+      if (ti >= (uint64_t)n) continue;
+      cplx cw = cmul(t.coeff, w);  // conj(c_k) w
+      cplx xv = x[ti];
+      sx += cw.x * xv.x + cw.y * xv.y;  // conj(cw) * xv
+      sy += cw.x * xv.y - cw.y * xv.x;
+    }
+  }
+  double sc = xscale ? xscale[xscale_idx] : 1.0;
+  cplx res = cmake(sx * sc, sy * sc);
+  if (diag || shift.x != 0.0 || shift.y != 0.0) {
+    cplx d = diag ? cadd(diag[r], shift) : shift;
+    res = cadd(res, crmul(sc, cmul(d, x[r])));
+  }
+  y[r] = res;
+}
+
+// ---------------------------------------------------------------- dispatch
+static unsigned grid8(int64_t blocks) {
+  int64_t g = (blocks + 7) / 8 * 8;
+  return (unsigned)(g < 8 ? 8 : g);
+}
+
+// y = s * (H + diag + shift) x, s = xscale[idx] (device) or 1
+int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale, int xscale_idx,
+                   const cplx* d_diag, cplx shift) {
+  qd_ctx* ctx = op->ctx;
+  int64_t n = op->n;
+  if (n == 0) return QD_OK;
+  if (op->kind == QD_LINOP_CSR) {
+    switch (op->lanes_per_row) {
+#define QD_SPMV_CASE(GG)                                                                                    \
+  case GG:                                                                                                  \
+    spmv_pull_kernel<GG><<<grid8((n + (256 / GG) - 1) / (256 / GG)), 256, 0, ctx->stream>>>(              \
+        n, op->outer, op->inner, op->vals, x, y, d_xscale, xscale_idx, d_diag, shift);                      \
+    break;
+      QD_SPMV_CASE(2)
+      QD_SPMV_CASE(4)
+      QD_SPMV_CASE(8)
+      QD_SPMV_CASE(16)
+      QD_SPMV_CASE(32)
+      QD_SPMV_CASE(64)
+#undef QD_SPMV_CASE
+      default:
+        return qd_set_error(ctx, QD_E_ARG, "lanes_per_row %d", op->lanes_per_row);
+    }
+  } else {
+    qd_termprog* p = op->prog;
+    spmv_matfree_kernel<<<grid8((n + 255) / 256), 256, 0, ctx->stream>>>(
+        op->basis->sec, op->basis->d_states, n, p->d_adj_terms, (int)p->h_terms.size(), p->d_adj_ops, x, y,
+        d_xscale, xscale_idx, d_diag, shift);
+  }
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}
+
+static int pick_lanes(int64_t n, int64_t nnz) {
+  double avg = n ? (double)nnz / (double)n : 1.0;
+  int g = 2;
+  while (g < 64 && g < avg) g <<= 1;
+  return g;
+}
+
+extern "C" int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out) {
+  if (!ctx || !csr || !out) return qd_set_error(ctx, QD_E_ARG, "qd_linop_from_csr: null argument");
+  *out = nullptr;
+  qd_csr* use = csr;
+  if (csr->hermitian != 1) {
+    if (!csr->adj) {
+      if (!csr->basis || !csr->prog)
+        return qd_set_error(ctx, QD_E_UNSUPPORTED, "non-Hermitian matrix without its term program");
+      int rc = qd_csr_build_impl(ctx, csr->basis, csr->prog, 1, &csr->adj);
+      if (rc) return rc;
+    }
+    use = csr->adj;
+  }
+  qd_linop* op = new qd_linop();
+  op->ctx = ctx;
+  op->kind = QD_LINOP_CSR;
+  op->n = use->n;
+  op->outer = use->d_outer;
+  op->inner = use->d_inner;
+  op->vals = use->d_vals;
+  op->nnz = use->nnz;
+  op->lanes_per_row = pick_lanes(use->n, use->nnz);
+  *out = op;
+  return QD_OK;
+}
+
+extern "C" int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out) {
+  if (!ctx || !basis || !prog || !out) return qd_set_error(ctx, QD_E_ARG, "qd_linop_matfree: null argument");
+  *out = nullptr;
+  if (prog->nbits > basis->sec.nbits)
+    return qd_set_error(ctx, QD_E_ARG, "program acts on %d bits, basis has %d", prog->nbits, basis->sec.nbits);
+  int rc = qd_termprog_ensure_adjoint(prog);
+  if (rc) return rc;
+  qd_linop* op = new qd_linop();
+  op->ctx = ctx;
+  op->kind = QD_LINOP_MATFREE;
+  op->n = (int64_t)basis->sec.total;
+  op->basis = basis;
+  op->prog = prog;
+  *out = op;
+  return QD_OK;
+}
+
+extern "C" int qd_linop_destroy(qd_linop* op) {
+  delete op;
+  return QD_OK;
+}
+
+extern "C" int64_t qd_linop_dim(const qd_linop* op) { return op ? op->n : -1; }
+
+// Override the lanes-per-row choice of a stored-matrix operator (2..64, a
+// power of two); used by the tile autotuner.
+extern "C" int qd_linop_set_lanes(qd_linop* op, int lanes) {
+  if (!op || op->kind != QD_LINOP_CSR) return QD_E_ARG;
+  if (lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return QD_E_ARG;
+  op->lanes_per_row = lanes;
+  return QD_OK;
+}
+
+extern "C" int qd_spmv(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y) {
+  if (!ctx || !op || (op->n && (!d_x || !d_y))) return qd_set_error(ctx, QD_E_ARG, "qd_spmv: null argument");
+  if (d_x == d_y) return qd_set_error(ctx, QD_E_ARG, "qd_spmv: x and y must not alias");
+  return qd_linop_apply(op, (const cplx*)d_x, (cplx*)d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+}
+
+// ---------------------------------------------------------------- reductions
+// Deterministic dot products: a fixed grid (function of n only) accumulates
+// grid-stride partial sums, a wave butterfly and an LDS pass give one value
+// per block, and one block sums the block values in index order.  The result
+// depends on n alone, never on what else runs on the device.
+#define RED_THREADS 256
+static int red_blocks(int64_t n) {
+  int64_t b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+__device__ inline cplx block_reduce_c(cplx v, cplx* sh) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  cplx r = cmake(0.0, 0.0);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = cadd(r, sh[i]);
+  return r;
+}
+
+// partial[b] = sum over the block's grid-stride slice of conj(a_i) b_i
+__global__ void __launch_bounds__(RED_THREADS)
+    dot_partial_kernel(int64_t n, const cplx* __restrict__ a, const cplx* __restrict__ b, cplx* __restrict__ part) {
+  __shared__ cplx sh[RED_THREADS / 64];
+  cplx s = cmake(0.0, 0.0);
+  for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_THREADS) {
+    cplx av = a[i], bv = b[i];
+    s = cadd(s, cdotmul(av, bv));
+  }
+  cplx r = block_reduce_c(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+    sum_partials_kernel(const cplx* __restrict__ part, int nb, cplx* __restrict__ out) {
+  __shared__ cplx sh[RED_THREADS / 64];
+  cplx s = cmake(0.0, 0.0);
+  for (int i = threadIdx.x; i < nb; i += RED_THREADS) s = cadd(s, part[i]);
+  cplx r = block_reduce_c(s, sh);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// d_out (device, one complex) = a^H b
+int qd_dev_dot(qd_ctx* ctx, int64_t n, const cplx* a, const cplx* b, cplx* d_out) {
+  int nb = red_blocks(n);
+  void* sp = nullptr;
+  int rc = qd_ctx_scratch(ctx, sizeof(cplx) * (nb + 8), &sp);
+  if (rc) return rc;
+  cplx* part = (cplx*)sp;
+  dot_partial_kernel<<<nb, RED_THREADS, 0, ctx->stream>>>(n, a, b, part);
+  sum_partials_kernel<<<1, RED_THREADS, 0, ctx->stream>>>(part, nb, d_out);
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}
+
+static int read_one(qd_ctx* ctx, const cplx* d_val, double* h2) {
+  double* pin = nullptr;
+  int rc = qd_ctx_pinned(ctx, 2, &pin);
+  if (rc) return rc;
+  QD_HIP(ctx, hipMemcpyAsync(pin, d_val, sizeof(cplx), hipMemcpyDeviceToHost, ctx->stream));
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  h2[0] = pin[0];
+  h2[1] = pin[1];
+  return QD_OK;
+}
+
+extern "C" int qd_norm2(qd_ctx* ctx, int64_t n, const double* d_psi, double* h_out) {
+  if (!ctx || !h_out || (n && !d_psi)) return qd_set_error(ctx, QD_E_ARG, "qd_norm2: null argument");
+  cplx* d_res = nullptr;
+  QD_HIP(ctx, hipMallocAsync((void**)&d_res, sizeof(cplx), ctx->stream));
+  int rc = qd_dev_dot(ctx, n, (const cplx*)d_psi, (const cplx*)d_psi, d_res);
+  double r[2] = {0, 0};
+  if (!rc) rc = read_one(ctx, d_res, r);
+  hipFreeAsync(d_res, ctx->stream);
+  if (rc) return rc;
+  *h_out = r[0];
+  return QD_OK;
+}
+
+extern "C" int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* h_out) {
+  if (!ctx || !op || !h_out || (op->n && !d_psi)) return qd_set_error(ctx, QD_E_ARG, "qd_expect: null argument");
+  int64_t n = op->n;
+  cplx* tmp = nullptr;
+  QD_HIP(ctx, hipMallocAsync((void**)&tmp, sizeof(cplx) * (n + 1), ctx->stream));
+  int rc = qd_linop_apply(op, (const cplx*)d_psi, tmp, nullptr, 0, nullptr, cmake(0.0, 0.0));
+  if (!rc) rc = qd_dev_dot(ctx, n, (const cplx*)d_psi, tmp, tmp + n);
+  double r[2] = {0, 0};
+  if (!rc) rc = read_one(ctx, tmp + n, r);
+  hipFreeAsync(tmp, ctx->stream);
+  if (rc) return rc;
+  h_out[0] = r[0];
+  h_out[1] = r[1];
+  return QD_OK;
+}
+
+// ---------------------------------------------------------------- seeded state
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double seeded_uniform(uint64_t seed, uint64_t c) {
+  uint64_t u = splitmix64((seed << 32) ^ c);
+  return ((double)(u >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+}
+__global__ void seeded_fill_kernel(int64_t n, uint64_t seed, cplx* __restrict__ psi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  psi[i] = cmake(seeded_uniform(seed, 2 * (uint64_t)i), seeded_uniform(seed, 2 * (uint64_t)i + 1));
+}
+__global__ void scale_by_inv_sqrt_kernel(int64_t n, cplx* __restrict__ psi, const cplx* __restrict__ nrm2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = sqrt(nrm2->x);
+  cplx v = psi[i];
+  psi[i] = cmake(v.x / s, v.y / s);
+}
+
+extern "C" int qd_seeded_state(qd_ctx* ctx, int64_t n, uint64_t seed, double* d_psi) {
+  if (!ctx || (n && !d_psi)) return qd_set_error(ctx, QD_E_ARG, "qd_seeded_state: null argument");
+  if (n == 0) return QD_OK;
+  cplx* psi = (cplx*)d_psi;
+  seeded_fill_kernel<<<qd_grid(n, 256), 256, 0, ctx->stream>>>(n, seed, psi);
+  cplx* d_res = nullptr;
+  QD_HIP(ctx, hipMallocAsync((void**)&d_res, sizeof(cplx), ctx->stream));
+  int rc = qd_dev_dot(ctx, n, psi, psi, d_res);
+  if (!rc) scale_by_inv_sqrt_kernel<<<qd_grid(n, 256), 256, 0, ctx->stream>>>(n, psi, d_res);
+  hipFreeAsync(d_res, ctx->stream);
+  if (rc) return rc;
+  QD_CHECK_LAUNCH(ctx);
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QD_OK;
+}

@@ -1,0 +1,365 @@
+"""Python mirror of the reference's operator interface, producing term programs.
+
+Same names and argument meaning as the reference headers:
+  getQbits / getNset / getNup / getNdo / Constrain   (index.hpp, conservation.hpp)
+  getBoseGate / getFermiGate / getHopFermiGate / getHopBoseGate, ``gate << U``,
+  ``gate(site)``, ``gate(site2, site1)``                (operators/bit.hpp, hopbit.hpp)
+  Operator products, OperatorSum algebra, getEmptyOperator (operator.hpp,
+  operator_abel.hpp)
+The reference's ``cq = value`` assignment becomes ``cq == value``.
+
+``OperatorSum.program()`` flattens the sum into the qd_term / qd_op arrays of
+include/qudrip_hip.h, in the reference's term order.  No device work happens
+here: this is host logic that both the Python tests/bench and the C++
+drop-in headers (include/qudrip/) implement identically.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+II = 1j
+pauli_x = np.array([[0, 1], [1, 0]], dtype=np.complex128)       # bit.hpp:42
+pauli_y = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)    # bit.hpp:43
+pauli_z = np.array([[1, 0], [0, -1]], dtype=np.complex128)      # bit.hpp:44
+id2 = np.array([[1, 0], [0, 1]], dtype=np.complex128)           # bit.hpp:45
+
+
+class QbitsIndex:
+    """QbitsIndex (index.hpp:177-191): range 1 << N."""
+
+    def __init__(self, nbits):
+        self.nbits = int(nbits)
+
+    def range(self):
+        return 1 << self.nbits
+
+
+def getQbits(N):
+    return QbitsIndex(N)
+
+
+class Constraint:
+    def __init__(self, mask, count):
+        self.mask = int(mask)
+        self.count = int(count)
+
+
+class Nset:
+    """popcount(x & mask) over bits start..end inclusive (conservation.hpp:89-115)."""
+
+    def __init__(self, index, start=-1, end=-1):
+        if start == -1:
+            start = 0
+        if end == -1:
+            end = bin(index.range() - 1).count("1")  # N: masks N+1 bits (:112)
+        self.index = index
+        self.mask = 0
+        for i in range(start, end + 1):
+            if i < 64:
+                self.mask |= 1 << i
+
+    def __eq__(self, value):  # reference: `cq = value`
+        return Constraint(self.mask, value)
+
+
+def getNset(index):
+    return Nset(index, -1, -1)
+
+
+def getNup(index, Nsite):
+    return Nset(index, 0, Nsite - 1)
+
+
+def getNdo(index, Nsite):
+    return Nset(index, Nsite, 2 * Nsite - 1)
+
+
+class Sector:
+    """Result of Constrain: the QbitsIndex width plus the constraints."""
+
+    def __init__(self, index, constraints):
+        self.index = index
+        self.nbits = index.nbits
+        self.constraints = list(constraints)
+
+    def ctypes_constraints(self):
+        arr = (abi.qd_constraint * max(1, len(self.constraints)))()
+        for i, c in enumerate(self.constraints):
+            arr[i].mask = c.mask
+            arr[i].count = c.count
+        return arr, len(self.constraints)
+
+
+def Constrain(index, *constraints):
+    """Constrain(index, cq == v, ...) (conservation.hpp:227-231)."""
+    return Sector(index, constraints)
+
+
+class _OpRecord:
+    """One gate instance, i.e. one shared_ptr<elOpBase> (operator.hpp:46)."""
+
+    __slots__ = ("kind", "site_a", "site_b", "string", "U")
+
+    def __init__(self, kind, site_a, site_b, string, U):
+        self.kind = kind
+        self.site_a = site_a
+        self.site_b = site_b
+        self.string = string
+        self.U = U
+
+
+class Gate:
+    """bitOp / hopBitOp prototype: holds U and the JW string (+1 / -1)."""
+
+    def __init__(self, index, kind, string):
+        self.index = index
+        self.kind = kind
+        self.string = string
+        self.U = np.zeros((2, 2), dtype=np.complex128)
+
+    def __lshift__(self, U):  # elOpBase::operator<< (operator.hpp:24-29)
+        self.U = np.array(U, dtype=np.complex128).reshape(2, 2).copy()
+        return self
+
+    def __call__(self, site, site1=None):
+        if self.kind == abi.QD_OP_BIT:  # bit.hpp:17-21
+            if site1 is not None:
+                raise TypeError("bit gate takes one site")
+            return Operator([_OpRecord(self.kind, int(site), int(site), self.string, self.U.copy())])
+        if site1 is None:  # hopbit.hpp:17-22: gate(site2, site1) = c+_{site2} c_{site1}
+            raise TypeError("hop gate takes (site2, site1)")
+        return Operator([_OpRecord(self.kind, int(site), int(site1), self.string, self.U.copy())])
+
+
+def getBoseGate(index):
+    return Gate(index, abi.QD_OP_BIT, 1)
+
+
+def getFermiGate(index):
+    return Gate(index, abi.QD_OP_BIT, -1)
+
+
+def getHopFermiGate(index):
+    return Gate(index, abi.QD_OP_HOP, -1)
+
+
+def getHopBoseGate(index):
+    return Gate(index, abi.QD_OP_HOP, 1)
+
+
+def _is_scalar(x):
+    return isinstance(x, (int, float, complex, np.number))
+
+
+class Operator:
+    """Ordered product of gate instances (operator.hpp:68-132)."""
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+
+    def same(self, other):  # operator== (:123-131): shared_ptr identity
+        return len(self.ops) == len(other.ops) and all(a is b for a, b in zip(self.ops, other.ops))
+
+    def __mul__(self, other):
+        if isinstance(other, Operator):  # :82-91
+            return Operator(self.ops + other.ops)
+        if isinstance(other, OperatorSum):
+            return OperatorSum([(self, 1.0 + 0j)]) * other
+        if _is_scalar(other):  # operator_abel.hpp:147-149
+            return OperatorSum([(self, complex(other))])
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if _is_scalar(other):  # operator_abel.hpp:143-145
+            return OperatorSum([(self, complex(other))])
+        return NotImplemented
+
+    def __add__(self, other):
+        if isinstance(other, Operator):  # :159-163
+            return OperatorSum([(self, 1.0 + 0j)]).append(other, 1.0 + 0j)
+        if isinstance(other, OperatorSum):
+            return OperatorSum([(self, 1.0 + 0j)]) + other
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, Operator):  # :165-169
+            return OperatorSum([(self, 1.0 + 0j)]).append(other, -1.0 + 0j)
+        if isinstance(other, OperatorSum):
+            return OperatorSum([(self, 1.0 + 0j)]) - other
+        return NotImplemented
+
+    def __rshift__(self, sector):  # `op >> index`
+        return OperatorSum([(self, 1.0 + 0j)]), sector
+
+
+class OperatorSum:
+    """Terms with complex coefficients, in construction order (operator_abel.hpp:9-128)."""
+
+    def __init__(self, terms=None):
+        self.terms = [[op, complex(c)] for op, c in (terms or [])]
+
+    def size(self):
+        return len(self.terms)
+
+    def copy(self):
+        return OperatorSum([(op, c) for op, c in self.terms])
+
+    def append(self, op, coeff):  # :115-127, merges only identical gate instances
+        for t in self.terms:
+            if t[0].same(op):
+                t[1] = t[1] + complex(coeff)
+                return self
+        self.terms.append([op, complex(coeff)])
+        return self
+
+    def __add__(self, other):
+        if isinstance(other, Operator):  # :83-87
+            return self.copy().append(other, 1.0 + 0j)
+        if isinstance(other, OperatorSum):  # :89-96, concatenation
+            return OperatorSum([(op, c) for op, c in self.terms] + [(op, c) for op, c in other.terms])
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, Operator):  # :109-113
+            return self.copy().append(other, -1.0 + 0j)
+        if isinstance(other, OperatorSum):  # :98-107
+            return OperatorSum([(op, c) for op, c in self.terms] + [(op, -c) for op, c in other.terms])
+        return NotImplemented
+
+    def __mul__(self, other):
+        if _is_scalar(other):  # :65-71
+            return OperatorSum([(op, c * complex(other)) for op, c in self.terms])
+        if isinstance(other, Operator):  # :51-58
+            return OperatorSum([(op * other, c) for op, c in self.terms])
+        if isinstance(other, OperatorSum):  # :73-81, i-major
+            out = OperatorSum()
+            for op_i, c_i in self.terms:
+                for op_j, c_j in other.terms:
+                    out.append(op_i * op_j, c_i * c_j)
+            return out
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if _is_scalar(other):  # :139-142
+            return self * other
+        return NotImplemented
+
+    def program(self):
+        """Flatten to (terms, ops) ctypes arrays in the reference's order."""
+        nops = sum(len(op.ops) for op, _ in self.terms)
+        terms = (abi.qd_term * max(1, len(self.terms)))()
+        ops = (abi.qd_op * max(1, nops))()
+        pos = 0
+        for k, (op, c) in enumerate(self.terms):
+            if len(op.ops) > abi.QD_MAX_OPS_PER_TERM:
+                raise ValueError(f"term {k} has {len(op.ops)} ops (max {abi.QD_MAX_OPS_PER_TERM})")
+            terms[k].coeff[0] = c.real
+            terms[k].coeff[1] = c.imag
+            terms[k].op_begin = pos
+            terms[k].n_ops = len(op.ops)
+            for rec in op.ops:
+                o = ops[pos]
+                o.kind = rec.kind
+                o.site_a = rec.site_a
+                o.site_b = rec.site_b
+                o.string = rec.string
+                for q in range(4):
+                    u = rec.U[q // 2, q % 2]
+                    o.U[2 * q] = u.real
+                    o.U[2 * q + 1] = u.imag
+                pos += 1
+        return terms, len(self.terms), ops, nops
+
+    def nbits_used(self):
+        m = 0
+        for op, _ in self.terms:
+            for r in op.ops:
+                m = max(m, r.site_a + 1, r.site_b + 1)
+        return m
+
+
+def getEmptyOperator():
+    return OperatorSum()
+
+
+# ---------------------------------------------------------------- models
+# Hamiltonians of the benchmark configurations, written with the API above in
+# the way a reference driver builds them (programs/spin.cpp:23-35,
+# programs/hubbard_diss.cpp:125-159).
+
+def xy_chain(N, J=1.0, pbc=False):
+    """H = J sum_i (S+_{i+1} S-_i + S+_i S-_{i+1}) on N qubits (spin.cpp:23-35)."""
+    idx = getQbits(N)
+    gc, ga = getBoseGate(idx), getBoseGate(idx)
+    gc << 0.5 * (pauli_x + II * pauli_y)
+    ga << 0.5 * (pauli_x - II * pauli_y)
+    ham = getEmptyOperator()
+    nb = N if pbc else N - 1
+    for i in range(nb):
+        j = (i + 1) % N
+        if J == 1.0:
+            ham = ham + gc(j) * ga(i) + gc(i) * ga(j)
+        else:
+            ham = ham + J * (gc(j) * ga(i) + gc(i) * ga(j))
+    return idx, ham
+
+
+def xxz_chain(N, Jxy=1.0, Jz=1.0, pbc=True):
+    """H = sum_i [Jxy/2 (S+_{i+1} S-_i + h.c.) + Jz Sz_{i+1} Sz_i] (S = sigma/2)."""
+    idx = getQbits(N)
+    gc, ga, gz = getBoseGate(idx), getBoseGate(idx), getBoseGate(idx)
+    gc << 0.5 * (pauli_x + II * pauli_y)
+    ga << 0.5 * (pauli_x - II * pauli_y)
+    gz << 0.5 * pauli_z
+    ham = getEmptyOperator()
+    nb = N if pbc else N - 1
+    for i in range(nb):
+        j = (i + 1) % N
+        ham = ham + (0.5 * Jxy) * (gc(j) * ga(i) + gc(i) * ga(j)) + Jz * (gz(j) * gz(i))
+    return idx, ham
+
+
+def sz_sector(idx, N, up):
+    return Constrain(idx, getNset(idx) == up)
+
+
+def hubbard_bonds_chain(L, pbc):
+    return [(i, (i + 1) % L) for i in range(L if pbc else L - 1)]
+
+
+def hubbard_bonds_square(Lx, Ly):
+    """Nearest-neighbour bonds of an Lx x Ly periodic square lattice, each once."""
+    bonds = []
+    for y in range(Ly):
+        for x in range(Lx):
+            s = x + Lx * y
+            for dx, dy in ((1, 0), (0, 1)):
+                nx, ny = (x + dx) % Lx, (y + dy) % Ly
+                t = nx + Lx * ny
+                if (Lx == 2 and dx == 1 and x == 1) or (Ly == 2 and dy == 1 and y == 1):
+                    continue  # a 2-site ring has one bond
+                if s != t:
+                    bonds.append((s, t))
+    return bonds
+
+
+def hubbard(nsite, bonds, t=1.0, U=4.0):
+    """H = -t sum_<ij>,s (c+_i c_j + c+_j c_i) + U sum_i n_i,up n_i,dn.
+
+    Up spins on bits 0..nsite-1, down spins on nsite..2 nsite-1, fermionic
+    hop gates (hubbard_diss.cpp:97-159 layout)."""
+    idx = getQbits(2 * nsite)
+    gate = getHopFermiGate(idx)
+    ham = getEmptyOperator()
+    for (a, b) in bonds:
+        for off in (0, nsite):
+            ham = ham + (-t) * gate(a + off, b + off) + (-t) * gate(b + off, a + off)
+    for i in range(nsite):
+        ham = ham + U * (gate(i, i) * gate(i + nsite, i + nsite))
+    return idx, ham
+
+
+def hubbard_sector(idx, nsite, nup, ndn):
+    return Constrain(idx, getNup(idx, nsite) == nup, getNdo(idx, nsite) == ndn)

@@ -1,0 +1,212 @@
+"""Python handles over the C ABI (include/qudrip_hip.h).
+
+The product is the HIP library; these classes only own its handles and hand
+it device pointers.  Device vectors are torch complex128 tensors on the
+context's GPU; the context issues its work on torch's current stream, so
+the two orders agree without extra synchronisation.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Context:
+    def __init__(self, device=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the HIP path has no CPU fallback")
+        self.lib = abi.load()
+        self.device = device
+        torch.cuda.set_device(device)
+        self.torch_stream = torch.cuda.current_stream(device)
+        p = ctypes.c_void_p()
+        abi.check(None, self.lib.qd_ctx_create_on_stream(device, ctypes.c_void_p(self.torch_stream.cuda_stream),
+                                                         ctypes.byref(p)), "qd_ctx_create_on_stream")
+        self.ptr = p
+
+    def check(self, rc, what=""):
+        abi.check(self.ptr, rc, what)
+
+    def sync(self):
+        self.check(self.lib.qd_ctx_sync(self.ptr), "sync")
+
+    def close(self):
+        if self.ptr:
+            self.lib.qd_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def zeros(self, n):
+        return _torch().zeros(n, dtype=_torch().complex128, device=f"cuda:{self.device}")
+
+    def seeded_state(self, n, seed):
+        v = self.zeros(n)
+        self.check(self.lib.qd_seeded_state(self.ptr, n, seed, ctypes.c_void_p(v.data_ptr())), "seeded_state")
+        return v
+
+    def norm2(self, v):
+        out = ctypes.c_double()
+        self.check(self.lib.qd_norm2(self.ptr, v.numel(), ctypes.c_void_p(v.data_ptr()), ctypes.byref(out)),
+                   "norm2")
+        return out.value
+
+
+class Basis:
+    def __init__(self, ctx, sector):
+        self.ctx = ctx
+        arr, n = sector.ctypes_constraints()
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_basis_create(ctx.ptr, sector.nbits, arr, n, ctypes.byref(p)), "basis_create")
+        self.ptr = p
+        self.nbits = sector.nbits
+        self.size = ctx.lib.qd_basis_size(p)
+
+    def states(self):
+        out = np.empty(self.size, dtype=np.uint64)
+        if self.size:
+            self.ctx.check(self.ctx.lib.qd_basis_download(self.ptr, out.ctypes.data), "basis_download")
+        return out
+
+    def rank(self, states_np):
+        torch = _torch()
+        st = torch.from_numpy(np.ascontiguousarray(states_np, dtype=np.uint64).view(np.int64)).to(
+            f"cuda:{self.ctx.device}")
+        out = torch.empty(st.numel(), dtype=torch.int64, device=st.device)
+        self.ctx.check(self.ctx.lib.qd_basis_rank(self.ptr, ctypes.c_void_p(st.data_ptr()), st.numel(),
+                                                  ctypes.c_void_p(out.data_ptr())), "basis_rank")
+        return out.cpu().numpy()
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_basis_destroy(self.ptr)
+            self.ptr = None
+
+
+class TermProgram:
+    def __init__(self, ctx, opsum, nbits):
+        self.ctx = ctx
+        terms, nt, ops, no = opsum.program()
+        self._keep = (terms, ops)
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_termprog_create(ctx.ptr, nbits, terms, nt, ops, no, ctypes.byref(p)), "termprog")
+        self.ptr = p
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_termprog_destroy(self.ptr)
+            self.ptr = None
+
+
+class CSR:
+    """`ham >> index` on the GPU (reference ColMajor arrays)."""
+
+    def __init__(self, ctx, basis, prog):
+        self.ctx = ctx
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_csr_build(ctx.ptr, basis.ptr, prog.ptr, ctypes.byref(p)), "csr_build")
+        self.ptr = p
+        n, nnz, herm = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        ctx.check(ctx.lib.qd_csr_info(p, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(herm)), "csr_info")
+        self.n, self.nnz, self.hermitian = n.value, nnz.value, herm.value
+
+    def download(self):
+        outer = np.empty(self.n + 1, dtype=np.int32)
+        inner = np.empty(max(self.nnz, 1), dtype=np.int32)
+        vals = np.empty(2 * max(self.nnz, 1), dtype=np.float64)
+        self.ctx.check(self.ctx.lib.qd_csr_download(self.ptr, outer.ctypes.data, inner.ctypes.data,
+                                                    vals.ctypes.data), "csr_download")
+        return outer, inner[:self.nnz], vals[:2 * self.nnz]
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_csr_destroy(self.ptr)
+            self.ptr = None
+
+
+class LinOp:
+    def __init__(self, ctx, ptr, keep=()):
+        self.ctx = ctx
+        self.ptr = ptr
+        self._keep = keep
+        self.n = ctx.lib.qd_linop_dim(ptr)
+
+    @classmethod
+    def from_csr(cls, ctx, csr):
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_linop_from_csr(ctx.ptr, csr.ptr, ctypes.byref(p)), "linop_from_csr")
+        return cls(ctx, p, (csr,))
+
+    @classmethod
+    def matfree(cls, ctx, basis, prog):
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_linop_matfree(ctx.ptr, basis.ptr, prog.ptr, ctypes.byref(p)), "linop_matfree")
+        return cls(ctx, p, (basis, prog))
+
+    def set_lanes(self, lanes):
+        self.ctx.check(self.ctx.lib.qd_linop_set_lanes(self.ptr, lanes), "set_lanes")
+
+    def apply(self, x, y=None):
+        if y is None:
+            y = self.ctx.zeros(self.n)
+        self.ctx.check(self.ctx.lib.qd_spmv(self.ctx.ptr, self.ptr, ctypes.c_void_p(x.data_ptr()),
+                                            ctypes.c_void_p(y.data_ptr())), "spmv")
+        return y
+
+    def expect(self, psi):
+        out = (ctypes.c_double * 2)()
+        self.ctx.check(self.ctx.lib.qd_expect(self.ctx.ptr, self.ptr, ctypes.c_void_p(psi.data_ptr()), out),
+                       "expect")
+        return complex(out[0], out[1])
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_linop_destroy(self.ptr)
+            self.ptr = None
+
+
+class Krylov:
+    """Arnoldi::getSolver (herm=0) / getLanczosSolver (herm=1)."""
+
+    def __init__(self, ctx, n, kry_dim, herm):
+        self.ctx = ctx
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_krylov_create(ctx.ptr, n, kry_dim, 1 if herm else 0, ctypes.byref(p)), "krylov")
+        self.ptr = p
+        self.n = n
+
+    def ground_state(self, op, psi, add_dim=-1):
+        e0 = ctypes.c_double()
+        self.ctx.check(self.ctx.lib.qd_krylov_ground_state(self.ptr, op.ptr, ctypes.c_void_p(psi.data_ptr()),
+                                                           add_dim, ctypes.byref(e0)), "ground_state")
+        return e0.value
+
+    def evolve(self, op, psi_in, psi_out, h, kry_dim_rt=0):
+        self.ctx.check(self.ctx.lib.qd_krylov_evolve(self.ptr, op.ptr, ctypes.c_void_p(psi_in.data_ptr()),
+                                                     ctypes.c_void_p(psi_out.data_ptr()), float(h), kry_dim_rt),
+                       "evolve")
+
+    def info(self):
+        inf = abi.qd_krylov_info()
+        self.ctx.check(self.ctx.lib.qd_krylov_get_info(self.ptr, ctypes.byref(inf)), "info")
+        return {"act_dim": inf.act_dim, "breakdown": inf.breakdown, "error": inf.error,
+                "beta_last": inf.beta_last, "n_spmv": inf.n_spmv}
+
+    def eigenvalues(self, cap=4096):
+        buf = (ctypes.c_double * cap)()
+        cnt = ctypes.c_int()
+        self.ctx.check(self.ctx.lib.qd_krylov_eigenvalues(self.ptr, buf, cap, ctypes.byref(cnt)), "eigenvalues")
+        return np.array(buf[:cnt.value])
+
+    def reset(self):
+        self.ctx.check(self.ctx.lib.qd_krylov_reset(self.ptr), "reset")
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_krylov_destroy(self.ptr)
+            self.ptr = None

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One library context for the whole GPU session (one process, one GPU)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from qudrip_amd.runtime import Context
+    c = Context(0)
+    yield c
+    c.close()

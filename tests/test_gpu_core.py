@@ -1,0 +1,320 @@
+"""GPU parity of the core path through the C ABI against the CPU oracle:
+basis (bit-exact), CSR build (bit-exact), products, Lanczos, Arnoldi."""
+import numpy as np
+import pytest
+
+import oracle
+from qudrip_amd import operators as Q
+from qudrip_amd.runtime import CSR, Basis, Krylov, LinOp, TermProgram
+from tests import physics as P
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def xxz(N, Jxy=1.0, Jz=1.0, pbc=True):
+    idx, ham = Q.xxz_chain(N, Jxy, Jz, pbc)
+    return idx, ham, Q.Constrain(idx, Q.getNset(idx) == N // 2)
+
+
+def hub(nsite, bonds, nup, ndn, t=1.0, U=4.0):
+    idx, ham = Q.hubbard(nsite, bonds, t, U)
+    return idx, ham, Q.hubbard_sector(idx, nsite, nup, ndn)
+
+
+def to_dev(ctx, x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex128)).to(f"cuda:{ctx.device}")
+
+
+# ------------------------------------------------------------------ basis
+@pytest.mark.parametrize("case", ["xy16", "xxz20", "xxz24", "hub3x4_66", "hub8_44", "hub8_31", "free_hi_bits"])
+def test_basis_bit_exact(ctx, case):
+    if case == "xy16":
+        idx = Q.getQbits(16); sec = Q.Constrain(idx, Q.getNset(idx) == 8)
+    elif case == "xxz20":
+        idx = Q.getQbits(20); sec = Q.Constrain(idx, Q.getNset(idx) == 10)
+    elif case == "xxz24":
+        idx = Q.getQbits(24); sec = Q.Constrain(idx, Q.getNset(idx) == 12)
+    elif case == "hub3x4_66":
+        idx = Q.getQbits(24); sec = Q.hubbard_sector(idx, 12, 6, 6)
+    elif case == "hub8_44":
+        idx = Q.getQbits(16); sec = Q.hubbard_sector(idx, 8, 4, 4)
+    elif case == "hub8_31":
+        idx = Q.getQbits(16); sec = Q.hubbard_sector(idx, 8, 3, 1)
+    else:  # constraint on the low bits only: the high bits are free
+        idx = Q.getQbits(14); sec = Q.Constrain(idx, Q.getNup(idx, 6) == 2)
+    ref = oracle.basis_scan(sec)
+    b = Basis(ctx, sec)
+    try:
+        assert b.size == len(ref)
+        assert np.array_equal(b.states(), ref)
+        sample = ref[:: max(1, len(ref) // 5000)]
+        assert np.array_equal(b.rank(sample), np.arange(len(ref))[:: max(1, len(ref) // 5000)])
+    finally:
+        b.close()
+
+
+def test_basis_rank_miss_and_general_path(ctx):
+    idx = Q.getQbits(12)
+    # overlapping masks (total count + up count) -> predicate-scan path
+    sec = Q.Constrain(idx, Q.getNset(idx) == 6, Q.getNup(idx, 6) == 2)
+    ref = oracle.basis_scan(sec)
+    b = Basis(ctx, sec)
+    try:
+        assert np.array_equal(b.states(), ref)
+        out = np.array([0b111, int(ref[5]), (1 << 12) | int(ref[0])], dtype=np.uint64)
+        r = b.rank(out)
+        assert r[0] == len(ref) and r[1] == 5 and r[2] == len(ref)
+    finally:
+        b.close()
+
+
+def test_basis_empty_sector(ctx):
+    idx = Q.getQbits(6)
+    b = Basis(ctx, Q.Constrain(idx, Q.getNset(idx) == 9))
+    assert b.size == 0
+    b.close()
+
+
+# ------------------------------------------------------------------ CSR
+def _build(ctx, ham, sec):
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, sec.nbits)
+    c = CSR(ctx, b, p)
+    return b, p, c
+
+
+def _close(*hs):
+    for h in hs:
+        h.close()
+
+
+CSR_CASES = {
+    "xy16_obc": lambda: (lambda i, h: (h, Q.Constrain(i, Q.getNset(i) == 8)))(*Q.xy_chain(16, pbc=False)),
+    "xxz16_pbc": lambda: xxz(16)[1:],
+    "xxz20_pbc": lambda: xxz(20)[1:],
+    "xxz18_random_coeffs": lambda: xxz(18, 0.9317, 0.3711)[1:],
+    "hub8_obc_44": lambda: hub(8, Q.hubbard_bonds_chain(8, False), 4, 4)[1:],
+    "hub8_obc_random_U": lambda: hub(8, Q.hubbard_bonds_chain(8, False), 3, 2, t=0.77, U=5.123)[1:],
+    "hub3x4_33": lambda: hub(12, Q.hubbard_bonds_square(3, 4), 3, 3)[1:],
+}
+
+
+@pytest.mark.parametrize("case", list(CSR_CASES))
+def test_csr_bit_exact(ctx, case):
+    ham, sec = CSR_CASES[case]()
+    ref_basis = oracle.basis_scan(sec)
+    r_outer, r_inner, r_vals = oracle.build_csc(ref_basis, ham)
+    b, p, c = _build(ctx, ham, sec)
+    try:
+        outer, inner, vals = c.download()
+        assert c.nnz == len(r_inner)
+        assert np.array_equal(outer, r_outer)
+        assert np.array_equal(inner, r_inner)
+        assert np.array_equal(bits(vals), bits(r_vals))  # bit-identical, signed zeros included
+        assert c.hermitian == 1
+    finally:
+        _close(c, p, b)
+
+
+@pytest.mark.parametrize("N,nnz", [(22, 8834696), (24, 36564892)])
+def test_csr_xxz_full_size_nnz_and_symmetry(ctx, N, nnz):
+    _, ham, sec = xxz(N)
+    b, p, c = _build(ctx, ham, sec)
+    try:
+        assert c.nnz == nnz and c.hermitian == 1
+        outer, inner, vals = c.download()
+        v = vals.view(np.complex128)
+        assert np.all(v.imag == 0)
+        # values are Jxy/2 = 0.5 off the diagonal, multiples of 0.25 on it
+        assert set(np.unique(np.round(v.real * 4)).astype(int)) <= set(range(-24, 25))
+        d = np.diff(inner.astype(np.int64))
+        boundary = np.zeros(len(d), dtype=bool)
+        st = outer[1:-1].astype(np.int64)
+        st = st[(st > 0) & (st < len(inner))]
+        boundary[st - 1] = True
+        assert np.all((d > 0) | boundary)  # rows ascending inside every column
+    finally:
+        _close(c, p, b)
+
+
+def test_csr_hub3x4_66_nnz(ctx):
+    ham, sec = hub(12, Q.hubbard_bonds_square(3, 4), 6, 6)[1:]
+    b, p, c = _build(ctx, ham, sec)
+    try:
+        assert b.size == 853776 and c.nnz == 23206260
+    finally:
+        _close(c, p, b)
+
+
+def test_csr_non_hermitian_operator(ctx):
+    """A lone hop c+_3 c_1 (not Hermitian): exact CSC and the adjoint-built product."""
+    idx = Q.getQbits(8)
+    g = Q.getHopFermiGate(idx)
+    ham = Q.getEmptyOperator() + (0.5 + 0.25j) * g(3, 1) + 0.3 * (g(5, 5) * g(2, 2))
+    sec = Q.Constrain(idx, Q.getNset(idx) == 4)
+    ref_basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(ref_basis, ham)
+    b, p, c = _build(ctx, ham, sec)
+    try:
+        outer, inner, vals = c.download()
+        assert np.array_equal(outer, ref[0]) and np.array_equal(inner, ref[1])
+        assert np.array_equal(bits(vals), bits(ref[2]))
+        assert c.hermitian == 0
+        op = LinOp.from_csr(ctx, c)
+        x = oracle.seeded_state(len(ref_basis), 9)
+        y = op.apply(to_dev(ctx, x)).cpu().numpy()
+        assert np.allclose(y, oracle.spmv(ref, x), atol=1e-14)
+        op.close()
+    finally:
+        _close(c, p, b)
+
+
+# ------------------------------------------------------------------ products
+@pytest.mark.parametrize("case", ["xxz20_pbc", "xxz18_random_coeffs", "hub3x4_33"])
+def test_spmv_matches_oracle(ctx, case):
+    ham, sec = CSR_CASES[case]()
+    ref_basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(ref_basis, ham)
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    mf = LinOp.matfree(ctx, b, p)
+    try:
+        x = oracle.seeded_state(len(ref_basis), 21)
+        yr = oracle.spmv(ref, x)
+        xd = to_dev(ctx, x)
+        y = op.apply(xd).cpu().numpy()
+        ym = mf.apply(xd).cpu().numpy()
+        scale = np.linalg.norm(yr)
+        assert np.linalg.norm(y - yr) / scale < 1e-14
+        assert np.linalg.norm(ym - yr) / scale < 1e-14
+        e = op.expect(xd)
+        er = oracle.eval_expect(ref, x)
+        assert abs(e - er) / abs(er) < 1e-13
+        for lanes in (2, 4, 8, 32, 64):
+            op.set_lanes(lanes)
+            assert np.linalg.norm(op.apply(xd).cpu().numpy() - yr) / scale < 1e-14
+    finally:
+        _close(mf, op, c, p, b)
+
+
+def test_matfree_matches_map_acc(ctx):
+    """Lazy operator vs the reference's push-form map_acc (operator.hpp:215-232)."""
+    ham, sec = hub(8, Q.hubbard_bonds_chain(8, False), 4, 4)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, sec.nbits)
+    mf = LinOp.matfree(ctx, b, p)
+    try:
+        x = oracle.seeded_state(len(ref_basis), 13)
+        to = np.zeros(len(ref_basis), dtype=np.complex128)
+        oracle.map_acc(ref_basis, ham, x, to)
+        y = mf.apply(to_dev(ctx, x)).cpu().numpy()
+        assert np.linalg.norm(y - to) / np.linalg.norm(to) < 1e-14
+    finally:
+        _close(mf, p, b)
+
+
+def test_seeded_state_matches_oracle(ctx):
+    v = ctx.seeded_state(100003, 2).cpu().numpy()
+    r = oracle.seeded_state(100003, 2)
+    assert np.max(np.abs(v - r)) < 1e-15
+    assert abs(ctx.norm2(to_dev(ctx, r)) - 1.0) < 1e-14
+
+
+# ------------------------------------------------------------------ solvers
+def test_lanczos_xy16_closed_form(ctx):
+    """spin.cpp sequence: XY16 OBC, N_up = 8, k = 100."""
+    ham, sec = CSR_CASES["xy16_obc"]()
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    kr = Krylov(ctx, b.size, 100, herm=True)
+    try:
+        psi = to_dev(ctx, oracle.seeded_state(b.size, 1))
+        e0 = kr.ground_state(op, psi)
+        e_ref = P.xy_free_fermion_e0(16, 8)
+        assert abs(e0 - e_ref) / abs(e_ref) < 1e-10
+        ev = op.expect(psi)
+        nrm = ctx.norm2(psi)
+        assert abs(ev.real / nrm - e_ref) / abs(e_ref) < 1e-10
+        assert abs(nrm - 1.0) < 1e-10  # full reorthogonalisation keeps V orthonormal
+        inf = kr.info()
+        assert inf["act_dim"] == 100 and inf["breakdown"] == 0
+    finally:
+        _close(kr, op, c, p, b)
+
+
+def test_lanczos_matches_oracle_and_dense(ctx):
+    ham, sec = xxz(12, 1.0, 0.6)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(ref_basis, ham)
+    w = np.linalg.eigvalsh(oracle.csc_to_dense(ref))
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    kr = Krylov(ctx, b.size, 120, herm=True)
+    try:
+        psi0 = oracle.seeded_state(b.size, 2)
+        r = oracle.lanczos_gs(ref, 120, psi0)
+        psi = to_dev(ctx, psi0)
+        e0 = kr.ground_state(op, psi)
+        assert abs(e0 - w[0]) / abs(w[0]) < 1e-10
+        assert abs(e0 - r["E0"]) / abs(w[0]) < 1e-10
+        g = psi.cpu().numpy()
+        ov = abs(np.vdot(g, r["psi"])) / (np.linalg.norm(g) * np.linalg.norm(r["psi"]))
+        assert ov > 1 - 1e-9
+    finally:
+        _close(kr, op, c, p, b)
+
+
+def test_lanczos_incremental_and_breakdown(ctx):
+    ham, sec = xxz(10, 1.0, 1.0)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    H = oracle.csc_to_dense(oracle.build_csc(ref_basis, ham))
+    w, z = np.linalg.eigh(H)
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    kr = Krylov(ctx, b.size, 60, herm=True)
+    try:
+        psi = to_dev(ctx, oracle.seeded_state(b.size, 3))
+        e_a = kr.ground_state(op, psi, 20)  # getGroundState(H, psi, 0, 20)
+        assert kr.info()["act_dim"] == 20
+        e_b = kr.ground_state(op, psi, 40)
+        assert kr.info()["act_dim"] == 60
+        assert e_b <= e_a + 1e-12 and abs(e_b - w[0]) < 1e-9 * abs(w[0])
+        kr.reset()
+        psi2 = to_dev(ctx, z[:, 2])  # eigenvector start: breakdown at the first step
+        e_c = kr.ground_state(op, psi2)
+        inf = kr.info()
+        assert inf["breakdown"] == 1 and inf["act_dim"] == 1
+        assert abs(e_c - w[2]) < 1e-10
+    finally:
+        _close(kr, op, c, p, b)
+
+
+def test_arnoldi_evolve_matches_oracle(ctx):
+    ham, sec = hub(8, Q.hubbard_bonds_chain(8, False), 3, 3, U=2.0)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(ref_basis, ham)
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    mf = LinOp.matfree(ctx, b, p)
+    kr = Krylov(ctx, b.size, 30, herm=False)
+    try:
+        psi0 = oracle.seeded_state(b.size, 4)
+        r = oracle.arnoldi_evolve(ref, 30, 0.05, psi0)
+        xin = to_dev(ctx, psi0)
+        out = ctx.zeros(b.size)
+        kr.evolve(op, xin, out, 0.05, 30)
+        assert np.max(np.abs(out.cpu().numpy() - r)) < 1e-11
+        kr.evolve(mf, xin, out, 0.05, 30)
+        assert np.max(np.abs(out.cpu().numpy() - r)) < 1e-11
+        # forward then backward returns to the start; the norm is kept
+        back = ctx.zeros(b.size)
+        kr.evolve(op, out, back, -0.05, 30)
+        assert np.max(np.abs(back.cpu().numpy() - psi0)) < 1e-11
+    finally:
+        _close(kr, mf, op, c, p, b)
