@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 E_BETHE_XXX24 = -10.67001451653724  # Bethe ansatz, N=24 PBC (S=1/2 XXX)
+# HBM bytes per launch of the default product kernel on this workload, from
+# two rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from
+# a plain 1 GiB copy); commands and raw counts in
+# profiles/spmv_xxz24_hbm_traffic.md.
+SPMV_XXZ24_TRAFFIC_BYTES = 1003.0e6
 
 
 def main():
@@ -36,6 +41,9 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -44,6 +52,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.calib_copy_mb:
+        # plain copy kernel of known size: reads and writes calib_copy_mb MiB each
+        a = torch.ones(args.calib_copy_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.numel() * 8}), flush=True)
+        return
 
     from qudrip_amd import operators as Q
     from qudrip_amd.runtime import CSR, Basis, Context, Krylov, LinOp, TermProgram
@@ -61,6 +79,8 @@ def main():
     build_s = time.perf_counter() - t0
     n, nnz = csr.n, csr.nnz
     op = LinOp.from_csr(ctx, csr)
+    if args.lanes is not None:
+        op.set_lanes(args.lanes)
     B = 20 * nnz + 4 * (n + 1) + 32 * n
 
     x = ctx.seeded_state(n, 2)
@@ -99,9 +119,11 @@ def main():
         "dtype": "c128",
         "data": "synthetic (seeded splitmix64 start vector, seed 2)",
         "config": {"workload": f"XXZ chain N={N} PBC Jxy=Jz=1 Sz=0, complex128 CSR SpMV in Lanczos",
-                   "n": n, "nnz": nnz, "bytes_per_spmv": B, "lanes_per_row": None},
+                   "n": n, "nnz": nnz, "bytes_per_spmv": B,
+                   "spmv_row_mapping": op.lanes()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_XXZ24_TRAFFIC_BYTES if (N == 24 and op.lanes() == 0) else None},
         "csr_build_ms": round(build_s * 1e3, 2),
     }
 

@@ -166,9 +166,17 @@ int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);
 int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out);
 int qd_linop_destroy(qd_linop* op);
 int64_t qd_linop_dim(const qd_linop* op);
-/* Lanes per row of the stored-matrix product (2, 4, ..., 64); chosen from
- * nnz/n at creation, overridable by a tuner. */
+/* Row mapping of the stored-matrix product: 0, -1, ..., -8 select a
+ * CSR-stream variant (a wave streams 64 rows' nonzeros through LDS; the
+ * variants differ in LDS chunk, loads in flight and cache hint), 2..64 the
+ * lanes per row of the vector kernel.  Chosen from nnz/n at creation. */
 int qd_linop_set_lanes(qd_linop* op, int lanes);
+int qd_linop_get_lanes(const qd_linop* op);
+/* Tile autotuner: times every row mapping on d_x -> d_y (reps launches
+ * each, HIP events on the context stream) and keeps the fastest; reports it
+ * and its time per launch in microseconds (host pointers, may be NULL). */
+int qd_linop_autotune(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y, int reps, int* best,
+                      float* best_us);
 
 /* ------------------------------------------------------- vector kernels */
 /* d_y = H d_x (complex128, n entries). */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "qd_linop_destroy": (I, [VP]),
     "qd_linop_dim": (I64, [VP]),
     "qd_linop_set_lanes": (I, [VP, I]),
+    "qd_linop_get_lanes": (I, [VP]),
+    "qd_linop_autotune": (I, [VP, VP, VP, VP, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_float)]),
     "qd_spmv": (I, [VP, VP, VP, VP]),
     "qd_expect": (I, [VP, VP, VP, DP]),
     "qd_norm2": (I, [VP, I64, VP, DP]),

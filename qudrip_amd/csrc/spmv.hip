@@ -10,6 +10,7 @@
 #include "qd_internal.hpp"
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+#define QD_STREAM_VARIANTS 9
 
 // ---------------------------------------------------------------- K5
 // G lanes cooperate on one row: each lane strides the row's entries by G,
@@ -55,6 +56,90 @@ __global__ void __launch_bounds__(256)
     sy += __shfl_xor(sy, o, G);
   }
   if (l == 0 && row < nrows) {
+    double s = xscale ? xscale[xscale_idx] : 1.0;
+    cplx r = cmake(sx * s, sy * s);
+    if (diag || shift.x != 0.0 || shift.y != 0.0) {
+      cplx xr = x[row];
+      cplx d = diag ? cadd(diag[row], shift) : shift;
+      r = cadd(r, crmul(s, cmul(d, xr)));
+    }
+    y[row] = r;
+  }
+}
+
+// ---------------------------------------------------------------- K5 stream
+// CSR-stream form for short rows.  A wave owns 64 consecutive rows.  Phase
+// 1: the 64 lanes stream the wave's nonzeros (contiguous in memory) with U
+// independent loads in flight per lane, gather x, and park conj(v) * x in
+// the wave's LDS slice.  Phase 2: lane l adds the products of row r0 + l in
+// entry order.  Rows longer than the slice are handled chunk by chunk.  The
+// summation order depends only on the matrix, never on the launch shape of
+// other work.
+template <int CAPW, int U, bool NT>
+__global__ void __launch_bounds__(256)
+    spmv_stream_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                       const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
+                       const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
+                       cplx shift) {
+  __shared__ cplx lds[4][CAPW];
+  const unsigned nb = gridDim.x;
+  const unsigned per = nb >> 3;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)lb * 4 + w) * 64;
+  if (r0 >= nrows) return;  // whole wave leaves together
+  const int64_t row = r0 + lane;
+  const int32_t nnz_end = outer[nrows];
+  int32_t rs = (row < nrows) ? outer[row] : nnz_end;
+  int32_t re = __shfl_down(rs, 1, 64);
+  if (lane == 63) re = (r0 + 64 < nrows) ? outer[r0 + 64] : nnz_end;
+  const int32_t wbeg = __shfl(rs, 0, 64);
+  const int32_t wend = __shfl(re, 63, 64);
+  cplx* __restrict__ buf = lds[w];
+  const dbl2* vp = reinterpret_cast<const dbl2*>(vals);
+  double sx = 0.0, sy = 0.0;
+  for (int32_t c0 = wbeg; c0 < wend; c0 += CAPW) {
+    const int32_t cn = min(wend - c0, (int32_t)CAPW);
+    for (int32_t k = lane; k < cn; k += 64 * U) {
+      dbl2 v[U];
+      int32_t c[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int32_t kk = k + 64 * u;
+        if (kk < cn) {
+          if (NT) {
+            v[u] = __builtin_nontemporal_load(vp + c0 + kk);
+            c[u] = __builtin_nontemporal_load(inner + c0 + kk);
+          } else {
+            v[u] = vp[c0 + kk];
+            c[u] = inner[c0 + kk];
+          }
+        }
+      }
+      cplx xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k + 64 * u < cn) xv[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int32_t kk = k + 64 * u;
+        if (kk < cn) buf[kk] = cmake(v[u].x * xv[u].x + v[u].y * xv[u].y, v[u].x * xv[u].y - v[u].y * xv[u].x);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int32_t a = max(rs, c0), b = min(re, c0 + cn);
+    for (int32_t e = a; e < b; ++e) {
+      cplx p = buf[e - c0];
+      sx += p.x;
+      sy += p.y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (row < nrows) {
     double s = xscale ? xscale[xscale_idx] : 1.0;
     cplx r = cmake(sx * s, sy * s);
     if (diag || shift.x != 0.0 || shift.y != 0.0) {
@@ -124,7 +209,29 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
   qd_ctx* ctx = op->ctx;
   int64_t n = op->n;
   if (n == 0) return QD_OK;
-  if (op->kind == QD_LINOP_CSR) {
+  if (op->kind == QD_LINOP_CSR && op->lanes_per_row <= 0) {
+    // CSR-stream: 4 waves x 64 rows per block; variant = -lanes_per_row
+    unsigned grid = grid8((n + 255) / 256);
+    switch (-op->lanes_per_row) {
+#define QD_STREAM_CASE(V, CAPW, U, NT)                                                               \
+  case V:                                                                                           \
+    spmv_stream_kernel<CAPW, U, NT><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y, \
+                                                                   d_xscale, xscale_idx, d_diag, shift);   \
+    break;
+      QD_STREAM_CASE(0, 512, 8, true)
+      QD_STREAM_CASE(1, 512, 4, true)
+      QD_STREAM_CASE(2, 256, 4, true)
+      QD_STREAM_CASE(3, 256, 8, true)
+      QD_STREAM_CASE(4, 1024, 8, true)
+      QD_STREAM_CASE(5, 512, 4, false)
+      QD_STREAM_CASE(6, 512, 8, false)
+      QD_STREAM_CASE(7, 128, 4, true)
+      QD_STREAM_CASE(8, 1024, 16, true)
+#undef QD_STREAM_CASE
+      default:
+        return qd_set_error(ctx, QD_E_ARG, "stream variant %d", -op->lanes_per_row);
+    }
+  } else if (op->kind == QD_LINOP_CSR) {
     switch (op->lanes_per_row) {
 #define QD_SPMV_CASE(GG)                                                                                    \
   case GG:                                                                                                  \
@@ -153,6 +260,7 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
 
 static int pick_lanes(int64_t n, int64_t nnz) {
   double avg = n ? (double)nnz / (double)n : 1.0;
+  if (avg <= 48.0) return 0;  // CSR-stream kernel
   int g = 2;
   while (g < 64 && g < avg) g <<= 1;
   return g;
@@ -208,11 +316,14 @@ extern "C" int qd_linop_destroy(qd_linop* op) {
 
 extern "C" int64_t qd_linop_dim(const qd_linop* op) { return op ? op->n : -1; }
 
-// Override the lanes-per-row choice of a stored-matrix operator (2..64, a
-// power of two); used by the tile autotuner.
+// Override the row mapping of a stored-matrix operator: 0, -1, ..., -8
+// select a CSR-stream variant (LDS chunk, loads in flight, cache hint),
+// 2..64 (powers of two) the vector kernel with that many lanes per row.
 extern "C" int qd_linop_set_lanes(qd_linop* op, int lanes) {
   if (!op || op->kind != QD_LINOP_CSR) return QD_E_ARG;
-  if (lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return QD_E_ARG;
+  if (!(lanes <= 0 && lanes >= -QD_STREAM_VARIANTS + 1) && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 &&
+      lanes != 32 && lanes != 64)
+    return QD_E_ARG;
   op->lanes_per_row = lanes;
   return QD_OK;
 }
@@ -364,3 +475,50 @@ extern "C" int qd_seeded_state(qd_ctx* ctx, int64_t n, uint64_t seed, double* d_
   QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QD_OK;
 }
+
+// ---------------------------------------------------------------- tuner
+// Time every row mapping of a stored-matrix operator on (x -> y) with HIP
+// events on the context stream and keep the fastest.  The results of all
+// mappings agree to rounding (each has a fixed summation order).
+extern "C" int qd_linop_autotune(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y, int reps,
+                                 int* best_out, float* best_us) {
+  if (!ctx || !op || !d_x || !d_y || reps < 1) return qd_set_error(ctx, QD_E_ARG, "qd_linop_autotune: bad arguments");
+  if (op->kind != QD_LINOP_CSR) {
+    if (best_out) *best_out = 0;
+    return QD_OK;
+  }
+  std::vector<int> cands;
+  for (int v = 0; v < QD_STREAM_VARIANTS; ++v) cands.push_back(-v);
+  for (int g = 4; g <= 64; g <<= 1) cands.push_back(g);
+  hipEvent_t e0, e1;
+  QD_HIP(ctx, hipEventCreate(&e0));
+  QD_HIP(ctx, hipEventCreate(&e1));
+  int keep = op->lanes_per_row, best = keep;
+  float best_ms = 1e30f;
+  for (int c : cands) {
+    op->lanes_per_row = c;
+    int rc = qd_linop_apply(op, (const cplx*)d_x, (cplx*)d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    if (rc) {
+      op->lanes_per_row = keep;
+      return rc;
+    }
+    hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; ++r) qd_linop_apply(op, (const cplx*)d_x, (cplx*)d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    hipEventRecord(e1, ctx->stream);
+    QD_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  op->lanes_per_row = best;
+  if (best_out) *best_out = best;
+  if (best_us) *best_us = best_ms * 1000.f / reps;
+  return QD_OK;
+}
+
+extern "C" int qd_linop_get_lanes(const qd_linop* op) { return op ? op->lanes_per_row : 0; }

@@ -151,6 +151,17 @@ class LinOp:
     def set_lanes(self, lanes):
         self.ctx.check(self.ctx.lib.qd_linop_set_lanes(self.ptr, lanes), "set_lanes")
 
+    def lanes(self):
+        return self.ctx.lib.qd_linop_get_lanes(self.ptr)
+
+    def autotune(self, x, y, reps=20):
+        best = ctypes.c_int()
+        us = ctypes.c_float()
+        self.ctx.check(self.ctx.lib.qd_linop_autotune(self.ctx.ptr, self.ptr, ctypes.c_void_p(x.data_ptr()),
+                                                      ctypes.c_void_p(y.data_ptr()), reps, ctypes.byref(best),
+                                                      ctypes.byref(us)), "autotune")
+        return best.value, us.value
+
     def apply(self, x, y=None):
         if y is None:
             y = self.ctx.zeros(self.n)
