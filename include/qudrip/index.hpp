@@ -1,0 +1,153 @@
+// qudrip/index.hpp -- Hilbert-space indices and conserved quantities.
+//
+// Same call surface as the reference:
+//   getQbits(N)                         index.hpp:397 (QbitsIndex :177-191)
+//   getNset / getNup / getNdo, `cq = v`  conservation.hpp:89-115, 233-246
+//   Constrain(idx, cq = v, ...)          conservation.hpp:212-231 -> SubIndex
+//   cursor protocol idx[i], ++idx, (idxv_type)idx, idx.range()
+// The sector table is produced on the GPU (qd_basis_create); the host keeps
+// a lazily downloaded copy only for cursor-style element access.
+#pragma once
+
+#include <algorithm>
+#include <bitset>
+#include <cstdint>
+#include <iostream>
+#include <memory>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace qudrip {
+
+using idx_size_t = unsigned long long;
+using idxv_type = unsigned long long;
+static const double eps = 1e-10;  // qudrip.hpp:18
+constexpr short BIT_LIMIT = 64;
+using bits_type = std::bitset<BIT_LIMIT>;
+
+template <class T = idx_size_t>
+class QbitsIndex {
+  int nbits_;
+  T range_;
+  T idx_value_ = 0;
+
+ public:
+  struct is_index {
+    using type = int;
+  };
+  explicit QbitsIndex(T N) : nbits_((int)N), range_(T(1) << N) {}
+  int nbits() const { return nbits_; }
+  T range() const { return range_; }
+  operator T() const { return idx_value_; }
+  void operator[](T v) { idx_value_ = v; }
+  QbitsIndex& operator++() {
+    ++idx_value_;
+    return *this;
+  }
+};
+
+inline QbitsIndex<> getQbits(int N) { return QbitsIndex<>(N); }
+
+// popcount(x & mask) == value  (Nset, conservation.hpp:89-115)
+template <class T1 = idx_size_t>
+class Nset {
+  uint64_t mask_ = 0;
+  long long val_ = 0;
+
+ public:
+  using quant_type = T1;
+  struct is_cq {};
+  Nset(const QbitsIndex<>& index, int start = -1, int end = -1) {
+    if (start == -1) start = 0;
+    if (end == -1) end = (int)bits_type(index.range() - 1).count();  // N: masks N+1 bits (:112)
+    for (int i = start; i <= end && i < 64; ++i) mask_ |= (1ull << i);
+  }
+  Nset& operator=(quant_type v) {  // `cq = value` (:78-81)
+    val_ = (long long)v;
+    return *this;
+  }
+  uint64_t mask() const { return mask_; }
+  long long val() const { return val_; }
+};
+
+inline Nset<> getNset(const QbitsIndex<>& index) { return Nset<>(index, -1, -1); }
+inline Nset<> getNup(const QbitsIndex<>& index, int Nsite) { return Nset<>(index, 0, Nsite - 1); }
+inline Nset<> getNdo(const QbitsIndex<>& index, int Nsite) { return Nset<>(index, Nsite, 2 * Nsite - 1); }
+
+namespace detail {
+// device sector table shared by every object built on it
+struct BasisHolder {
+  qd_basis* b = nullptr;
+  int nbits = 0;
+  std::vector<uint64_t> host;  // lazily downloaded mapping
+  ~BasisHolder() {
+    if (b) qd_basis_destroy(b);
+  }
+  const std::vector<uint64_t>& mapping() {
+    if (host.empty() && qd_basis_size(b) > 0) {
+      host.resize((size_t)qd_basis_size(b));
+      check(qd_basis_download(b, host.data()), "qd_basis_download");
+    }
+    return host;
+  }
+};
+inline std::shared_ptr<BasisHolder> make_basis(int nbits, const std::vector<qd_constraint>& cons) {
+  auto h = std::make_shared<BasisHolder>();
+  h->nbits = nbits;
+  check(qd_basis_create(ctx(), nbits, cons.data(), (int)cons.size(), &h->b), "qd_basis_create");
+  return h;
+}
+}  // namespace detail
+
+// Constrained index (SubIndex, index.hpp:284-358): ascending sector table,
+// unrank via idx[i], rank via (idxv_type)idx, a miss ranks to range().
+template <class T = QbitsIndex<>&>
+class SubIndex {
+  QbitsIndex<>& earless_;
+  std::shared_ptr<detail::BasisHolder> basis_;
+  idxv_type idx_value_ = 0;
+
+ public:
+  struct is_index {
+    using type = int;
+  };
+  SubIndex(QbitsIndex<>& idx, std::shared_ptr<detail::BasisHolder> b) : earless_(idx), basis_(std::move(b)) {}
+
+  idxv_type range() const { return (idxv_type)qd_basis_size(basis_->b); }
+  QbitsIndex<>& earless() const { return earless_; }
+  const std::shared_ptr<detail::BasisHolder>& basis() const { return basis_; }
+  int nbits() const { return basis_->nbits; }
+
+  operator idxv_type() const {
+    const auto& m = basis_->mapping();
+    auto v = (uint64_t)(idxv_type)earless_;
+    auto it = std::lower_bound(m.begin(), m.end(), v);
+    if (it == m.end() || *it != v) return range();
+    return (idxv_type)(it - m.begin());
+  }
+  void operator[](idxv_type i) {
+    idx_value_ = i;
+    const auto& m = basis_->mapping();
+    if (i < m.size())
+      earless_[m[i]];
+    else
+      earless_[earless_.range()];
+  }
+  SubIndex& operator++() {
+    (*this)[idx_value_ + 1];
+    return *this;
+  }
+};
+
+template <class... CQs>
+SubIndex<> Constrain(QbitsIndex<>& index, CQs&&... cqs) {
+  std::cout << "Constrain: original range " << index.range() << std::endl;  // :214
+  std::vector<qd_constraint> cons;
+  (cons.push_back(qd_constraint{cqs.mask(), (int64_t)cqs.val()}), ...);
+  SubIndex<> sub(index, detail::make_basis(index.nbits(), cons));
+  sub[0];  // :223
+  return sub;
+}
+
+}  // namespace qudrip

@@ -181,6 +181,10 @@ int qd_linop_autotune(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y,
 /* ------------------------------------------------------- vector kernels */
 /* d_y = H d_x (complex128, n entries). */
 int qd_spmv(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y);
+/* d_y += alpha * H d_x, alpha host[2] (OperatorSum::map_acc,
+ * operator_abel.hpp:36-42; with a matrix-free operator this is the lazy
+ * apply of operator.hpp:215-232). */
+int qd_spmv_acc(qd_ctx* ctx, qd_linop* op, const double* h_alpha, const double* d_x, double* d_y);
 /* <psi|H|psi>, unnormalised (State::eval, state.hpp:33-44).  out: host[2]. */
 int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* h_out);
 /* Re(psi^H psi) (Hermitian::norm, sparse_algorithm.hpp:7-16). */

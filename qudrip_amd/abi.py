@@ -79,6 +79,7 @@ SIGNATURES = {
     "qd_linop_get_lanes": (I, [VP]),
     "qd_linop_autotune": (I, [VP, VP, VP, VP, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_float)]),
     "qd_spmv": (I, [VP, VP, VP, VP]),
+    "qd_spmv_acc": (I, [VP, VP, DP, VP, VP]),
     "qd_expect": (I, [VP, VP, VP, DP]),
     "qd_norm2": (I, [VP, I64, VP, DP]),
     "qd_seeded_state": (I, [VP, I64, ctypes.c_uint64, VP]),

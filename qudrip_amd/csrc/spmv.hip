@@ -334,6 +334,31 @@ extern "C" int qd_spmv(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y
   return qd_linop_apply(op, (const cplx*)d_x, (cplx*)d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
 }
 
+// d_y += alpha * H d_x  (OperatorSum::map_acc, operator_abel.hpp:36-42)
+__global__ void axpy_kernel(int64_t n, cplx alpha, const cplx* __restrict__ t, cplx* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx a = cmul(alpha, t[i]);
+  cplx v = y[i];
+  y[i] = cmake(v.x + a.x, v.y + a.y);
+}
+
+extern "C" int qd_spmv_acc(qd_ctx* ctx, qd_linop* op, const double* h_alpha, const double* d_x, double* d_y) {
+  if (!ctx || !op || !h_alpha || (op->n && (!d_x || !d_y))) return qd_set_error(ctx, QD_E_ARG, "qd_spmv_acc: null argument");
+  int64_t n = op->n;
+  if (n == 0) return QD_OK;
+  cplx* tmp = nullptr;
+  QD_HIP(ctx, hipMallocAsync((void**)&tmp, sizeof(cplx) * n, ctx->stream));
+  int rc = qd_linop_apply(op, (const cplx*)d_x, tmp, nullptr, 0, nullptr, cmake(0.0, 0.0));
+  if (!rc) {
+    axpy_kernel<<<qd_grid(n, 256), 256, 0, ctx->stream>>>(n, cmake(h_alpha[0], h_alpha[1]), tmp, (cplx*)d_y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = qd_set_error(ctx, QD_E_HIP, "axpy launch: %s", hipGetErrorString(e));
+  }
+  hipFreeAsync(tmp, ctx->stream);
+  return rc;
+}
+
 // ---------------------------------------------------------------- reductions
 // Deterministic dot products: a fixed grid (function of n only) accumulates
 // grid-stride partial sums, a wave butterfly and an LDS pass give one value

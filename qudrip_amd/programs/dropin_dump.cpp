@@ -1,0 +1,83 @@
+// dropin_dump.cpp -- builds a model through the drop-in C++ API and writes
+// its term program and `>>` matrix as raw binary files, so the tests can
+// compare them with the Python mirror and the CPU oracle.
+//   usage: dropin_dump <xy|xxz|hubchain|lazy> <N> <out-prefix> [Jxy Jz | t U]
+// Files: <prefix>.terms (qd_term[]), <prefix>.ops (qd_op[]), <prefix>.outer,
+// <prefix>.inner (int32), <prefix>.vals (complex128), <prefix>.y (H x for the
+// seeded x of seed 7; for "lazy" computed as psi2 = ham * psi1).
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "qudrip/qudrip.hpp"
+
+using namespace qudrip;
+
+template <class T>
+static void dump(const std::string& path, const T* p, size_t n) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(p), sizeof(T) * n);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::cerr << "usage: dropin_dump <xy|xxz|hubchain|lazy> <N> <out-prefix> [a b]" << std::endl;
+    return 2;
+  }
+  std::string model = argv[1];
+  int N = std::stoi(argv[2]);
+  std::string out = argv[3];
+  double a = argc > 4 ? std::stod(argv[4]) : 1.0, b = argc > 5 ? std::stod(argv[5]) : 1.0;
+
+  bool hub = (model == "hubchain" || model == "lazy");
+  auto q = getQbits(hub ? 2 * N : N);
+  OperatorSum ham = getEmptyOperator();
+  if (!hub) {
+    auto gc = getBoseGate(q), ga = getBoseGate(q), gz = getBoseGate(q);
+    gc << 0.5 * (pauli_x + II * pauli_y);
+    ga << 0.5 * (pauli_x - II * pauli_y);
+    gz << 0.5 * pauli_z;
+    if (model == "xy") {
+      for (int i = 0; i + 1 < N; ++i) ham = ham + gc(i + 1) * ga(i) + gc(i) * ga(i + 1);
+    } else {
+      for (int i = 0; i < N; ++i) {
+        int j = (i + 1) % N;
+        ham = ham + (0.5 * a) * (gc(j) * ga(i) + gc(i) * ga(j)) + b * (gz(j) * gz(i));
+      }
+    }
+  } else {
+    auto g = getHopFermiGate(q);
+    for (int i = 0; i + 1 < N; ++i)
+      for (int off : {0, N}) ham = ham + (-a) * g(i + off, i + 1 + off) + (-a) * g(i + 1 + off, i + off);
+    for (int i = 0; i < N; ++i) ham = ham + b * (g(i, i) * g(i + N, i + N));
+  }
+  std::vector<qd_term> terms;
+  std::vector<qd_op> ops;
+  int nb = 0;
+  ham.program(terms, ops, &nb);
+  dump(out + ".terms", terms.data(), terms.size());
+  dump(out + ".ops", ops.data(), ops.size());
+
+  auto nset = getNset(q);
+  auto nup = getNup(q, N);
+  auto ndo = getNdo(q, N);
+  auto sub = hub ? Constrain(q, nup = N / 2, ndo = N / 2) : Constrain(q, nset = N / 2);
+  auto psi = getState(sub, 2);
+  shuffle_seed() = 7;
+  shuffle_state(psi(0));
+  if (model == "lazy") {
+    psi(1) = ham * psi(0);  // lazy product, evaluated matrix-free
+  } else {
+    auto H = ham >> sub;
+    dump(out + ".outer", H.outerIndexPtr(), (size_t)H.rows() + 1);
+    dump(out + ".inner", H.innerIndexPtr(), (size_t)H.nonZeros());
+    dump(out + ".vals", H.valuePtr(), (size_t)H.nonZeros());
+    std::vector<double> y(2 * sub.range());
+    qd_spmv(detail::ctx(), H.linop(), psi.device(0), psi.device(1, true));
+  }
+  dump(out + ".x", psi.host(0, false), sub.range());
+  dump(out + ".y", psi.host(1, false), sub.range());
+  std::cout << "dumped " << terms.size() << " terms, n = " << sub.range() << std::endl;
+  return 0;
+}
