@@ -100,9 +100,11 @@ class State {
 
   // <psi_t|M|psi_t>, unnormalised (state.hpp:33-44)
   value_type eval(int tstp, const SpMatrix& M) const {
-    if ((size_t)M.cols() != ndim_)
+    if ((size_t)M.cols() != ndim_) {
       std::cout << "State: evaluation failure due to dimension mismatch. Range = " << ndim_
                 << " and Matrix col = " << M.cols() << std::endl;
+      throw std::runtime_error("State::eval: dimension mismatch");
+    }
     double out[2] = {0.0, 0.0};
     detail::check(qd_expect(detail::ctx(), M.linop(), device(tstp), out), "qd_expect");
     return value_type(out[0], out[1]);
