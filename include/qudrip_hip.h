@@ -166,6 +166,12 @@ int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);
 int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out);
 int qd_linop_destroy(qd_linop* op);
 int64_t qd_linop_dim(const qd_linop* op);
+/* Matrix-free operators over a two-block sector (up bits low, down bits
+ * high) whose terms are hops inside the low block, ops inside the high block
+ * or diagonal are applied as H = T_A (x) 1 + 1 (x) T_B + D with small block
+ * matrices; on = 0 forces the generic per-row kernel.  *has_structured
+ * (may be NULL) reports whether the operator has that form. */
+int qd_linop_set_structured(qd_linop* op, int on, int* has_structured);
 /* Row mapping of the stored-matrix product: 0, -1, ..., -8 select a
  * CSR-stream variant (a wave streams 64 rows' nonzeros through LDS; the
  * variants differ in LDS chunk, loads in flight and cache hint), 2..64 the
@@ -229,6 +235,9 @@ int qd_krylov_eigenvalues(qd_krylov* kr, double* h_evals, int cap, int* count);
 int qd_krylov_converge_ev(qd_krylov* kr, int N, double iv_re, double iv_im);
 /* Forget the held basis (act_dim = 0). */
 int qd_krylov_reset(qd_krylov* kr);
+/* Reorthogonalisation scheme: 1 = delayed CGS2 (two passes over the basis
+ * per step; default), 0 = CGS2 (four passes). */
+int qd_krylov_set_method(qd_krylov* kr, int method);
 
 /* ------------------------------------------------- FP64 host numerics */
 /* The small dense problems of the solvers, exposed for direct checks.

@@ -77,6 +77,7 @@ SIGNATURES = {
     "qd_linop_dim": (I64, [VP]),
     "qd_linop_set_lanes": (I, [VP, I]),
     "qd_linop_get_lanes": (I, [VP]),
+    "qd_linop_set_structured": (I, [VP, I, ctypes.POINTER(I)]),
     "qd_linop_autotune": (I, [VP, VP, VP, VP, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_float)]),
     "qd_spmv": (I, [VP, VP, VP, VP]),
     "qd_spmv_acc": (I, [VP, VP, DP, VP, VP]),
@@ -91,6 +92,7 @@ SIGNATURES = {
     "qd_krylov_eigenvalues": (I, [VP, DP, I, ctypes.POINTER(I)]),
     "qd_krylov_converge_ev": (I, [VP, I, ctypes.c_double, ctypes.c_double]),
     "qd_krylov_reset": (I, [VP]),
+    "qd_krylov_set_method": (I, [VP, I]),
     "qd_host_tridiag_eig": (I, [I, DP, DP, DP, DP]),
     "qd_host_expm": (I, [I, DP, DP]),
 }

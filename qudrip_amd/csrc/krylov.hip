@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(KT)
 // h[c] = sum_b part[b * ncols + c]; kcol[c] (+)= h[c].  One block per column.
 __global__ void __launch_bounds__(KT)
     reduce_cols_kernel(const cplx* __restrict__ part, int nb, int ncols, cplx* __restrict__ h,
-                       cplx* __restrict__ kcol, int accumulate, const int* __restrict__ flag) {
+                       cplx* __restrict__ kcol, int accumulate, const int* __restrict__ flag, int nk = 1 << 30) {
   if (flag && *flag) return;
   __shared__ cplx sh[KT / 64];
   const int c = blockIdx.x;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(KT)
     cplx t = sh[0];
     for (int q = 1; q < KT / 64; ++q) t = cadd(t, sh[q]);
     h[c] = t;
-    if (kcol) kcol[c] = accumulate ? cadd(kcol[c], t) : t;
+    if (kcol && c < nk) kcol[c] = accumulate ? cadd(kcol[c], t) : t;
   }
 }
 
@@ -249,6 +249,223 @@ __global__ void sum_double_kernel(const double* __restrict__ part, int nb, doubl
   }
 }
 
+
+// ---------------------------------------------------------------- DCGS2
+// Delayed classical Gram-Schmidt with reorthogonalisation (after Bielich,
+// Langou, Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch Gram-Schmidt with
+// delayed reorthogonalization for Krylov solvers", Parallel Computing 2022).
+// Step j holds u_j = A q_{j-1} projected ONCE against Q_{j-1}.  One pass over
+// Q_{j-1} computes c = Q^H u_j, d = Q^H w (w = A u_j), u^H u, u^H w; then
+//   alpha = sqrt(u^H u - |c|^2),  q_j = (u_j - Q c) / alpha      (2nd pass of u_j)
+//   H[:, j-1] += c, H[j, j-1] = alpha
+//   A q_j = (w - Q_j e) / alpha,  e = H[0:j+1, 0:j] c            (Arnoldi relation)
+//   f = Q_j^H w / alpha = [d ; (u^H w - c^H d) / alpha] / alpha
+//   H[0:j+1, j] = f - e / alpha                                   (1st pass of A q_j)
+//   u_{j+1} = w / alpha - Q_{j-1} (f[:j] - c f[j] / alpha) - u_j f[j] / alpha
+// and a second pass over Q_{j-1} writes q_j and u_{j+1}: 2 passes per step
+// instead of CGS2's 4.
+
+// part[b * P + k]: k < ncols -> Q_k^H u; NR == 2: ncols + k -> Q_k^H w;
+// P - 2 -> u^H u, P - 1 -> u^H w (0 when NR == 1), P = NR * ncols + 2
+template <int RPT, int NR>
+__global__ void __launch_bounds__(KT)
+    dc_red_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ u,
+                  const cplx* __restrict__ w, cplx* __restrict__ part, const int* __restrict__ flag) {
+  if (*flag) return;
+  __shared__ cplx sh[KT / 64][64][NR];
+  __shared__ cplx sx[KT / 64][2];
+  const int P = NR * ncols + 2;
+  const int64_t base = (int64_t)blockIdx.x * KT * RPT;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  cplx ur[RPT], wr[RPT];
+  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    int64_t r = base + (int64_t)q * KT + threadIdx.x;
+    ur[q] = (r < n) ? u[r] : cmake(0.0, 0.0);
+    wr[q] = (NR == 2 && r < n) ? w[r] : cmake(0.0, 0.0);
+    uu = cadd(uu, cdotmul(ur[q], ur[q]));
+    if (NR == 2) uw = cadd(uw, cdotmul(ur[q], wr[q]));
+  }
+  uu = wave_sum(uu);
+  uw = wave_sum(uw);
+  if (lane == 0) {
+    sx[wv][0] = uu;
+    sx[wv][1] = uw;
+  }
+  for (int c0 = 0; c0 < ncols; c0 += 64) {
+    const int cend = min(ncols, c0 + 64);
+    for (int c = c0; c < cend; ++c) {
+      const cplx* __restrict__ Vc = V + (int64_t)c * ld;
+      cplx a = cmake(0.0, 0.0), bb = cmake(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        int64_t r = base + (int64_t)q * KT + threadIdx.x;
+        if (r < n) {
+          cplx v = Vc[r];
+          a = cadd(a, cdotmul(v, ur[q]));
+          if (NR == 2) bb = cadd(bb, cdotmul(v, wr[q]));
+        }
+      }
+      a = wave_sum(a);
+      if (NR == 2) bb = wave_sum(bb);
+      if (lane == 0) {
+        sh[wv][c - c0][0] = a;
+        if (NR == 2) sh[wv][c - c0][NR - 1] = bb;
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < cend - c0) {
+      const int c = threadIdx.x;
+      cplx t = sh[0][c][0];
+      for (int q = 1; q < KT / 64; ++q) t = cadd(t, sh[q][c][0]);
+      part[(int64_t)blockIdx.x * P + c0 + c] = t;
+      if (NR == 2) {
+        cplx t2 = sh[0][c][NR - 1];
+        for (int q = 1; q < KT / 64; ++q) t2 = cadd(t2, sh[q][c][NR - 1]);
+        part[(int64_t)blockIdx.x * P + ncols + c0 + c] = t2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    cplx t = sx[0][0], t2 = sx[0][1];
+    for (int q = 1; q < KT / 64; ++q) {
+      t = cadd(t, sx[q][0]);
+      t2 = cadd(t2, sx[q][1]);
+    }
+    part[(int64_t)blockIdx.x * P + P - 2] = t;
+    part[(int64_t)blockIdx.x * P + P - 1] = t2;
+  }
+}
+
+// small algebra of step j (one block).  red: reduced partials (layout above).
+// KR column k at KR + k * ldk, rows 0..k+1.  Outputs coefA (= c), coefB and
+// scal = {1/alpha, Re, Im of f[j]/alpha}.  last != 0: finalisation (no w).
+__global__ void __launch_bounds__(KT)
+    dc_small_kernel(const cplx* __restrict__ red, int j, int last, cplx* __restrict__ KR, int ldk,
+                    cplx* __restrict__ coefA, cplx* __restrict__ coefB, double* __restrict__ scal,
+                    int* __restrict__ flag, int* __restrict__ bd_step, double eps) {
+  if (*flag) return;
+  __shared__ double sred[KT / 64];
+  __shared__ cplx cred[KT / 64];
+  __shared__ double s_alpha;
+  __shared__ cplx s_fj;
+  const int NR = last ? 1 : 2;
+  const int P = NR * j + 2;
+  const cplx* c = red;
+  const cplx* d = red + j;
+  const double uu = red[P - 2].x;
+  const cplx uw = red[P - 1];
+  // |c|^2 and c^H d
+  double cc = 0.0;
+  cplx chd = cmake(0.0, 0.0);
+  for (int i = threadIdx.x; i < j; i += KT) {
+    cc += c[i].x * c[i].x + c[i].y * c[i].y;
+    if (!last) chd = cadd(chd, cdotmul(c[i], d[i]));
+  }
+  for (int o = 32; o > 0; o >>= 1) cc += __shfl_xor(cc, o, 64);
+  chd = wave_sum(chd);
+  if ((threadIdx.x & 63) == 0) {
+    sred[threadIdx.x >> 6] = cc;
+    cred[threadIdx.x >> 6] = chd;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    cplx t2 = cmake(0.0, 0.0);
+    for (int q = 0; q < KT / 64; ++q) {
+      t += sred[q];
+      t2 = cadd(t2, cred[q]);
+    }
+    double a2 = uu - t;
+    double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
+    s_alpha = alpha;
+    if (!(alpha >= eps)) {
+      *flag = 1;
+      *bd_step = j;
+    } else {
+      // f[j] = (u^H w - c^H d) / alpha^2
+      s_fj = last ? cmake(0.0, 0.0) : crmul(1.0 / (alpha * alpha), csub(uw, t2));
+    }
+  }
+  __syncthreads();
+  const double alpha = s_alpha;
+  // finalise column j-1 (also on breakdown: its entries up to row j-1 are final)
+  cplx* col = KR + (size_t)(j - 1) * ldk;
+  for (int i = threadIdx.x; i < j; i += KT) col[i] = cadd(col[i], c[i]);
+  if (threadIdx.x == 0) col[j] = cmake(alpha, 0.0);
+  if (!(alpha >= eps)) return;
+  const double inv = 1.0 / alpha;
+  for (int i = threadIdx.x; i < j; i += KT) coefA[i] = c[i];
+  if (threadIdx.x == 0) scal[0] = inv;
+  if (last) return;
+  __threadfence_block();
+  __syncthreads();
+  const cplx fj = s_fj;
+  // e = H[0:j+1, 0:j] c;  column j first pass = f - e/alpha
+  cplx* colj = KR + (size_t)j * ldk;
+  for (int i = threadIdx.x; i <= j; i += KT) {
+    cplx e = cmake(0.0, 0.0);
+    for (int k = (i > 0 ? i - 1 : 0); k < j; ++k) e = cadd(e, cmul(KR[(size_t)k * ldk + i], c[k]));
+    cplx f = (i < j) ? crmul(inv, d[i]) : fj;
+    colj[i] = csub(f, crmul(inv, e));
+    if (i < j) coefB[i] = csub(f, crmul(inv, cmul(c[i], fj)));
+  }
+  if (threadIdx.x == 0) {
+    scal[1] = fj.x * inv;
+    scal[2] = fj.y * inv;
+  }
+}
+
+// writes q_j = (u - Q coefA) / alpha into V[:, j]; unless FINAL also
+// u <- w / alpha - Q coefB - u (f[j] / alpha)
+template <int RPT, bool FINAL>
+__global__ void __launch_bounds__(KT)
+    dc_update_kernel(int64_t n, cplx* __restrict__ V, int64_t ld, int j, cplx* __restrict__ u,
+                     const cplx* __restrict__ w, const cplx* __restrict__ coefA, const cplx* __restrict__ coefB,
+                     const double* __restrict__ scal, const int* __restrict__ flag) {
+  if (*flag) return;
+  const int64_t base = (int64_t)blockIdx.x * KT * RPT;
+  cplx s1[RPT], s2[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) s1[q] = s2[q] = cmake(0.0, 0.0);
+  for (int c = 0; c < j; ++c) {
+    const cplx a = coefA[c];
+    const cplx bcf = FINAL ? cmake(0.0, 0.0) : coefB[c];
+    const cplx* __restrict__ Vc = V + (int64_t)c * ld;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      int64_t r = base + (int64_t)q * KT + threadIdx.x;
+      if (r < n) {
+        cplx v = Vc[r];
+        s1[q].x += v.x * a.x - v.y * a.y;
+        s1[q].y += v.x * a.y + v.y * a.x;
+        if (!FINAL) {
+          s2[q].x += v.x * bcf.x - v.y * bcf.y;
+          s2[q].y += v.x * bcf.y + v.y * bcf.x;
+        }
+      }
+    }
+  }
+  const double inv = scal[0];
+  const cplx us = cmake(scal[1], scal[2]);
+  cplx* __restrict__ Vj = V + (int64_t)j * ld;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    int64_t r = base + (int64_t)q * KT + threadIdx.x;
+    if (r < n) {
+      cplx ur = u[r];
+      Vj[r] = cmake((ur.x - s1[q].x) * inv, (ur.y - s1[q].y) * inv);
+      if (!FINAL) {
+        cplx wr = w[r];
+        cplx t = cmul(ur, us);
+        u[r] = cmake(wr.x * inv - s2[q].x - t.x, wr.y * inv - s2[q].y - t.y);
+      }
+    }
+  }
+}
+
 int pick_rpt(int64_t n, int ncu) {
   if (n >= (int64_t)KT * 8 * ncu * 4) return 8;
   if (n >= (int64_t)KT * 4 * ncu * 2) return 4;
@@ -277,6 +494,10 @@ struct KrylovDev {
   cplx* part;
   double* npart;
   cplx* y;
+  cplx* coefA;
+  cplx* coefB;
+  double* scal;
+  cplx* red;
 };
 
 static KrylovDev kr_layout(qd_krylov* kr) {
@@ -294,6 +515,14 @@ static KrylovDev kr_layout(qd_krylov* kr) {
   p += sizeof(cplx) * (K + 2);
   d.KR = (cplx*)p;
   p += sizeof(cplx) * (size_t)(K + 2) * (K + 1);
+  d.coefA = (cplx*)p;
+  p += sizeof(cplx) * (K + 2);
+  d.coefB = (cplx*)p;
+  p += sizeof(cplx) * (K + 2);
+  d.red = (cplx*)p;
+  p += sizeof(cplx) * (2 * K + 8);
+  d.scal = (double*)p;
+  p += sizeof(double) * 8;
   d.npart = (double*)p;
   int64_t nb = kr_nb(kr, 1);
   p += sizeof(double) * (size_t)(nb + 1);
@@ -305,7 +534,8 @@ static KrylovDev kr_layout(qd_krylov* kr) {
 static size_t kr_scal_bytes(int64_t n, int K) {
   int64_t nb = (n + KT - 1) / KT;  // rpt = 1 is the largest block count
   return 64 + sizeof(cplx) * (size_t)(2 * (K + 2)) + sizeof(cplx) * (size_t)(K + 2) * (K + 1) +
-         sizeof(double) * (size_t)(nb + 1) + 256 + sizeof(cplx) * (size_t)nb * (K + 1);
+         sizeof(cplx) * (size_t)(2 * (K + 2) + 2 * K + 8) + sizeof(double) * 8 + sizeof(double) * (size_t)(nb + 1) +
+         256 + sizeof(cplx) * (size_t)nb * (2 * K + 4);
 }
 
 extern "C" int qd_krylov_create(qd_ctx* ctx, int64_t n, int kry_dim, int herm, qd_krylov** out) {
@@ -324,8 +554,10 @@ extern "C" int qd_krylov_create(qd_ctx* ctx, int64_t n, int kry_dim, int herm, q
     delete kr;
     return qd_set_error(ctx, QD_E_OOM, "Krylov basis: %zu bytes", vbytes);
   }
-  if (hipMalloc(&kr->d_scal, kr_scal_bytes(n, kry_dim)) != hipSuccess) {
+  if (hipMalloc(&kr->d_scal, kr_scal_bytes(n, kry_dim)) != hipSuccess ||
+      hipMalloc(&kr->d_w, sizeof(cplx) * 2 * (size_t)kr->ld) != hipSuccess) {
     hipFree(kr->d_V);
+    hipFree(kr->d_scal);
     delete kr;
     return qd_set_error(ctx, QD_E_OOM, "Krylov scalars");
   }
@@ -341,7 +573,16 @@ extern "C" int qd_krylov_destroy(qd_krylov* kr) {
   hipStreamSynchronize(kr->ctx->stream);
   hipFree(kr->d_V);
   hipFree(kr->d_scal);
+  hipFree(kr->d_w);
   delete kr;
+  return QD_OK;
+}
+
+// Gram-Schmidt variant: 0 = CGS2 (four passes over the basis per step),
+// 1 = DCGS2 (two passes, the default).
+extern "C" int qd_krylov_set_method(qd_krylov* kr, int method) {
+  if (!kr || (method != 0 && method != 1)) return QD_E_ARG;
+  kr->method = method;
   return QD_OK;
 }
 
@@ -389,8 +630,83 @@ static void launch_gs_step(qd_krylov* kr, const KrylovDev& d, int j, cplx* w) {
   scale_kernel<<<qd_grid(n, 256), 256, 0, st>>>(n, w, d.inv, d.flag);
 }
 
+template <int RPT>
+static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int s, int add) {
+  qd_ctx* ctx = kr->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = kr->n;
+  const int nb = (int)kr_nb(kr, RPT);
+  const int ldk = kr->kry_dim + 2;
+  cplx* U = kr->d_w;
+  cplx* W = kr->d_w + kr->ld;
+  // bootstrap: u_{s+1} = A q_s - Q_s (Q_s^H A q_s); H[0:s+1, s] = first pass
+  int rc = qd_linop_apply(op, kr->d_V + (size_t)s * kr->ld, U, nullptr, 0, nullptr, cmake(0.0, 0.0));
+  if (rc) return rc;
+  gemv_t_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, U, d.part, d.flag);
+  reduce_cols_kernel<<<s + 1, KT, 0, st>>>(d.part, nb, s + 1, d.h, d.KR + (size_t)s * ldk, 0, d.flag);
+  gemv_n_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, d.h, U, nullptr, d.flag);
+  for (int j = s + 1; j < s + add; ++j) {
+    rc = qd_linop_apply(op, U, W, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    if (rc) return rc;
+    const int P = 2 * j + 2;
+    dc_red_kernel<RPT, 2><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.part, d.flag);
+    reduce_cols_kernel<<<P, KT, 0, st>>>(d.part, nb, P, d.red, nullptr, 0, d.flag);
+    dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 0, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS);
+    dc_update_kernel<RPT, false><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.coefA, d.coefB, d.scal, d.flag);
+  }
+  // finalise: reorthogonalise the pending vector into V[:, s + add]
+  const int j = s + add;
+  dc_red_kernel<RPT, 1><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.part, d.flag);
+  reduce_cols_kernel<<<j + 2, KT, 0, st>>>(d.part, nb, j + 2, d.red, nullptr, 0, d.flag);
+  dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 1, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS);
+  dc_update_kernel<RPT, true><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.coefA, d.coefB, d.scal, d.flag);
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}
+
+// DCGS2 expansion by add columns from act_dim (two passes over the basis per
+// step).  Breakdown at step j leaves act_dim = j.
+static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add) {
+  qd_ctx* ctx = kr->ctx;
+  if (kr->breakdown || add <= 0) return QD_OK;
+  KrylovDev d = kr_layout(kr);
+  QD_HIP(ctx, hipMemsetAsync(d.flag, 0, 16, ctx->stream));
+  const int s = kr->act_dim;
+  int rpt = pick_rpt(kr->n, ctx->num_cus);
+  int rc = (rpt == 8) ? dc_expand_launch<8>(kr, op, d, s, add)
+                      : (rpt == 4) ? dc_expand_launch<4>(kr, op, d, s, add) : dc_expand_launch<1>(kr, op, d, s, add);
+  if (rc) return rc;
+  const int K = kr->kry_dim, ldk = K + 2;
+  std::vector<cplx> cols((size_t)add * ldk);
+  int hflag[4] = {0, 0, 0, 0};  // flag at +0, breakdown step at +8
+  QD_HIP(ctx, hipMemcpyAsync(hflag, d.flag, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  QD_HIP(ctx, hipMemcpyAsync(cols.data(), d.KR + (size_t)s * ldk, sizeof(cplx) * cols.size(), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int end = s + add;
+  if (hflag[0]) {
+    end = hflag[2];  // columns 0..end-1 span an invariant subspace
+    kr->breakdown = 1;
+  }
+  kr->n_spmv += add;
+  for (int j = s; j < end; ++j) {
+    const cplx* c = cols.data() + (size_t)(j - s) * ldk;
+    for (int i = 0; i <= j && i < K; ++i) kr->KR[(size_t)j * K + i] = c[i];
+    if (j + 1 < K) kr->KR[(size_t)j * K + j + 1] = c[j + 1];
+    kr->beta_last = c[j + 1].x;
+  }
+  kr->n_last = cmake(kr->beta_last, 0.0);
+  kr->act_dim = end;
+  return QD_OK;
+}
+
 // extend the basis from act_dim by add columns; reads KR back.
+static int kr_expand_cgs2(qd_krylov* kr, qd_linop* op, int add);
 static int kr_expand(qd_krylov* kr, qd_linop* op, int add) {
+  return kr->method == 0 ? kr_expand_cgs2(kr, op, add) : kr_expand_dc(kr, op, add);
+}
+
+static int kr_expand_cgs2(qd_krylov* kr, qd_linop* op, int add) {
   qd_ctx* ctx = kr->ctx;
   if (kr->breakdown || add <= 0) return QD_OK;
   KrylovDev d = kr_layout(kr);
@@ -413,14 +729,14 @@ static int kr_expand(qd_krylov* kr, qd_linop* op, int add) {
   // read back flag, breakdown step and the new Hessenberg columns
   int K = kr->kry_dim;
   std::vector<cplx> cols((size_t)add * (K + 2));
-  int hflag[2] = {0, 0};
-  QD_HIP(ctx, hipMemcpyAsync(hflag, d.flag, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  int hflag[4] = {0, 0, 0, 0};  // flag at +0, breakdown step at +8
+  QD_HIP(ctx, hipMemcpyAsync(hflag, d.flag, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   QD_HIP(ctx, hipMemcpyAsync(cols.data(), d.KR + (size_t)start * (K + 2), sizeof(cplx) * cols.size(),
                              hipMemcpyDeviceToHost, ctx->stream));
   QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
   int end = start + add;
   if (hflag[0]) {
-    end = hflag[1] + 1;  // column bd_step is complete; the space is invariant
+    end = hflag[2] + 1;  // column bd_step is complete; the space is invariant
     kr->breakdown = 1;
   }
   kr->n_spmv += (end - start) + (hflag[0] ? 0 : 0);

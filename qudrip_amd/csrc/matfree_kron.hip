@@ -1,0 +1,611 @@
+// matfree_kron.hip -- K6 fast path: matrix-free product for programs that
+// split over a two-block sector (the Hubbard layout of
+// programs/hubbard_diss.cpp:97-104: up spins on the low bits, down spins on
+// the high bits).
+//
+// With the sector a mixed radix of block A (low bits, n_A states) and block
+// B (high bits, n_B states), a state vector is a dense matrix X[b][a]
+// (index b * n_A + a).  When every term is either
+//   - a product of hops inside A        (the Jordan-Wigner popcounts of a hop
+//                                        pair each pick up the constant
+//                                        particle number of B: an even total,
+//                                        so the sign is A-local),
+//   - any product of ops inside B        (B is the highest block: its JW
+//                                        strings only see B bits), or
+//   - diagonal (number operators, diagonal bit gates),
+// the operator is H = T_A (x) I + I (x) T_B + D and
+//   Y[b][a] = sum_a' T_A[a][a'] X[b][a'] + sum_b' T_B[b][b'] X[b'][a] + D(b,a) X[b][a].
+// T_A and T_B are small sparse matrices built once with the exact CSR
+// builder on the block sectors; D is evaluated on the fly from the diagonal
+// terms.  Two kernels:
+//   kron_rows_kernel  one workgroup per row b: X[b][:] staged in LDS, the
+//                     T_A gathers read LDS, plus D -> writes Y
+//   kron_slab_kernel  column slabs of W entries: every row b adds
+//                     sum T_B[b][b'] X[b'][slab]; workgroups of one slab run
+//                     on one XCD so the slab (n_B * W * 16 B) stays in its L2.
+// HBM traffic ~ X twice + Y three times (vs. 32 n algorithmic).
+#include <algorithm>
+
+#include "qd_internal.hpp"
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+struct KronDesc {
+  int64_t nA, nB;
+  int loB;  // first bit of block B
+  // T_A / T_B in pull form: row r = conj of column r of the Hermitian block
+  const int32_t* Aouter;
+  const int32_t* Ainner;
+  const cplx* Avals;
+  const int32_t* Bouter;
+  const int32_t* Binner;
+  const cplx* Bvals;
+  const uint64_t* statesA;
+  const uint64_t* statesB;
+  const DevTerm* dterms;  // diagonal terms other than number products
+  const DevOp* dops;
+  int ndterms;
+  // number-operator products c * prod n_s (hopBitOp(s, s), hopbit.hpp:50-51):
+  // value c when every bit of the mask is set
+  const uint64_t* nmask;
+  const cplx* ncoef;
+  int nnum;
+  // packed T_A (ELL, entry j of row a at ell[j * nA + a]): column in the high
+  // 16 bits, index into vtab in the low 16 bits; vtab[0] = 0 pads short rows.
+  const uint32_t* ell;
+  int ell_w;
+  const cplx* vtab;
+  int nvtab;
+  // packed T_B, row-major (entry j of row b at ellB[b * ellB_w + j]), same
+  // encoding; vtabB[0] = 0 pads short rows
+  const uint32_t* ellB;
+  int ellB_w;
+  const cplx* vtabB;
+  int nvtabB;
+};
+
+struct qd_kron {
+  uint32_t* d_ellB = nullptr;
+  cplx* d_vtabB = nullptr;
+  uint64_t* d_nmask = nullptr;
+  cplx* d_ncoef = nullptr;
+  uint32_t* d_ell = nullptr;
+  cplx* d_vtab = nullptr;
+  qd_basis* basisA = nullptr;
+  qd_basis* basisB = nullptr;
+  qd_termprog* progA = nullptr;
+  qd_termprog* progB = nullptr;
+  qd_csr* csrA = nullptr;
+  qd_csr* csrB = nullptr;
+  DevTerm* d_dterms = nullptr;
+  DevOp* d_dops = nullptr;
+  int ndterms = 0;
+  KronDesc desc{};
+};
+
+__device__ inline cplx diag_value(const KronDesc& k, uint64_t s) {
+  cplx d = cmake(0.0, 0.0);
+  for (int t = 0; t < k.nnum; ++t) {
+    const uint64_t m = k.nmask[t];
+    if ((s & m) == m) d = cadd(d, k.ncoef[t]);
+  }
+  for (int t = 0; t < k.ndterms; ++t) {
+    const DevTerm term = k.dterms[t];
+    const DevOp* tops = k.dops + term.op_begin;
+    int nbitops = 0;
+    for (int q = 0; q < term.n_ops; ++q) nbitops += (tops[q].kind == QD_OP_BIT);
+    for (int L = 0; L < term.n_leaves; ++L) {
+      uint64_t xt;
+      cplx w;
+      if (!eval_leaf(tops, term.n_ops, nbitops, s, L, &xt, &w)) continue;
+      if (xt != s) continue;
+      d = cadd(d, cmul(term.coeff, w));
+    }
+  }
+  return d;
+}
+
+// one workgroup per row b; LDS holds X[b][:]
+template <bool LDSX>
+__global__ void __launch_bounds__(1024)
+    kron_rows_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift) {
+  extern __shared__ cplx xrow[];
+  const int64_t b = blockIdx.x;
+  const cplx* __restrict__ xb = x + b * k.nA;
+  if (LDSX) {
+    for (int64_t a = threadIdx.x; a < k.nA; a += blockDim.x) xrow[a] = xb[a];
+    __syncthreads();
+  }
+  const uint64_t sB = k.statesB[b] << k.loB;
+  for (int64_t a = threadIdx.x; a < k.nA; a += blockDim.x) {
+    double sx = 0.0, sy = 0.0;
+    const int32_t e0 = k.Aouter[a], e1 = k.Aouter[a + 1];
+    for (int32_t e = e0; e < e1; ++e) {
+      cplx v = k.Avals[e];
+      cplx xv = LDSX ? xrow[k.Ainner[e]] : xb[k.Ainner[e]];
+      sx += v.x * xv.x + v.y * xv.y;  // conj(v) * x
+      sy += v.x * xv.y - v.y * xv.x;
+    }
+    cplx xa = LDSX ? xrow[a] : xb[a];
+    cplx d = cadd(diag_value(k, sB | k.statesA[a]), shift);
+    cplx dx = cmul(d, xa);
+    y[b * k.nA + a] = cmake(sx + dx.x, sy + dx.y);
+  }
+}
+
+// Packed variant: RB rows per workgroup share one pass over the packed T_A;
+// X rows and the value table live in LDS.
+template <int RB>
+__global__ void __launch_bounds__(1024)
+    kron_rows_packed_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift) {
+  extern __shared__ cplx lds[];
+  cplx* vt = lds;                 // nvtab entries
+  cplx* xr = lds + k.nvtab;       // RB rows of nA
+  const int64_t b0 = (int64_t)blockIdx.x * RB;
+  const int rb = (int)min<int64_t>(RB, k.nB - b0);
+  for (int i = threadIdx.x; i < k.nvtab; i += blockDim.x) vt[i] = k.vtab[i];
+  for (int r = 0; r < rb; ++r) {
+    const cplx* __restrict__ xb = x + (b0 + r) * k.nA;
+    for (int64_t a = threadIdx.x; a < k.nA; a += blockDim.x) xr[r * k.nA + a] = xb[a];
+  }
+  __syncthreads();
+  uint64_t sB[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) sB[r] = (r < rb) ? (k.statesB[b0 + r] << k.loB) : 0;
+  for (int64_t a = threadIdx.x; a < k.nA; a += blockDim.x) {
+    double sx[RB], sy[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) sx[r] = sy[r] = 0.0;
+    for (int j = 0; j < k.ell_w; ++j) {
+      const uint32_t e = k.ell[(int64_t)j * k.nA + a];
+      const cplx v = vt[e & 0xFFFFu];
+      const int c = (int)(e >> 16);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const cplx xv = xr[r * k.nA + c];
+        sx[r] += v.x * xv.x + v.y * xv.y;  // conj(v) * x
+        sy[r] += v.x * xv.y - v.y * xv.x;
+      }
+    }
+    const uint64_t sA = k.statesA[a];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (r < rb) {
+        cplx d = cadd(diag_value(k, sB[r] | sA), shift);
+        cplx dx = cmul(d, xr[r * k.nA + a]);
+        y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
+      }
+    }
+  }
+}
+
+// Y[b][slab] += sum_e conj(TB[b][e]) X[col_e][slab]; W = 32 columns per slab,
+// a wave covers two rows (32 lanes each).  Block id -> (XCD, slab, row chunk)
+// so that one XCD works through its slabs one after the other.
+template <int W>
+__global__ void __launch_bounds__(256)
+    kron_slab_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, int chunks, int rows_per_chunk) {
+  const int xcd = blockIdx.x & 7;
+  const int kk = blockIdx.x >> 3;
+  const int64_t slab = (int64_t)xcd + 8 * (kk / chunks);
+  const int chunk = kk % chunks;
+  const int64_t a0 = slab * W;
+  if (a0 >= k.nA) return;
+  const int lane = threadIdx.x % W;
+  const int sub = threadIdx.x / W;  // rows handled by this block in parallel
+  const int nsub = blockDim.x / W;
+  const int64_t a = a0 + lane;
+  const bool col_ok = a < k.nA;
+  const int64_t rb = (int64_t)chunk * rows_per_chunk;
+  const int64_t re = min<int64_t>(k.nB, rb + rows_per_chunk);
+  for (int64_t b = rb + sub; b < re; b += nsub) {
+    double sx = 0.0, sy = 0.0;
+    const int32_t e0 = k.Bouter[b], e1 = k.Bouter[b + 1];
+    for (int32_t e = e0; e < e1; ++e) {
+      const cplx v = k.Bvals[e];
+      const int32_t bp = k.Binner[e];
+      if (col_ok) {
+        cplx xv = x[(int64_t)bp * k.nA + a];
+        sx += v.x * xv.x + v.y * xv.y;
+        sy += v.x * xv.y - v.y * xv.x;
+      }
+    }
+    if (col_ok) {
+      cplx o = y[b * k.nA + a];
+      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+    }
+  }
+}
+
+// Packed slab kernel: T_B entries of a row are read once per 32-lane group
+// (broadcast) U at a time, and the U gathers of X row segments are issued
+// before any is consumed.
+template <int W, int U>
+__global__ void __launch_bounds__(256)
+    kron_slab_packed_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, int chunks,
+                            int rows_per_chunk) {
+  __shared__ cplx vt[256];
+  const int xcd = blockIdx.x & 7;
+  const int kk = blockIdx.x >> 3;
+  const int64_t slab = (int64_t)xcd + 8 * (kk / chunks);
+  const int chunk = kk % chunks;
+  const int64_t a0 = slab * W;
+  if (a0 >= k.nA) return;
+  for (int i = threadIdx.x; i < k.nvtabB; i += blockDim.x) vt[i] = k.vtabB[i];
+  __syncthreads();
+  const int lane = threadIdx.x % W;
+  const int sub = threadIdx.x / W;
+  const int nsub = blockDim.x / W;
+  const int64_t a = min<int64_t>(a0 + lane, k.nA - 1);  // clamp: duplicate lanes recompute the last column
+  const bool col_ok = a0 + lane < k.nA;
+  const int64_t rb = (int64_t)chunk * rows_per_chunk;
+  const int64_t re = min<int64_t>(k.nB, rb + rows_per_chunk);
+  const int w = k.ellB_w;
+  for (int64_t b = rb + sub; b < re; b += nsub) {
+    const uint32_t* __restrict__ er = k.ellB + b * w;
+    double sx = 0.0, sy = 0.0;
+    for (int j = 0; j < w; j += U) {
+      uint32_t e[U];
+      cplx xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) e[u] = (j + u < w) ? er[j + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[(int64_t)(e[u] >> 16) * k.nA + a];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const cplx v = vt[e[u] & 0xFFFFu];
+        sx += v.x * xv[u].x + v.y * xv[u].y;
+        sy += v.x * xv[u].y - v.y * xv[u].x;
+      }
+    }
+    if (col_ok) {
+      cplx o = y[b * k.nA + a];
+      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host
+// pack a pull-form block matrix: entries (col << 16 | value id); row-major
+// when rowmajor, else entry-major (ELL column j contiguous over rows)
+static bool pack_block(qd_ctx* ctx, const qd_csr* c, int64_t n, bool rowmajor, int max_tab, uint32_t** d_ell,
+                       cplx** d_tab, int* width, int* ntab) {
+  if (n >= 65536) return false;
+  int64_t nnz = 0;
+  qd_csr_info(c, nullptr, &nnz, nullptr);
+  std::vector<int32_t> ao(n + 1), ai(nnz);
+  std::vector<cplx> av(nnz);
+  if (qd_csr_download(const_cast<qd_csr*>(c), ao.data(), ai.data(), (double*)av.data()) != QD_OK) return false;
+  std::vector<cplx> tab(1, cmake(0.0, 0.0));
+  std::vector<uint32_t> vid(nnz);
+  for (int64_t e = 0; e < nnz; ++e) {
+    size_t q = 0;
+    for (; q < tab.size(); ++q)
+      if (std::memcmp(&tab[q], &av[e], sizeof(cplx)) == 0) break;
+    if (q == tab.size()) {
+      if ((int)tab.size() >= max_tab) return false;
+      tab.push_back(av[e]);
+    }
+    vid[e] = (uint32_t)q;
+  }
+  int w = 1;
+  for (int64_t r = 0; r < n; ++r) w = std::max(w, ao[r + 1] - ao[r]);
+  std::vector<uint32_t> ell((size_t)w * n);
+  for (int64_t r = 0; r < n; ++r)
+    for (int j = 0; j < w; ++j) {
+      int32_t e = ao[r] + j;
+      uint32_t v = (e < ao[r + 1]) ? (((uint32_t)ai[e] << 16) | vid[e]) : ((uint32_t)r << 16);
+      ell[rowmajor ? (size_t)r * w + j : (size_t)j * n + r] = v;
+    }
+  if (hipMalloc(d_ell, sizeof(uint32_t) * ell.size()) != hipSuccess) return false;
+  if (hipMalloc(d_tab, sizeof(cplx) * tab.size()) != hipSuccess) {
+    hipFree(*d_ell);
+    *d_ell = nullptr;
+    return false;
+  }
+  hipMemcpyAsync(*d_ell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, ctx->stream);
+  hipMemcpyAsync(*d_tab, tab.data(), sizeof(cplx) * tab.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
+  *width = w;
+  *ntab = (int)tab.size();
+  return true;
+}
+
+static bool is_diag_op(const DevOp& o) {
+  if (o.kind == QD_OP_HOP) return o.a == o.b;
+  return o.U[1].x == 0.0 && o.U[1].y == 0.0 && o.U[2].x == 0.0 && o.U[2].y == 0.0;
+}
+
+// Try to build the Kronecker form; returns nullptr (no error) if the program
+// or sector does not have it.
+qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
+  const SectorDesc& sec = basis->sec;
+  if (!sec.cns || sec.nblocks != 2 || sec.cnt[0] < 0 || sec.cnt[1] < 0) return nullptr;
+  const int loA = sec.lo[0], lenA = sec.len[0], loB = sec.lo[1], lenB = sec.len[1];
+  if (loA != 0 || loB != lenA || loB + lenB != sec.nbits) return nullptr;
+  std::vector<DevTerm> tA, tB, tD;
+  std::vector<DevOp> oA, oB, oD;
+  auto inA = [&](int s) { return s >= loA && s < loA + lenA; };
+  auto inB = [&](int s) { return s >= loB && s < loB + lenB; };
+  for (const DevTerm& t : prog->h_terms) {
+    bool diag = true, allA = true, allB = true, hopsOnly = true;
+    for (int q = 0; q < t.n_ops; ++q) {
+      const DevOp& o = prog->h_ops[t.op_begin + q];
+      diag = diag && is_diag_op(o);
+      allA = allA && inA(o.a) && inA(o.b);
+      allB = allB && inB(o.a) && inB(o.b);
+      hopsOnly = hopsOnly && (o.kind == QD_OP_HOP);
+    }
+    std::vector<DevTerm>* dt;
+    std::vector<DevOp>* dopv;
+    int shift = 0;
+    if (diag) {
+      dt = &tD;
+      dopv = &oD;
+    } else if (allA && hopsOnly) {
+      dt = &tA;
+      dopv = &oA;
+    } else if (allB) {
+      dt = &tB;
+      dopv = &oB;
+      shift = loB;
+    } else {
+      return nullptr;
+    }
+    DevTerm c = t;
+    c.op_begin = (int)dopv->size();
+    for (int q = 0; q < t.n_ops; ++q) {
+      DevOp o = prog->h_ops[t.op_begin + q];
+      o.a -= shift;
+      o.b -= shift;
+      dopv->push_back(o);
+    }
+    dt->push_back(c);
+  }
+  // sub-programs through the public constructors (validation included)
+  auto make_prog = [&](int nbits, const std::vector<DevTerm>& ts, const std::vector<DevOp>& os) -> qd_termprog* {
+    std::vector<qd_term> qt(ts.size());
+    std::vector<qd_op> qo(os.size());
+    for (size_t i = 0; i < ts.size(); ++i) {
+      qt[i].coeff[0] = ts[i].coeff.x;
+      qt[i].coeff[1] = ts[i].coeff.y;
+      qt[i].op_begin = ts[i].op_begin;
+      qt[i].n_ops = ts[i].n_ops;
+    }
+    for (size_t i = 0; i < os.size(); ++i) {
+      qo[i].kind = os[i].kind;
+      qo[i].site_a = os[i].a;
+      qo[i].site_b = os[i].b;
+      qo[i].string = os[i].string;
+      for (int u = 0; u < 4; ++u) {
+        qo[i].U[2 * u] = os[i].U[u].x;
+        qo[i].U[2 * u + 1] = os[i].U[u].y;
+      }
+    }
+    qd_termprog* p = nullptr;
+    if (qd_termprog_create(ctx, nbits, qt.data(), (int)qt.size(), qo.data(), (int)qo.size(), &p) != QD_OK) return nullptr;
+    return p;
+  };
+  qd_kron* k = new qd_kron();
+  auto fail = [&]() -> qd_kron* {
+    if (k->csrA) qd_csr_destroy(k->csrA);
+    if (k->csrB) qd_csr_destroy(k->csrB);
+    if (k->progA) qd_termprog_destroy(k->progA);
+    if (k->progB) qd_termprog_destroy(k->progB);
+    if (k->basisA) qd_basis_destroy(k->basisA);
+    if (k->basisB) qd_basis_destroy(k->basisB);
+    hipFree(k->d_dterms);
+    hipFree(k->d_dops);
+    delete k;
+    return nullptr;
+  };
+  qd_constraint cA{(lenA >= 64) ? ~0ull : ((1ull << lenA) - 1ull), sec.cnt[0]};
+  qd_constraint cB{(lenB >= 64) ? ~0ull : ((1ull << lenB) - 1ull), sec.cnt[1]};
+  if (qd_basis_create(ctx, lenA, &cA, 1, &k->basisA) != QD_OK || qd_basis_create(ctx, lenB, &cB, 1, &k->basisB) != QD_OK)
+    return fail();
+  // an empty block program is a zero matrix: give it one zero-coefficient number term
+  DevOp zop{};
+  zop.kind = QD_OP_HOP;
+  zop.string = 1;
+  if (tA.empty()) {
+    tA.push_back(DevTerm{cmake(0.0, 0.0), 0, 1, 0, 1});
+    oA.push_back(zop);
+  }
+  if (tB.empty()) {
+    tB.push_back(DevTerm{cmake(0.0, 0.0), 0, 1, 0, 1});
+    oB.push_back(zop);
+  }
+  k->progA = make_prog(lenA, tA, oA);
+  k->progB = make_prog(lenB, tB, oB);
+  if (!k->progA || !k->progB) return fail();
+  if (qd_csr_build(ctx, k->basisA, k->progA, &k->csrA) != QD_OK || qd_csr_build(ctx, k->basisB, k->progB, &k->csrB) != QD_OK)
+    return fail();
+  // pull form needs rows of each block operator: for a non-Hermitian block use
+  // the CSC of its adjoint program
+  const qd_csr* useA = k->csrA;
+  const qd_csr* useB = k->csrB;
+  if (k->csrA->hermitian != 1) {
+    if (qd_csr_build_impl(ctx, k->basisA, k->progA, 1, &k->csrA->adj) != QD_OK) return fail();
+    useA = k->csrA->adj;
+  }
+  if (k->csrB->hermitian != 1) {
+    if (qd_csr_build_impl(ctx, k->basisB, k->progB, 1, &k->csrB->adj) != QD_OK) return fail();
+    useB = k->csrB->adj;
+  }
+  // number-operator products -> (mask, coefficient)
+  std::vector<uint64_t> nmask;
+  std::vector<cplx> ncoef;
+  {
+    std::vector<DevTerm> rest;
+    std::vector<DevOp> rest_ops;
+    for (const DevTerm& t : tD) {
+      bool num = true;
+      uint64_t m = 0;
+      for (int q = 0; q < t.n_ops; ++q) {
+        const DevOp& o = oD[t.op_begin + q];
+        if (o.kind != QD_OP_HOP || o.a != o.b) num = false;
+        else m |= (1ull << o.a);
+      }
+      if (num) {
+        nmask.push_back(m);
+        ncoef.push_back(t.coeff);
+      } else {
+        DevTerm c = t;
+        c.op_begin = (int)rest_ops.size();
+        for (int q = 0; q < t.n_ops; ++q) rest_ops.push_back(oD[t.op_begin + q]);
+        rest.push_back(c);
+      }
+    }
+    tD.swap(rest);
+    oD.swap(rest_ops);
+  }
+  if (hipMalloc(&k->d_nmask, sizeof(uint64_t) * std::max<size_t>(1, nmask.size())) != hipSuccess ||
+      hipMalloc(&k->d_ncoef, sizeof(cplx) * std::max<size_t>(1, ncoef.size())) != hipSuccess)
+    return fail();
+  if (!nmask.empty()) {
+    hipMemcpyAsync(k->d_nmask, nmask.data(), sizeof(uint64_t) * nmask.size(), hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(k->d_ncoef, ncoef.data(), sizeof(cplx) * ncoef.size(), hipMemcpyHostToDevice, ctx->stream);
+  }
+  k->ndterms = (int)tD.size();
+  if (hipMalloc(&k->d_dterms, sizeof(DevTerm) * std::max<size_t>(1, tD.size())) != hipSuccess ||
+      hipMalloc(&k->d_dops, sizeof(DevOp) * std::max<size_t>(1, oD.size())) != hipSuccess)
+    return fail();
+  if (!tD.empty()) {
+    // recompute leaf counts for the diagonal sub-program layout
+    for (auto& t : tD) {
+      int nl = 1;
+      for (int q = 0; q < t.n_ops; ++q) nl *= (oD[t.op_begin + q].kind == QD_OP_BIT) ? 2 : 1;
+      t.n_leaves = nl;
+    }
+    hipMemcpyAsync(k->d_dterms, tD.data(), sizeof(DevTerm) * tD.size(), hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(k->d_dops, oD.data(), sizeof(DevOp) * oD.size(), hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail();
+  KronDesc& d = k->desc;
+  d.nA = (int64_t)k->basisA->sec.total;
+  d.nB = (int64_t)k->basisB->sec.total;
+  d.loB = loB;
+  d.Aouter = useA->d_outer;
+  d.Ainner = useA->d_inner;
+  d.Avals = useA->d_vals;
+  d.Bouter = useB->d_outer;
+  d.Binner = useB->d_inner;
+  d.Bvals = useB->d_vals;
+  d.statesA = k->basisA->d_states;
+  d.statesB = k->basisB->d_states;
+  d.dterms = k->d_dterms;
+  d.dops = k->d_dops;
+  d.ndterms = k->ndterms;
+  d.nmask = k->d_nmask;
+  d.ncoef = k->d_ncoef;
+  d.nnum = (int)nmask.size();
+  // packed T_A when columns fit 16 bits and the block has < 65536 distinct values
+  d.ell = nullptr;
+  if (d.nA < 65536) {
+    int64_t nnz = 0;
+    qd_csr_info(useA, nullptr, &nnz, nullptr);
+    std::vector<int32_t> ao(d.nA + 1), ai(nnz);
+    std::vector<cplx> av(nnz);
+    if (qd_csr_download(const_cast<qd_csr*>(useA), ao.data(), ai.data(), (double*)av.data()) == QD_OK) {
+      std::vector<cplx> tab(1, cmake(0.0, 0.0));
+      std::vector<uint32_t> vid(nnz);
+      bool ok = true;
+      for (int64_t e = 0; e < nnz && ok; ++e) {
+        size_t q = 0;
+        for (; q < tab.size(); ++q)
+          if (std::memcmp(&tab[q], &av[e], sizeof(cplx)) == 0) break;
+        if (q == tab.size()) {
+          if (tab.size() >= 4096) ok = false;
+          tab.push_back(av[e]);
+        }
+        vid[e] = (uint32_t)q;
+      }
+      int w = 0;
+      for (int64_t a = 0; a < d.nA; ++a) w = std::max(w, ao[a + 1] - ao[a]);
+      if (ok) {
+        std::vector<uint32_t> ell((size_t)w * d.nA);
+        for (int64_t a = 0; a < d.nA; ++a)
+          for (int j = 0; j < w; ++j) {
+            int32_t e = ao[a] + j;
+            ell[(size_t)j * d.nA + a] = (e < ao[a + 1]) ? (((uint32_t)ai[e] << 16) | vid[e]) : ((uint32_t)a << 16);
+          }
+        if (hipMalloc(&k->d_ell, sizeof(uint32_t) * std::max<size_t>(1, ell.size())) == hipSuccess &&
+            hipMalloc(&k->d_vtab, sizeof(cplx) * tab.size()) == hipSuccess) {
+          hipMemcpyAsync(k->d_ell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, ctx->stream);
+          hipMemcpyAsync(k->d_vtab, tab.data(), sizeof(cplx) * tab.size(), hipMemcpyHostToDevice, ctx->stream);
+          if (hipStreamSynchronize(ctx->stream) == hipSuccess) {
+            d.ell = k->d_ell;
+            d.ell_w = w;
+            d.vtab = k->d_vtab;
+            d.nvtab = (int)tab.size();
+          }
+        }
+      }
+    }
+  }
+  d.ellB = nullptr;
+  if (pack_block(ctx, useB, d.nB, true, 256, &k->d_ellB, &k->d_vtabB, &d.ellB_w, &d.nvtabB)) {
+    d.ellB = k->d_ellB;
+    d.vtabB = k->d_vtabB;
+  }
+  return k;
+}
+
+void qd_kron_destroy(qd_kron* k) {
+  if (!k) return;
+  hipFree(k->d_ell);
+  hipFree(k->d_vtab);
+  hipFree(k->d_nmask);
+  hipFree(k->d_ncoef);
+  hipFree(k->d_ellB);
+  hipFree(k->d_vtabB);
+  qd_csr_destroy(k->csrA);
+  qd_csr_destroy(k->csrB);
+  qd_termprog_destroy(k->progA);
+  qd_termprog_destroy(k->progB);
+  qd_basis_destroy(k->basisA);
+  qd_basis_destroy(k->basisB);
+  hipFree(k->d_dterms);
+  hipFree(k->d_dops);
+  delete k;
+}
+
+int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
+  const KronDesc& d = k->desc;
+  if (d.nA == 0 || d.nB == 0) return QD_OK;
+  size_t lds = sizeof(cplx) * (size_t)d.nA;
+  static int optin = -1;
+  if (optin < 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess) v = 65536;
+    optin = v;
+    if ((size_t)optin > 65536) hipFuncSetAttribute((const void*)kron_rows_kernel<true>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, optin);
+  }
+  size_t lds2 = sizeof(cplx) * ((size_t)d.nvtab + 2 * (size_t)d.nA);
+  if (d.ell && lds2 <= (size_t)optin) {
+    static bool attr2 = false;
+    if (!attr2 && (size_t)optin > 65536) {
+      hipFuncSetAttribute((const void*)kron_rows_packed_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, optin);
+      attr2 = true;
+    }
+    kron_rows_packed_kernel<2><<<(unsigned)((d.nB + 1) / 2), 1024, lds2, ctx->stream>>>(d, x, y, shift);
+  } else if (lds <= (size_t)optin) {
+    kron_rows_kernel<true><<<(unsigned)d.nB, 1024, lds, ctx->stream>>>(d, x, y, shift);
+  } else {
+    kron_rows_kernel<false><<<(unsigned)d.nB, 1024, 0, ctx->stream>>>(d, x, y, shift);
+  }
+  QD_CHECK_LAUNCH(ctx);
+  constexpr int W = 32;
+  int64_t nslabs = (d.nA + W - 1) / W;
+  int chunks = (int)std::min<int64_t>(128, std::max<int64_t>(1, d.nB / 8));
+  int rows_per_chunk = (int)((d.nB + chunks - 1) / chunks);
+  int64_t slab_groups = (nslabs + 7) / 8;  // slabs per XCD
+  int64_t grid = slab_groups * chunks * 8;
+  if (d.ellB)
+    kron_slab_packed_kernel<W, 4><<<(unsigned)grid, 256, 0, ctx->stream>>>(d, x, y, chunks, rows_per_chunk);
+  else
+    kron_slab_kernel<W><<<(unsigned)grid, 256, 0, ctx->stream>>>(d, x, y, chunks, rows_per_chunk);
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}

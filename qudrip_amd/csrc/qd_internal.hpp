@@ -223,6 +223,7 @@ struct qd_csr {
 };
 
 enum { QD_LINOP_CSR = 0, QD_LINOP_MATFREE = 1 };
+struct qd_kron;
 struct qd_linop {
   qd_ctx* ctx = nullptr;
   int kind = QD_LINOP_CSR;
@@ -236,7 +237,12 @@ struct qd_linop {
   // matrix-free kind
   qd_basis* basis = nullptr;
   qd_termprog* prog = nullptr;
+  qd_kron* kron = nullptr;  // two-block Kronecker form, when the program has it
+  int use_kron = 1;
 };
+qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog);
+void qd_kron_destroy(qd_kron* k);
+int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift);
 
 struct qd_krylov {
   qd_ctx* ctx = nullptr;
@@ -256,6 +262,7 @@ struct qd_krylov {
   double beta_last = 0.0;
   int breakdown = 0;
   int64_t n_spmv = 0;
+  int method = 1;  // 0 = CGS2, 1 = DCGS2
 };
 
 // --------------------------------------------------------------- errors

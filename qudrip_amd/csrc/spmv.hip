@@ -248,6 +248,8 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
       default:
         return qd_set_error(ctx, QD_E_ARG, "lanes_per_row %d", op->lanes_per_row);
     }
+  } else if (op->kron && op->use_kron && !d_xscale && !d_diag) {
+    return qd_kron_apply(ctx, op->kron, x, y, shift);
   } else {
     qd_termprog* p = op->prog;
     spmv_matfree_kernel<<<grid8((n + 255) / 256), 256, 0, ctx->stream>>>(
@@ -305,11 +307,23 @@ extern "C" int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog,
   op->n = (int64_t)basis->sec.total;
   op->basis = basis;
   op->prog = prog;
+  op->kron = qd_kron_try_build(ctx, basis, prog);
   *out = op;
   return QD_OK;
 }
 
+// Matrix-free operators: 1 = use the two-block Kronecker kernels when the
+// program has that form (default), 0 = always the generic per-row kernel.
+// Returns whether the structured form exists through *has_structured.
+extern "C" int qd_linop_set_structured(qd_linop* op, int on, int* has_structured) {
+  if (!op) return QD_E_ARG;
+  op->use_kron = on ? 1 : 0;
+  if (has_structured) *has_structured = op->kron ? 1 : 0;
+  return QD_OK;
+}
+
 extern "C" int qd_linop_destroy(qd_linop* op) {
+  if (op && op->kron) qd_kron_destroy(op->kron);
   delete op;
   return QD_OK;
 }

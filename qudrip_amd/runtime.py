@@ -151,6 +151,12 @@ class LinOp:
     def set_lanes(self, lanes):
         self.ctx.check(self.ctx.lib.qd_linop_set_lanes(self.ptr, lanes), "set_lanes")
 
+    def set_structured(self, on):
+        has = ctypes.c_int()
+        self.ctx.check(self.ctx.lib.qd_linop_set_structured(self.ptr, 1 if on else 0, ctypes.byref(has)),
+                       "set_structured")
+        return bool(has.value)
+
     def lanes(self):
         return self.ctx.lib.qd_linop_get_lanes(self.ptr)
 
@@ -213,6 +219,9 @@ class Krylov:
         cnt = ctypes.c_int()
         self.ctx.check(self.ctx.lib.qd_krylov_eigenvalues(self.ptr, buf, cap, ctypes.byref(cnt)), "eigenvalues")
         return np.array(buf[:cnt.value])
+
+    def set_method(self, method):
+        self.ctx.check(self.ctx.lib.qd_krylov_set_method(self.ptr, method), "set_method")
 
     def reset(self):
         self.ctx.check(self.ctx.lib.qd_krylov_reset(self.ptr), "reset")

@@ -318,3 +318,89 @@ def test_arnoldi_evolve_matches_oracle(ctx):
         assert np.max(np.abs(back.cpu().numpy() - psi0)) < 1e-11
     finally:
         _close(kr, mf, op, c, p, b)
+
+
+@pytest.mark.parametrize("case", ["chain8_44", "sq3x4_33", "sq3x4_66", "chain6_nonherm", "phase_hops"])
+def test_matfree_structured_matches_oracle(ctx, case):
+    """Two-block Kronecker form of the matrix-free product (Hubbard layout)."""
+    if case == "chain8_44":
+        idx, ham = Q.hubbard(8, Q.hubbard_bonds_chain(8, False), 1.0, 4.0)
+        sec = Q.hubbard_sector(idx, 8, 4, 4)
+    elif case == "sq3x4_33":
+        idx, ham = Q.hubbard(12, Q.hubbard_bonds_square(3, 4), 1.0, 2.5)
+        sec = Q.hubbard_sector(idx, 12, 3, 3)
+    elif case == "sq3x4_66":
+        idx, ham = Q.hubbard(12, Q.hubbard_bonds_square(3, 4), 1.0, 4.0)
+        sec = Q.hubbard_sector(idx, 12, 6, 6)
+    elif case == "chain6_nonherm":  # right-moving hops only, complex amplitude: not Hermitian
+        idx = Q.getQbits(12)
+        g = Q.getHopFermiGate(idx)
+        ham = Q.getEmptyOperator()
+        for i in range(5):
+            for off in (0, 6):
+                ham = ham + (0.3 - 0.7j) * g(i + 1 + off, i + off)
+        for i in range(6):
+            ham = ham + 1.7 * (g(i, i) * g(i + 6, i + 6))
+        sec = Q.hubbard_sector(idx, 6, 3, 2)
+    else:  # Peierls phases and a chemical potential (bit gates on the down block)
+        idx = Q.getQbits(12)
+        g = Q.getHopFermiGate(idx)
+        nb = Q.getFermiGate(idx)
+        nb << np.diag([0.0, 1.0])
+        ph = np.exp(0.4j)
+        ham = Q.getEmptyOperator()
+        for (a, b) in Q.hubbard_bonds_chain(6, True):
+            for off in (0, 6):
+                ham = ham + (-ph) * g(a + off, b + off) + (-np.conj(ph)) * g(b + off, a + off)
+        for i in range(6):
+            ham = ham + 3.0 * (g(i, i) * g(i + 6, i + 6)) + (-0.4) * nb(i + 6)
+        sec = Q.hubbard_sector(idx, 6, 3, 3)
+    basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(basis, ham)
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, sec.nbits)
+    mf = LinOp.matfree(ctx, b, p)
+    try:
+        assert mf.set_structured(True)
+        x = oracle.seeded_state(len(basis), 17)
+        yr = oracle.spmv(ref, x)
+        xd = to_dev(ctx, x)
+        ys = mf.apply(xd).cpu().numpy()
+        mf.set_structured(False)
+        yg = mf.apply(xd).cpu().numpy()
+        scale = np.linalg.norm(yr)
+        assert np.linalg.norm(ys - yr) / scale < 1e-14
+        assert np.linalg.norm(yg - yr) / scale < 1e-14
+    finally:
+        _close(mf, p, b)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_reorth_methods_agree(ctx, method):
+    """CGS2 (4 passes) and delayed CGS2 (2 passes): same E0, same evolve, orthonormal basis."""
+    ham, sec = hub(8, Q.hubbard_bonds_chain(8, False), 4, 4, U=4.0)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(ref_basis, ham)
+    w = np.linalg.eigvalsh(oracle.csc_to_dense(ref))
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    kr = Krylov(ctx, b.size, 150, herm=True)
+    kr.set_method(method)
+    try:
+        psi = to_dev(ctx, oracle.seeded_state(b.size, 5))
+        e0 = kr.ground_state(op, psi)
+        assert abs(e0 - w[0]) / abs(w[0]) < 1e-10
+        assert abs(ctx.norm2(psi) - 1.0) < 1e-9
+        ev = kr.eigenvalues()
+        # full reorthogonalisation: no spurious copies of the ground state
+        assert np.sum(np.abs(ev - w[0]) < 1e-8) == 1
+        kr2 = Krylov(ctx, b.size, 30, herm=False)
+        kr2.set_method(method)
+        x0 = oracle.seeded_state(b.size, 6)
+        out = ctx.zeros(b.size)
+        kr2.evolve(op, to_dev(ctx, x0), out, 0.05, 30)
+        r = oracle.arnoldi_evolve(ref, 30, 0.05, x0)
+        assert np.max(np.abs(out.cpu().numpy() - r)) < 1e-11
+        kr2.close()
+    finally:
+        _close(kr, op, c, p, b)
