@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--krylov", type=int, default=150)
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quench", action="store_true", help="skip the Hubbard 4x4 quench workload")
+    ap.add_argument("--quench-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -140,10 +142,28 @@ def main():
             "k": args.krylov, "E0": e0, "E_ref": ref,
             "rel_err": (abs(e0 - ref) / abs(ref)) if ref else None,
             "solve_s": round(solve_s, 4), "solves_per_sec": round(1.0 / solve_s, 3),
-            "reorth": "CGS2 (full)",
+            "reorth": "delayed CGS2 (full)",
         }
         print(f"[bench] lanczos k={args.krylov}: E0={e0:.14f} solve {solve_s:.3f} s", file=sys.stderr, flush=True)
         kr.close()
+
+    if not args.no_quench:
+        from qudrip_amd import workloads as WL
+        log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+        q = WL.hubbard_quench(ctx, 4, 4, 5, 5, U0=4.0, U1=2.0, k_gs=120, k_rt=30, dt=0.05,
+                              steps=args.quench_steps, matfree=True, log=log)
+        d = q["docc"]
+        result["hubbard4x4_quench"] = {
+            "n": q["n"], "E0_U4": q["E0"], "gs_s": round(q["gs_s"], 3), "k_gs": 120,
+            "rt_steps": q["steps"], "k_rt": 30, "dt": 0.05, "rt_s": round(q["rt_s"], 3),
+            "rt_step_ms": round(q["rt_step_ms"], 2),
+            "docc_t": {f"{t:.2f}": round(d[int(round(t / 0.05))], 10) for t in (0.0, 2.5, 5.0, 7.5, 10.0)
+                       if int(round(t / 0.05)) < len(d)},
+            "energy_drift": abs(q["energy_end"] - q["energy_start"]), "norm_end": q["norm_end"],
+            "operator": "matrix-free (two-block Kronecker form)"}
+        log(f"quench: {q['steps']} steps in {q['rt_s']:.2f} s ({q['rt_step_ms']:.1f} ms/step), "
+            f"energy drift {abs(q['energy_end'] - q['energy_start']):.2e}")
+        del q
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(csr, n, nnz, B, x, args.cpu_seconds)
