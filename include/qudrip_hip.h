@@ -239,6 +239,37 @@ int qd_krylov_reset(qd_krylov* kr);
  * per step; default), 0 = CGS2 (four passes). */
 int qd_krylov_set_method(qd_krylov* kr, int method);
 
+/* ------------------------------------------------ batched trajectories */
+/* Quantum-jump trajectories of a spinful fermion model (up spins on bits
+ * 0..nsite-1, down spins on nsite..2nsite-1), the loop of
+ * programs/hubbard_diss.cpp:440-482 with mc_manager.hpp:88-128 averaging,
+ * generalised to dephasing (L = sqrt(rate) n_mode) and loss (L = sqrt(rate)
+ * c_mode, fermionic) channels; a loss moves a trajectory to the sector with
+ * one particle less.  All trajectories of one sector advance together
+ * (multi-RHS product, per-trajectory Arnoldi with kry_dim_rt, device-side
+ * jump sampling).  Random numbers: key = splitmix64(seed*0x9E3779B97F4A7C15 +
+ * traj), r_k(step) = (splitmix64(key + 4 step + k) >> 11) 2^-53. */
+enum { QD_JUMP_DEPHASE = 0, QD_JUMP_LOSS = 1 };
+typedef struct {
+  int32_t kind;
+  int32_t mode;
+  double rate;
+} qd_jump;
+typedef struct {
+  int64_t jumps;
+  int64_t losses;
+  int32_t sectors;
+} qd_traj_stats;
+/* H: term program over 2*nsite bits conserving each spin's particle number.
+ * d_psi0: start state in sector (nup0, ndn0) (device, ascending basis order).
+ * h_dens_mean[(nsteps+1) * 2nsite]: <n_mode>(t_i) averaged over trajectories;
+ * h_dens_var (optional): their variance over trajectories; h_jump_log
+ * (optional, ntraj * nsteps): channel taken per step, -1 for none. */
+int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nterms, const qd_op* ops, int nops, int nup0,
+                int ndn0, const double* d_psi0, const qd_jump* jumps, int njumps, int ntraj, int nsteps,
+                double dt, int kry_dim_rt, uint64_t seed, double* h_dens_mean, double* h_dens_var,
+                int* h_jump_log, qd_traj_stats* stats);
+
 /* ------------------------------------------------- FP64 host numerics */
 /* The small dense problems of the solvers, exposed for direct checks.
  * Symmetric tridiagonal eigenpairs (d[m], e[m-1]) -> w[m] ascending and,

@@ -62,6 +62,12 @@ def lib():
     L.orc_lanczos_gs.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP, _VP]
     L.orc_expm.restype = ctypes.c_int
     L.orc_expm.argtypes = [ctypes.c_int, _VP, _VP]
+    L.orc_traj_run.restype = ctypes.c_int
+    L.orc_traj_run.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP, _VP,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_uint64, _VP, _VP]
+    L.orc_traj_uniform.restype = ctypes.c_double
+    L.orc_traj_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
     L.orc_arnoldi_evolve.restype = ctypes.c_int
     L.orc_arnoldi_evolve.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, _VP]
     _lib = L
@@ -222,3 +228,21 @@ def csc_to_dense(csc):
         for e in range(colptr[j], colptr[j + 1]):
             M[rowidx[e], j] += v[e]
     return M
+
+
+def traj_run(opsum, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, seed, traj_offset=0):
+    """Quantum-jump loop (hubbard_diss.cpp:440-482) for trajectories traj_offset.. ; returns
+    (sum of densities [(nsteps+1), 2 nsite], jump log [ntraj, nsteps])."""
+    terms, nt, ops, no = opsum.program()
+    M = 2 * nsite
+    jk = np.array([c[0] for c in channels], dtype=np.int32)
+    jm = np.array([c[1] for c in channels], dtype=np.int32)
+    jr = np.array([c[2] for c in channels], dtype=np.float64)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    dsum = np.zeros((nsteps + 1) * M)
+    log = np.zeros(ntraj * max(nsteps, 1), dtype=np.int32)
+    rc = lib().orc_traj_run(nsite, terms, nt, ops, nup0, ndn0, _p(psi0), _p(jk), _p(jm), _p(jr), len(channels),
+                            ntraj, traj_offset, nsteps, dt, k_rt, seed, _p(dsum), _p(log))
+    if rc != 0:
+        raise RuntimeError("oracle trajectories failed")
+    return dsum.reshape(nsteps + 1, M), log.reshape(ntraj, max(nsteps, 1))

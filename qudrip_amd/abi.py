@@ -34,6 +34,18 @@ class qd_term(ctypes.Structure):
     _fields_ = [("coeff", ctypes.c_double * 2), ("op_begin", ctypes.c_int32), ("n_ops", ctypes.c_int32)]
 
 
+class qd_jump(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("mode", ctypes.c_int32), ("rate", ctypes.c_double)]
+
+
+class qd_traj_stats(ctypes.Structure):
+    _fields_ = [("jumps", ctypes.c_int64), ("losses", ctypes.c_int64), ("sectors", ctypes.c_int32)]
+
+
+QD_JUMP_DEPHASE = 0
+QD_JUMP_LOSS = 1
+
+
 class qd_krylov_info(ctypes.Structure):
     _fields_ = [("act_dim", ctypes.c_int), ("breakdown", ctypes.c_int), ("error", ctypes.c_double),
                 ("beta_last", ctypes.c_double), ("n_spmv", ctypes.c_int64)]
@@ -93,6 +105,9 @@ SIGNATURES = {
     "qd_krylov_converge_ev": (I, [VP, I, ctypes.c_double, ctypes.c_double]),
     "qd_krylov_reset": (I, [VP]),
     "qd_krylov_set_method": (I, [VP, I]),
+    "qd_traj_run": (I, [VP, I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, I, I, VP,
+                        ctypes.POINTER(qd_jump), I, I, I, ctypes.c_double, I, ctypes.c_uint64, VP, VP, VP,
+                        ctypes.POINTER(qd_traj_stats)]),
     "qd_host_tridiag_eig": (I, [I, DP, DP, DP, DP]),
     "qd_host_expm": (I, [I, DP, DP]),
 }

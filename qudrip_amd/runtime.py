@@ -230,3 +230,36 @@ class Krylov:
         if self.ptr:
             self.ctx.lib.qd_krylov_destroy(self.ptr)
             self.ptr = None
+
+
+def jump_channels(nsite, dephase_rate, loss_rate):
+    """Dephasing sqrt(g) n_m and loss sqrt(g) c_m on every mode m (0 rates omitted)."""
+    ch = []
+    for m in range(2 * nsite):
+        if dephase_rate > 0:
+            ch.append((abi.QD_JUMP_DEPHASE, m, dephase_rate))
+        if loss_rate > 0:
+            ch.append((abi.QD_JUMP_LOSS, m, loss_rate))
+    return ch
+
+
+def traj_run(ctx, opsum, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, seed, want_log=False):
+    """Batched quantum-jump trajectories (qd_traj_run); returns mean/var densities [(nsteps+1), 2 nsite]."""
+    terms, nt, ops, no = opsum.program()
+    jarr = (abi.qd_jump * max(1, len(channels)))()
+    for i, (k, m, g) in enumerate(channels):
+        jarr[i].kind, jarr[i].mode, jarr[i].rate = k, m, g
+    M = 2 * nsite
+    mean = np.zeros((nsteps + 1) * M)
+    var = np.zeros((nsteps + 1) * M)
+    log = np.zeros(ntraj * max(nsteps, 1), dtype=np.int32) if want_log else None
+    st = abi.qd_traj_stats()
+    ctx.check(ctx.lib.qd_traj_run(ctx.ptr, nsite, terms, nt, ops, no, nup0, ndn0, ctypes.c_void_p(psi0.data_ptr()),
+                                  jarr, len(channels), ntraj, nsteps, float(dt), k_rt, seed, mean.ctypes.data,
+                                  var.ctypes.data, log.ctypes.data if log is not None else None,
+                                  ctypes.byref(st)), "traj_run")
+    out = {"mean": mean.reshape(nsteps + 1, M), "var": var.reshape(nsteps + 1, M),
+           "jumps": st.jumps, "losses": st.losses, "sectors": st.sectors}
+    if log is not None:
+        out["log"] = log.reshape(ntraj, max(nsteps, 1))
+    return out

@@ -1,0 +1,71 @@
+"""Batched quantum-jump trajectories on the GPU (qd_traj_run) against the
+oracle's restatement on the same per-trajectory random streams."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from qudrip_amd import operators as Q
+from qudrip_amd.runtime import jump_channels, traj_run
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(L, U=4.0):
+    _, ham = Q.hubbard(L, Q.hubbard_bonds_chain(L, False), 1.0, U)
+    return ham
+
+
+def _dev(v):
+    return torch.from_numpy(np.ascontiguousarray(v)).to("cuda")
+
+
+def test_traj_matches_oracle_small(ctx):
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 13)
+    ch = jump_channels(L, 0.5, 0.3)
+    ntraj, nsteps, dt, k = 150, 60, 0.05, 10
+    print(f"traj L={L} ntraj={ntraj} nsteps={nsteps}", flush=True)
+    g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 5, want_log=True)
+    dsum, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 5)
+    assert (g["log"] == log).all()
+    assert g["losses"] > 20 and g["jumps"] > g["losses"] and g["sectors"] > 3
+    np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
+    assert (g["var"] > -1e-12).all()
+
+
+def test_traj_result_independent_of_batch(ctx):
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 14)
+    ch = jump_channels(L, 0.5, 0.3)
+    args = (40, 0.05, 10, 21)
+    big = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, 130, *args, want_log=True)
+    small = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, 37, *args, want_log=True)
+    one = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, 1, *args, want_log=True)
+    assert (small["log"] == big["log"][:37]).all()
+    assert (one["log"] == big["log"][:1]).all()
+    d1, _ = oracle.traj_run(ham, L, nup, ndn, psi0, ch, 1, *args)
+    np.testing.assert_allclose(one["mean"], d1, rtol=0, atol=1e-10)
+
+
+def test_traj_full_size_closed_form_and_oracle_prefix(ctx):
+    """L=8 OBC, start (4,4) dim 4900, dephasing 0.1 and loss 0.05 on all 16
+    modes, T=10: E[N(T)] = 8 (1 - 0.05 dt)^200, and the first trajectories'
+    jump sequences equal the oracle's."""
+    L, nup, ndn = 8, 4, 4
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(4900, 15)
+    ch = jump_channels(L, 0.1, 0.05)
+    ntraj, nsteps, dt, k = 512, 200, 0.05, 16
+    print(f"traj L=8 ntraj={ntraj} nsteps={nsteps}", flush=True)
+    g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 5, want_log=True)
+    is_loss = np.array([c[0] == 1 for c in ch] + [False])
+    N = nup + ndn - is_loss[g["log"]].sum(axis=1)
+    assert g["losses"] == (nup + ndn) * ntraj - N.sum()
+    np.testing.assert_allclose(g["mean"][-1].sum(), N.mean(), rtol=0, atol=1e-9)
+    expect = (nup + ndn) * (1 - 0.05 * dt) ** nsteps
+    assert abs(N.mean() - expect) < 5 * N.std() / np.sqrt(ntraj)
+    _, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, 4, nsteps, dt, k, 5)
+    assert (g["log"][:4] == log).all()
