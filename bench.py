@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quench", action="store_true", help="skip the Hubbard 4x4 quench workload")
     ap.add_argument("--quench-steps", type=int, default=200)
+    ap.add_argument("--no-traj", action="store_true", help="skip the dissipative trajectory workload")
+    ap.add_argument("--traj", type=int, default=4096, help="number of trajectories")
+    ap.add_argument("--traj-cpu", type=int, default=8, help="trajectories in the oracle's CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -165,10 +168,50 @@ def main():
             f"energy drift {abs(q['energy_end'] - q['energy_start']):.2e}")
         del q
 
+    if not args.no_traj:
+        result["hubbard_l8_trajectories"] = trajectories(ctx, args)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(csr, n, nnz, B, x, args.cpu_seconds)
 
     print(json.dumps(result), flush=True)
+
+
+def trajectories(ctx, args):
+    """Dissipative L=8 Hubbard chain, 4096 trajectories to T=10 (dt=0.05).
+    Checks: E[N(T)] = 8 (1 - 0.05 dt)^200 within 5 sigma, and the jump
+    sequences of the first trajectories equal the oracle's on the same random
+    streams (timed as the CPU sample, single thread, precomputed operators)."""
+    import numpy as np
+
+    from qudrip_amd import workloads as WL
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+    r = WL.hubbard_trajectories(ctx, ntraj=args.traj, want_log=True, log=log)
+    ch, nsteps = r["channels"], r["nsteps"]
+    is_loss = np.array([c[0] == 1 for c in ch] + [False])
+    N = 8 - is_loss[r["log"]].sum(axis=1)
+    expect = 8 * (1 - 0.05 * 0.05) ** nsteps
+    sigma = N.std() / np.sqrt(len(N))
+    out = {"L": 8, "n_start": r["n"], "ntraj": args.traj, "steps": nsteps, "dt": 0.05, "k_rt": 13,
+           "run_s": round(r["run_s"], 3), "traj_steps_per_s": round(args.traj * nsteps / r["run_s"], 1),
+           "jumps": r["jumps"], "losses": r["losses"], "sectors": r["sectors"],
+           "N_T": round(float(N.mean()), 5), "N_T_closed_form": round(expect, 5), "N_T_sigma": round(float(sigma), 5),
+           "closed_form_z": round(abs(N.mean() - expect) / sigma, 3),
+           "density_site0_up_T": round(float(r["mean"][-1][0]), 6),
+           "density_site0_up_T_sigma": round(float(np.sqrt(max(r["var"][-1][0], 0.0) / args.traj)), 6)}
+    if args.traj_cpu > 0 and not args.no_cpu:
+        import oracle
+        psi0 = r["psi0"].cpu().numpy()
+        t0 = time.perf_counter()
+        _, olog = oracle.traj_run(r["ham"], 8, 4, 4, psi0, ch, args.traj_cpu, nsteps, 0.05, 13, 5)
+        cpu_s = time.perf_counter() - t0
+        out["oracle_prefix_logs_equal"] = bool((olog == r["log"][:args.traj_cpu]).all())
+        out["cpu_baseline"] = {"value": round(args.traj_cpu * nsteps / cpu_s, 2), "unit": "trajectory-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{args.traj_cpu} trajectories x {nsteps} steps in {cpu_s:.1f} s, oracle "
+                                         "restatement with operators precomputed per sector, 1 thread"}
+    log(f"trajectories: N(T) = {out['N_T']} vs {out['N_T_closed_form']} (z={out['closed_form_z']})")
+    return out
 
 
 def cpu_baseline(csr, n, nnz, B, x, budget_s):

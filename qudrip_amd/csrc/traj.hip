@@ -1,19 +1,30 @@
-// traj.hip -- K11: batched quantum-jump trajectories (programs/hubbard_diss.cpp:440-482,
+// traj.hip -- batched quantum-jump trajectories (programs/hubbard_diss.cpp:440-482,
 // mc_manager.hpp:88-128), generalised to dephasing and loss channels with
 // sector migration.
 //
 // Trajectories live in per-sector blocks, trajectory-minor: X[s * ld + t] is
-// state s of the trajectory in slot t (slots 0..m-1 packed).  All blocks of
-// one sector share its Hamiltonian, so the multi-RHS product reads each row
-// of H once for every trajectory of the sector, and a wave's 64 lanes read 64
-// consecutive trajectories (1 KB) per stored entry.  Each trajectory runs its
-// own Arnoldi step (the reference's evolve with H_eff, hubbard_diss.cpp:467-469)
-// with CGS2, per-trajectory Hessenberg and a per-trajectory exponential on the
-// device.  Jumps are drawn on the device from a counter-based stream per
-// trajectory, applied from the pre-step state into a staging area, and
-// committed after the sector's evolution (moving lost trajectories to their
-// new sector).  Every reduction has a fixed order, so a trajectory's result
-// does not depend on which other trajectories share its batch.
+// state s of the trajectory in slot t (slots 0..m-1 packed).  All slots of one
+// sector share its Hamiltonian, so the multi-RHS product reads each row of H
+// once for every trajectory of the sector, and a wave's 64 lanes read 64
+// consecutive trajectories (1 KB) per stored entry.
+//
+// Every phase of a time step runs as ONE grouped launch over all occupied
+// sectors: a small table (GSec) gives each sector's rows, slots, pool offsets
+// and tile ranges, and each workgroup finds its sector by binary search.  The
+// sectors become uneven as trajectories lose particles (4,900 rows down to 1),
+// so grouping keeps the small ones from running latency-bound and one at a
+// time.  The reduction tile (rows per chunk) of each sector follows an
+// occupancy target that a short timed search picks at the start of a run and
+// again once the trajectories have spread over smaller sectors.
+//
+// Each trajectory runs its own Arnoldi step (the reference's evolve with
+// H_eff, hubbard_diss.cpp:467-469; sparse_algorithm.hpp:548-562) with CGS2
+// fused into three passes per Krylov vector, a per-trajectory Hessenberg and a
+// per-trajectory exponential on the device.  Jumps are drawn on the device from
+// a counter-based stream per trajectory, applied from the pre-step state into
+// a staging area, and committed after the evolution (moving lost trajectories
+// to their new sector).  All reductions have a fixed order for a given tile
+// choice; a different choice moves sums by rounding only.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -22,9 +33,9 @@
 
 namespace {
 
-constexpr int TT = 64;      // trajectories per wave
-constexpr int MAXM = 32;    // modes (2 * nsite) supported
-constexpr int MAXK = 32;    // Krylov dimension limit for the batched step
+constexpr int TT = 64;    // trajectories per wave
+constexpr int MAXM = 32;  // modes (2 * nsite) supported
+constexpr int MAXK = 32;  // Krylov dimension limit for the batched step
 
 __device__ inline uint64_t tsm64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -35,6 +46,41 @@ __device__ inline uint64_t tsm64(uint64_t z) {
 __device__ inline double traj_uniform(uint64_t seed, int64_t traj, int64_t step, int k) {
   uint64_t key = tsm64(seed * 0x9E3779B97F4A7C15ull + (uint64_t)traj);
   return (double)(tsm64(key + 4ull * (uint64_t)step + (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// One occupied sector in a grouped launch.
+struct GSec {
+  int64_t n;      // rows (sector dimension)
+  int m;          // active slots
+  int ldx;        // leading dimension of X
+  int rpc, nch;   // rows per reduction chunk, chunks
+  int groups;     // ceil(m / 64)
+  int so;         // first global slot
+  int64_t vo;     // offset of the sector's n x m block in the V / W pools
+  int64_t co;     // sum of nch * m over the earlier sectors (partials unit)
+  int64_t tile0;  // first reduction tile (tiles = nch * groups)
+  int64_t blk0;   // first product block (blocks = ceil(n / SPMM_RB) * groups)
+  int64_t grp0;   // first 64-slot group (sum of groups over the earlier sectors)
+  cplx* X;
+  const int32_t* outer;
+  const int32_t* inner;
+  const cplx* vals;
+  const double* D;
+  const uint64_t* states;
+};
+
+// index of the last sector whose key <= v (keys ascending, first key 0)
+template <typename F>
+__device__ inline int find_sec(int ns, int64_t v, F key) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -50,222 +96,327 @@ __global__ void diag_rates_kernel(int64_t n, const uint64_t* __restrict__ states
   D[s] = d;
 }
 
-// column partials over row chunks.  Grid: (ceil(m/64), nchunk).
-// part[(chunk * nq + q) * m + t]
-// mode 0: q < j+1: conj(V_q[s,t]) W[s,t]          (Gram-Schmidt dots)
-// mode 1: q = 0: |X[s,t]|^2                        (norms)
+// Reduction tile -> (sector, chunk, slot of this lane); one wave per tile.
+struct Tile {
+  int sec, chunk, t;
+};
+__device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
+  Tile r;
+  const int64_t b = blockIdx.x;
+  r.sec = find_sec(ns, b, [&](int i) { return gs[i].tile0; });
+  const GSec& S = gs[r.sec];
+  const int64_t local = b - S.tile0;
+  r.chunk = (int)(local / S.groups);
+  r.t = (int)(local % S.groups) * TT + threadIdx.x;
+  return r;
+}
+
+// One pass over a row chunk of NQ basis vectors and W for every sector
+// (trajectory-minor blocks in the V / W pools, lds = m).
+//   MODE 0: part_q = sum_s conj(V_q[s,t]) W[s,t]                  (first Gram-Schmidt pass)
+//   MODE 1: W <- W - sum_q V_q h_q, then part_q as in MODE 0      (update + second pass)
+//   MODE 2: W <- W - sum_q V_q h_q, then part_0 = sum_s |W[s,t]|^2 (update + norm)
+// part[P co + (chunk P + q) m + t], P = NQ (modes 0, 1) or 1 (mode 2); h[q Mtot + slot].
+// NQ is a template parameter so the per-lane accumulators stay in registers.
+template <int NQ, int MODE>
 __global__ void __launch_bounds__(TT)
-    col_dots_kernel(int64_t n, int m, int ldv, const cplx* __restrict__ V, int64_t vstride, int nq,
-                    const cplx* __restrict__ W, int rows_per_chunk, cplx* __restrict__ part) {
-  const int t = blockIdx.x * TT + threadIdx.x;
-  const int chunk = blockIdx.y;
+    g_pass_kernel(const GSec* __restrict__ gs, int ns, const cplx* __restrict__ V, int64_t VS, cplx* __restrict__ W,
+                  const cplx* __restrict__ h, int Mtot, cplx* __restrict__ part) {
+  constexpr int NP = MODE == 2 ? 1 : NQ;
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
   if (t >= m) return;
-  const int64_t r0 = (int64_t)chunk * rows_per_chunk;
-  const int64_t r1 = min<int64_t>(n, r0 + rows_per_chunk);
-  cplx acc[MAXK + 1];
-  for (int q = 0; q < nq; ++q) acc[q] = cmake(0.0, 0.0);
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const cplx* __restrict__ Vb = V + S.vo;
+  cplx* __restrict__ Wb = W + S.vo;
+  cplx hc[NQ];
+  if (MODE != 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) hc[q] = h[(int64_t)q * Mtot + S.so + t];
+  }
+  cplx acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = cmake(0.0, 0.0);
+#pragma unroll 2
   for (int64_t s = r0; s < r1; ++s) {
-    const cplx w = W[s * ldv + t];
-    for (int q = 0; q < nq; ++q) {
-      const cplx v = V[q * vstride + s * ldv + t];
-      acc[q] = cadd(acc[q], cdotmul(v, w));
+    const int64_t o = s * m + t;
+    cplx v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = Vb[q * VS + o];
+    cplx w = Wb[o];
+    if (MODE != 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        w.x -= v[q].x * hc[q].x - v[q].y * hc[q].y;
+        w.y -= v[q].x * hc[q].y + v[q].y * hc[q].x;
+      }
+      Wb[o] = w;
+    }
+    if (MODE == 2) {
+      acc[0].x += w.x * w.x + w.y * w.y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = cadd(acc[q], cdotmul(v[q], w));
     }
   }
-  for (int q = 0; q < nq; ++q) part[((int64_t)chunk * nq + q) * m + t] = acc[q];
+  cplx* pb = part + NP * S.co;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) pb[((int64_t)tl.chunk * NP + q) * m + t] = acc[q];
 }
 
-// out[q * m + t] = sum_chunk part[(chunk * nq + q) * m + t]  (fixed order)
-__global__ void col_reduce_kernel(int m, int nq, int nchunk, const cplx* __restrict__ part, cplx* __restrict__ out) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= m * nq) return;
-  const int q = idx / m, t = idx % m;
-  cplx s = cmake(0.0, 0.0);
-  for (int c = 0; c < nchunk; ++c) s = cadd(s, part[((int64_t)c * nq + q) * m + t]);
-  out[q * m + t] = s;
-}
-
-// W[s,t] -= sum_q V_q[s,t] h[q*m+t]; Hess column j accumulates h
+// Sums the chunk partials of every slot (fixed order) and files them:
+//   RMODE 0: h = sum, Hess[0..nq-1][j] = sum;  RMODE 1: h = sum, Hess[..][j] += sum;
+//   RMODE 2: beta = sqrt(sum), Hess[j+1][j] = beta, scale = 1/beta (0 below 1e-10).
+// One 256-thread block per (q, 64-slot group): wave w sums chunks w, w+4, ...,
+// then the four wave sums are added in wave order.  Hess[(j (K+1) + i) Mtot + slot].
 __global__ void __launch_bounds__(256)
-    col_update_kernel(int64_t n, int m, int ldv, const cplx* __restrict__ V, int64_t vstride, int nq,
-                      const cplx* __restrict__ h, cplx* __restrict__ W) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int t = (int)(idx % ldv);
-  const int64_t s = idx / ldv;
-  if (s >= n || t >= m) return;
-  cplx w = W[s * ldv + t];
-  for (int q = 0; q < nq; ++q) {
-    const cplx v = V[q * vstride + s * ldv + t];
-    const cplx c = h[q * m + t];
-    w.x -= v.x * c.x - v.y * c.y;
-    w.y -= v.x * c.y + v.y * c.x;
+    g_reduce_kernel(const GSec* __restrict__ gs, int ns, int64_t ngroups, int Mtot, int nq,
+                    const cplx* __restrict__ part, cplx* __restrict__ h, cplx* __restrict__ hess, int j, int K,
+                    int rmode, double* __restrict__ scale) {
+  __shared__ cplx sh[4][TT];
+  const int q = (int)(blockIdx.x / ngroups);
+  const int64_t gi = blockIdx.x % ngroups;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  cplx s = cmake(0.0, 0.0);
+  if (t < S.m) {
+    const cplx* pb = part + (int64_t)nq * S.co;
+    for (int c = wv; c < S.nch; c += 4) s = cadd(s, pb[((int64_t)c * nq + q) * S.m + t]);
   }
-  W[s * ldv + t] = w;
+  sh[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  s = cadd(cadd(sh[0][lane], sh[1][lane]), cadd(sh[2][lane], sh[3][lane]));
+  const int g = S.so + t;
+  if (rmode == 2) {
+    const double b = sqrt(s.x);
+    hess[((int64_t)(j * (K + 1) + j + 1)) * Mtot + g] = cmake(b, 0.0);
+    scale[g] = (b >= 1e-10) ? 1.0 / b : 0.0;
+    return;
+  }
+  h[(int64_t)q * Mtot + g] = s;
+  cplx* p = hess + ((int64_t)(j * (K + 1) + q)) * Mtot + g;
+  *p = rmode ? cadd(*p, s) : s;
 }
 
-// Hess[(j * (K+1) + i) * m + t] (+)= h[i * m + t] for i <= j
-__global__ void hess_acc_kernel(int m, int j, int K, const cplx* __restrict__ h, cplx* __restrict__ hess, int acc) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= m * (j + 1)) return;
-  const int i = idx / m, t = idx % m;
-  cplx* p = hess + ((int64_t)(j * (K + 1) + i)) * m + t;
-  *p = acc ? cadd(*p, h[i * m + t]) : h[i * m + t];
-}
-
-// beta_t = sqrt(nrm[t]); Hess[j+1][j][t] = beta; scale[t] = 1/beta (0 on breakdown)
-__global__ void beta_kernel(int m, int j, int K, const cplx* __restrict__ nrm, cplx* __restrict__ hess,
-                            double* __restrict__ scale) {
+// per global slot: Gamma_t and 1/|psi_t| from the recorded densities and norm^2
+__global__ void slot_params_kernel(int Mtot, int M, const int* __restrict__ slot2traj, const double* __restrict__ gtraj,
+                                   const double* __restrict__ dens_all, double* __restrict__ g,
+                                   double* __restrict__ scale) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  const double b = sqrt(nrm[t].x);
-  hess[((int64_t)(j * (K + 1) + j + 1)) * m + t] = cmake(b, 0.0);
-  scale[t] = (b >= 1e-10) ? 1.0 / b : 0.0;
+  if (t >= Mtot) return;
+  const int tr = slot2traj[t];
+  g[t] = gtraj[tr];
+  const double n2 = dens_all[(int64_t)tr * (M + 1) + M];
+  scale[t] = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
 }
 
-// scale[t] = 1 / sqrt(nrm[t]) (0 for a zero column)
-__global__ void inv_norm_kernel(int m, const cplx* __restrict__ nrm, double* __restrict__ scale) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  const double b = sqrt(nrm[t].x);
-  scale[t] = (b > 0.0) ? 1.0 / b : 0.0;
+// Multi-RHS product with the normalisation of the source folded in, all sectors:
+//   Vout[s,t] = f_t src[s,t],   Wout[s,t] = (H_eff,t Vout)[s,t]
+//   H_eff,t = H - i/2 (D - Gamma_t),  H from the ColMajor arrays of a Hermitian
+//   operator (pull form: row s of H is column s conjugated).
+// src is the sector's X (from_x) or the W pool.  One wave per (SPMM_RB rows,
+// 64 slots): the wave loads the rows' pointers and up to 64 entries of a row
+// with one vector load each and broadcasts them with v_readlane, so all of a
+// row's gathers (64 consecutive trajectories each) are in flight together.
+//
+// Block order: workgroups are dealt to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8; it is given work item (b % 8) * (nb / 8) + b / 8, which
+// hands every XCD a contiguous range of (sector, group, rows) items.  Within a
+// sector the items run group-major (all rows of one 64-slot group, then the
+// next), so the rows an XCD gathers from belong to the one or two groups it is
+// working on and stay in its L2.
+constexpr int SPMM_RB = 4;
+__device__ inline double readlane_d(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
 }
-
-// dst[s,t] = src[s,t] * scale[t]
-__global__ void col_scale_kernel(int64_t n, int m, int lds, const cplx* __restrict__ src, int ldd, cplx* __restrict__ dst,
-                                 const double* __restrict__ scale) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int t = (int)(idx % m);
-  const int64_t s = idx / m;
-  if (s >= n) return;
-  const cplx v = src[s * lds + t];
-  const double f = scale[t];
-  dst[s * ldd + t] = cmake(v.x * f, v.y * f);
-}
-
-// multi-RHS product with the per-trajectory non-Hermitian shift:
-// W[s,t] = sum_e conj(val_e) V[col_e,t] - i/2 (D[s] - Gamma_t) V[s,t]
-// One wave per (row s, 64-trajectory group): the row's entries are uniform
-// across the wave (scalar loads), the gathers are 1 KB coalesced.
 __global__ void __launch_bounds__(TT)
-    spmm_tm_kernel(int64_t n, int m, int ldv, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
-                   const cplx* __restrict__ vals, const double* __restrict__ D, const double* __restrict__ gamma,
-                   const cplx* __restrict__ V, cplx* __restrict__ W) {
-  const int64_t s = blockIdx.x;
-  const int t = blockIdx.y * TT + threadIdx.x;
-  if (t >= m) return;
-  double sx = 0.0, sy = 0.0;
-  const int32_t e0 = outer[s], e1 = outer[s + 1];
-  for (int32_t e = e0; e < e1; ++e) {
-    const cplx v = vals[e];
-    const cplx x = V[(int64_t)inner[e] * ldv + t];
-    sx += v.x * x.x + v.y * x.y;
-    sy += v.x * x.y - v.y * x.x;
+    g_spmm_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
+                  const double* __restrict__ f, const double* __restrict__ gamma, cplx* __restrict__ Vout,
+                  cplx* __restrict__ Wout) {
+  const int64_t per_xcd = (int64_t)gridDim.x / 8;
+  const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (b >= nblocks) return;
+  const int si = find_sec(ns, b, [&](int i) { return gs[i].blk0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x;
+  const int64_t nrb = (S.n + SPMM_RB - 1) / SPMM_RB;
+  const int64_t local = b - S.blk0;
+  const int64_t s0 = (local % nrb) * SPMM_RB;
+  const int nr = (int)min<int64_t>(SPMM_RB, S.n - s0);
+  const int t = (int)(local / nrb) * TT + lane;
+  const int m = S.m;
+  const bool active = t < m;
+  const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
+  const int64_t lds = from_x ? S.ldx : m;
+  const int g = S.so + (active ? t : 0);
+  const double ft = active ? f[g] : 0.0;
+  const double gam = active ? gamma[g] : 0.0;
+  const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
+  for (int r = 0; r < nr; ++r) {
+    const int64_t s = s0 + r;
+    const int e0 = __builtin_amdgcn_readlane(optr, r);
+    const int e1 = __builtin_amdgcn_readlane(optr, r + 1);
+    double sx = 0.0, sy = 0.0;
+    for (int base = e0; base < e1; base += TT) {
+      const int cnt = min(TT, e1 - base);
+      const int ci = (lane < cnt) ? S.inner[base + lane] : 0;
+      const cplx cv = (lane < cnt) ? S.vals[base + lane] : cmake(0.0, 0.0);
+      // batches of 8 entries: all 8 gathers are issued before the sums; the
+      // padding entries of the last batch repeat a valid column with weight 0
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        cplx x[8];
+        double vx[8], vy[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = min(k0 + i, cnt - 1);
+          const int col = __builtin_amdgcn_readlane(ci, k);
+          const bool real = k0 + i < cnt;
+          vx[i] = real ? readlane_d(cv.x, k) : 0.0;
+          vy[i] = real ? readlane_d(cv.y, k) : 0.0;
+          x[i] = active ? src[(int64_t)col * lds + t] : cmake(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          sx += vx[i] * x[i].x + vy[i] * x[i].y;
+          sy += vx[i] * x[i].y - vy[i] * x[i].x;
+        }
+      }
+    }
+    if (active) {
+      const cplx x0 = src[s * lds + t];
+      const cplx x = cmake(x0.x * ft, x0.y * ft);
+      const double d = -0.5 * (S.D[s] - gam);  // (0, d) * x
+      Vout[S.vo + s * m + t] = x;
+      Wout[S.vo + s * m + t] = cmake(sx * ft - d * x.y, sy * ft + d * x.x);
+    }
   }
-  const cplx x = V[s * ldv + t];
-  const double d = -0.5 * (D[s] - gamma[t]);  // (0, d) * x
-  W[s * ldv + t] = cmake(sx - d * x.y, sy + d * x.x);
 }
 
 // y_t = expm(-i dt K_t) e_1 for the k x k Hessenberg of each trajectory:
 // scaling and squaring with a truncated Taylor series applied to vectors
 // (Moler & Van Loan, "Nineteen dubious ways to compute the exponential of a
-// matrix", methods 1 and 3).  One thread per trajectory.
-__global__ void small_expm_kernel(int m, int k, int K, const cplx* __restrict__ hess, double dt, cplx* __restrict__ coef) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  auto Hget = [&](int i, int j) -> cplx { return hess[((int64_t)(j * (K + 1) + i)) * m + t]; };
-  // 1-norm of A = -i dt K (upper Hessenberg: rows 0 .. j+1 of column j)
-  double nrm = 0.0;
-  for (int j = 0; j < k; ++j) {
-    double c = 0.0;
-    for (int i = 0; i <= min(j + 1, k - 1); ++i) {
-      cplx h = Hget(i, j);
-      c += sqrt(h.x * h.x + h.y * h.y);
-    }
-    nrm = fmax(nrm, c * fabs(dt));
+// matrix", methods 1 and 3): A = -i dt K / 2^s with |A|_1 <= 1/2, then
+// y <- T_16(A) y, 2^s times (remainder below 1e-19 relative).
+// One wave per trajectory; lane i owns row i, the matrix and the current
+// Taylor term live in LDS.
+__global__ void __launch_bounds__(64)
+    small_expm_kernel(int m, int k, int K, const cplx* __restrict__ hess, double dt, cplx* __restrict__ coef) {
+  __shared__ cplx Hs[MAXK][MAXK + 1];
+  __shared__ cplx tv[MAXK];
+  __shared__ double cn[MAXK];
+  const int t = blockIdx.x;
+  const int i = threadIdx.x;
+  for (int e = i; e < k * k; e += 64) {
+    const int r = e % k, c = e / k;
+    Hs[r][c] = (r <= c + 1) ? hess[((int64_t)(c * (K + 1) + r)) * m + t] : cmake(0.0, 0.0);
   }
+  __syncthreads();
+  if (i < k) {
+    double c = 0.0;
+    for (int r = 0; r <= min(i + 1, k - 1); ++r) c += sqrt(Hs[r][i].x * Hs[r][i].x + Hs[r][i].y * Hs[r][i].y);
+    cn[i] = c * fabs(dt);
+  }
+  __syncthreads();
+  double nrm = 0.0;
+  for (int c = 0; c < k; ++c) nrm = fmax(nrm, cn[c]);
   int sq = 0;
   while (nrm > 0.5 && sq < 16) {
     nrm *= 0.5;
     ++sq;
   }
   const double sc = ldexp(dt, -sq);
-  cplx y[MAXK], term[MAXK], nt[MAXK];
-  for (int i = 0; i < k; ++i) y[i] = cmake(i == 0 ? 1.0 : 0.0, 0.0);
+  cplx y = cmake(i == 0 ? 1.0 : 0.0, 0.0);
   const int reps = 1 << sq;
-  for (int r = 0; r < reps; ++r) {
-    for (int i = 0; i < k; ++i) term[i] = y[i];
-    for (int p = 1; p <= 24; ++p) {
-      // nt = (-i sc K) term / p
-      double tn = 0.0;
-      for (int i = 0; i < k; ++i) {
-        cplx a = cmake(0.0, 0.0);
-        for (int j = (i > 0 ? i - 1 : 0); j < k; ++j) a = cadd(a, cmul(Hget(i, j), term[j]));
-        // -i * sc * a / p
-        nt[i] = cmake(sc * a.y / p, -sc * a.x / p);
-      }
-      for (int i = 0; i < k; ++i) {
-        term[i] = nt[i];
-        y[i] = cadd(y[i], nt[i]);
-        tn += fabs(nt[i].x) + fabs(nt[i].y);
-      }
-      if (tn < 1e-18) break;
+  for (int rep = 0; rep < reps; ++rep) {
+    cplx term = y;
+    for (int p = 1; p <= 16; ++p) {
+      __syncthreads();
+      if (i < k) tv[i] = term;
+      __syncthreads();
+      cplx a = cmake(0.0, 0.0);
+      if (i < k)
+        for (int j = (i > 0 ? i - 1 : 0); j < k; ++j) a = cadd(a, cmul(Hs[i][j], tv[j]));
+      term = cmake(sc * a.y / p, -sc * a.x / p);  // -i sc a / p
+      y = cadd(y, term);
     }
   }
-  for (int i = 0; i < k; ++i) coef[i * m + t] = y[i];
+  if (i < k) coef[i * m + t] = y;
 }
 
-// X[s,t] = sum_j V_j[s,t] coef[j*m+t]
-__global__ void combine_tm_kernel(int64_t n, int m, int ldv, const cplx* __restrict__ V, int64_t vstride, int k,
-                                  const cplx* __restrict__ coef, int ldx, cplx* __restrict__ X) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int t = (int)(idx % m);
-  const int64_t s = idx / m;
-  if (s >= n) return;
+// X[s,t] = sum_j V_j[s,t] coef[j Mtot + slot], all sectors (one thread per element of the pools)
+__global__ void g_combine_kernel(const GSec* __restrict__ gs, int ns, int64_t total, const cplx* __restrict__ V,
+                                 int64_t VS, int k, const cplx* __restrict__ coef, int Mtot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int si = find_sec(ns, e, [&](int i) { return gs[i].vo; });
+  const GSec& S = gs[si];
+  const int64_t local = e - S.vo;
+  const int64_t s = local / S.m;
+  const int t = (int)(local % S.m);
+  const int g = S.so + t;
   cplx a = cmake(0.0, 0.0);
-  for (int j = 0; j < k; ++j) a = cadd(a, cmul(V[j * vstride + s * ldv + t], coef[j * m + t]));
-  X[s * ldx + t] = a;
+  for (int j = 0; j < k; ++j) a = cadd(a, cmul(V[j * VS + e], coef[(int64_t)j * Mtot + g]));
+  S.X[s * S.ldx + t] = a;
 }
 
-// densities: part[(chunk * (M+1) + q) * m + t] = sum_{s in chunk} |X[s,t]|^2 [bit q of state s]
-// (q = M: plain norm^2)
+// densities: part[(M+1) co + (chunk (M+1) + q) m + t] = sum_{s in chunk} |X[s,t]|^2 [bit q of state s]
+// (q = M: plain norm^2), all sectors
 __global__ void __launch_bounds__(TT)
-    dens_partial_kernel(int64_t n, int m, int ld, const cplx* __restrict__ X, const uint64_t* __restrict__ states, int M,
-                        int rows_per_chunk, double* __restrict__ part) {
-  const int t = blockIdx.x * TT + threadIdx.x;
-  const int chunk = blockIdx.y;
+    g_dens_partial_kernel(const GSec* __restrict__ gs, int ns, int M, double* __restrict__ part) {
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
   if (t >= m) return;
-  const int64_t r0 = (int64_t)chunk * rows_per_chunk;
-  const int64_t r1 = min<int64_t>(n, r0 + rows_per_chunk);
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
   double acc[MAXM + 1];
-  for (int q = 0; q <= M; ++q) acc[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q <= MAXM; ++q) acc[q] = 0.0;
   for (int64_t s = r0; s < r1; ++s) {
-    const cplx v = X[s * ld + t];
+    const cplx v = S.X[s * S.ldx + t];
     const double p = v.x * v.x + v.y * v.y;
-    const uint64_t x = states[s];
-    for (int q = 0; q < M; ++q)
+    const uint64_t x = S.states[s];
+#pragma unroll
+    for (int q = 0; q < MAXM; ++q)
       if ((x >> q) & 1ull) acc[q] += p;
-    acc[M] += p;
+    acc[MAXM] += p;
   }
-  for (int q = 0; q <= M; ++q) part[((int64_t)chunk * (M + 1) + q) * m + t] = acc[q];
+  double* pb = part + (int64_t)(M + 1) * S.co;
+#pragma unroll
+  for (int q = 0; q < MAXM; ++q)
+    if (q < M) pb[((int64_t)tl.chunk * (M + 1) + q) * m + t] = acc[q];
+  pb[((int64_t)tl.chunk * (M + 1) + M) * m + t] = acc[MAXM];
 }
 
-// per slot: normalised densities into dens_all[traj * (M+1) + q] (q = M: norm^2)
-__global__ void dens_reduce_kernel(int m, int M, int nchunk, const double* __restrict__ part,
-                                   const int* __restrict__ slot2traj, double* __restrict__ dens_all) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  double v[MAXM + 1];
-  for (int q = 0; q <= M; ++q) {
-    double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * (M + 1) + q) * m + t];
-    v[q] = s;
+// dens_all[traj (M+1) + q] = density q / norm^2 (q < M), norm^2 (q = M); one thread per (q, slot)
+__global__ void g_dens_reduce_kernel(const GSec* __restrict__ gs, int ns, int Mtot, int M,
+                                     const double* __restrict__ part, const int* __restrict__ slot2traj,
+                                     double* __restrict__ dens_all) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)Mtot * (M + 1)) return;
+  const int q = (int)(idx / Mtot), g = (int)(idx % Mtot);
+  const int si = find_sec(ns, g, [&](int i) { return (int64_t)gs[i].so; });
+  const GSec& S = gs[si];
+  const int t = g - S.so;
+  const double* pb = part + (int64_t)(M + 1) * S.co;
+  double v = 0.0, nn = 0.0;
+  for (int c = 0; c < S.nch; ++c) {
+    v += pb[((int64_t)c * (M + 1) + q) * S.m + t];
+    if (q < M) nn += pb[((int64_t)c * (M + 1) + M) * S.m + t];
   }
-  const int tr = slot2traj[t];
-  for (int q = 0; q < M; ++q) dens_all[(int64_t)tr * (M + 1) + q] = v[q] / v[M];
-  dens_all[(int64_t)tr * (M + 1) + M] = v[M];
+  dens_all[(int64_t)slot2traj[g] * (M + 1) + q] = (q < M) ? v / nn : v;
 }
 
-// dens_sum[step * M + q] = sum over trajectories (index order) of dens_all; dens_sq likewise
+// dens_sum[q] = sum over trajectories (index order) of dens_all; dens_sq likewise
 __global__ void dens_accumulate_kernel(int ntraj, int M, const double* __restrict__ dens_all,
                                        double* __restrict__ dsum, double* __restrict__ dsq) {
   const int q = blockIdx.x;
@@ -317,19 +468,12 @@ __global__ void decide_kernel(int ntraj, int M, const double* __restrict__ dens_
   action[tr] = ch;
 }
 
-// gather per-slot Gamma from the per-trajectory array
-__global__ void gather_gamma_kernel(int m, const int* __restrict__ slot2traj, const double* __restrict__ gtraj,
-                                    double* __restrict__ g) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < m) g[t] = gtraj[slot2traj[t]];
-}
-
 struct JumpRec {
-  int slot;       // source slot
-  int mode;       // mode m
-  int loss;       // 0 dephasing, 1 loss
-  int staging;    // staging vector index
-  double scale;   // 1 / |L psi| (from the recorded densities and norm)
+  int slot;      // source slot
+  int mode;      // mode m
+  int loss;      // 0 dephasing, 1 loss
+  int staging;   // staging vector index
+  double scale;  // 1 / |L psi| (from the recorded densities and norm)
 };
 
 // staged[j][target] = L psi_slot (normalised); one block per jump
@@ -381,7 +525,39 @@ __global__ void broadcast_kernel(int64_t n, int m, int ld, const cplx* __restric
   if (s < n) X[s * ld + t] = psi0[s];
 }
 
+template <int MODE>
+void launch_pass(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, const cplx* V, int64_t VS, cplx* W,
+                 const cplx* h, int Mtot, cplx* part) {
+  switch (nq) {
+#define QD_PASS_CASE(N)                                                            \
+  case N:                                                                          \
+    g_pass_kernel<N, MODE><<<tiles, TT, 0, st>>>(gs, ns, V, VS, W, h, Mtot, part); \
+    break;
+    QD_PASS_CASE(1) QD_PASS_CASE(2) QD_PASS_CASE(3) QD_PASS_CASE(4) QD_PASS_CASE(5) QD_PASS_CASE(6)
+    QD_PASS_CASE(7) QD_PASS_CASE(8) QD_PASS_CASE(9) QD_PASS_CASE(10) QD_PASS_CASE(11) QD_PASS_CASE(12)
+    QD_PASS_CASE(13) QD_PASS_CASE(14) QD_PASS_CASE(15) QD_PASS_CASE(16) QD_PASS_CASE(17) QD_PASS_CASE(18)
+    QD_PASS_CASE(19) QD_PASS_CASE(20) QD_PASS_CASE(21) QD_PASS_CASE(22) QD_PASS_CASE(23) QD_PASS_CASE(24)
+    QD_PASS_CASE(25) QD_PASS_CASE(26) QD_PASS_CASE(27) QD_PASS_CASE(28) QD_PASS_CASE(29) QD_PASS_CASE(30)
+    QD_PASS_CASE(31) QD_PASS_CASE(32)
+#undef QD_PASS_CASE
+    default:
+      break;
+  }
+}
+
 unsigned g1(int64_t work, int b) { return (unsigned)std::max<int64_t>(1, (work + b - 1) / b); }
+
+// Rows per reduction chunk of a sector: about `target` waves for the sector
+// alone (target / groups chunks), 16 to 1024 rows, at most QD_TRAJ_MAX_CHUNKS
+// chunks.
+constexpr int QD_TRAJ_MAX_CHUNKS = 256;
+int rows_per_chunk(int64_t n, int groups, int target) {
+  const int64_t want = std::max<int64_t>(1, target / std::max(1, groups));
+  int64_t rpc = std::max<int64_t>(16, (n + want - 1) / want);
+  rpc = std::min<int64_t>(rpc, 1024);
+  rpc = std::max<int64_t>(rpc, (n + QD_TRAJ_MAX_CHUNKS - 1) / QD_TRAJ_MAX_CHUNKS);
+  return (int)rpc;
+}
 
 // ---------------------------------------------------------------- host state
 struct Sector {
@@ -392,9 +568,8 @@ struct Sector {
   double* D = nullptr;
   cplx* X = nullptr;  // n x ld, trajectory-minor
   int ld = 0;
-  int m = 0;                 // active slots
+  int m = 0;  // active slots
   std::vector<int> slot2traj;
-  int* d_slot2traj = nullptr;
 };
 
 }  // namespace
@@ -427,12 +602,23 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   }
   std::vector<Sector> sec((nsite + 1) * (nsite + 1));
   auto sid = [&](int a, int b) { return a * (nsite + 1) + b; };
-  // device scratch
-  int *d_jmode = nullptr, *d_action = nullptr;
+  // largest sector reachable by losses from (nup0, ndn0): C(n, a) is not monotone in a
+  int64_t nmax = 1;
+  for (int a = 0; a <= nup0; ++a)
+    for (int b = 0; b <= ndn0; ++b)
+      nmax = std::max<int64_t>(nmax, (int64_t)qd_binom_host(nsite, a) * (int64_t)qd_binom_host(nsite, b));
+  const int NSMAX = (int)sec.size();
+  // chunk counts are at most min(QD_TRAJ_MAX_CHUNKS, ceil(n / 16)) per sector
+  const int max_nch = (int)std::min<int64_t>(QD_TRAJ_MAX_CHUNKS, (nmax + 15) / 16);
+  const int PMAX = std::max(K, M + 1);
+
+  // device buffers
+  int *d_jmode = nullptr, *d_action = nullptr, *d_slot2traj = nullptr;
   double *d_jrate = nullptr, *d_dens_all = nullptr, *d_gtraj = nullptr, *d_dsum = nullptr, *d_dsq = nullptr;
-  double *d_gslot = nullptr, *d_scale = nullptr, *d_dpart = nullptr;
-  cplx *d_V = nullptr, *d_W = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr, *d_coef = nullptr;
-  cplx* d_staging = nullptr;
+  double *d_gslot = nullptr, *d_scale = nullptr;
+  cplx *d_V = nullptr, *d_W = nullptr, *d_W2 = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr;
+  cplx *d_coef = nullptr, *d_staging = nullptr;
+  GSec* d_gsec = nullptr;
   JumpRec* d_jr = nullptr;
   int2 *d_pairs = nullptr, *d_moves = nullptr;
   std::vector<double> dsum((size_t)(nsteps + 1) * M, 0.0), dsq((size_t)(nsteps + 1) * M, 0.0);
@@ -449,6 +635,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       if (S.csr) qd_csr_destroy(S.csr);
       if (S.basis) qd_basis_destroy(S.basis);
     }
+    if (d_staging) hipFree(d_staging);
     for (void* p : allocs) hipFree(p);
     qd_termprog_destroy(prog);
   };
@@ -466,34 +653,28 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     if (S.csr->hermitian != 1) return qd_set_error(ctx, QD_E_UNSUPPORTED, "trajectory Hamiltonian must be Hermitian");
     S.ld = ntraj;
     S.slot2traj.reserve(ntraj);
-    if (!dalloc((void**)&S.D, sizeof(double) * S.n) || !dalloc((void**)&S.X, sizeof(cplx) * S.n * S.ld) ||
-        !dalloc((void**)&S.d_slot2traj, sizeof(int) * ntraj))
+    if (!dalloc((void**)&S.D, sizeof(double) * S.n) || !dalloc((void**)&S.X, sizeof(cplx) * S.n * S.ld))
       return qd_set_error(ctx, QD_E_OOM, "sector (%d,%d) buffers", a, b);
     diag_rates_kernel<<<g1(S.n, 256), 256, 0, st>>>(S.n, S.basis->d_states, d_jmode, d_jrate, njumps, S.D);
     return QD_OK;
   };
 
-  // largest sector reachable by losses from (nup0, ndn0): C(n, a) is not monotone in a
-  int64_t nmax = 1;
-  for (int a = 0; a <= nup0; ++a)
-    for (int b = 0; b <= ndn0; ++b)
-      nmax = std::max<int64_t>(nmax, (int64_t)qd_binom_host(nsite, a) * (int64_t)qd_binom_host(nsite, b));
-  const int max_chunks = 256;
   if (!dalloc((void**)&d_jmode, sizeof(int) * std::max(1, njumps)) ||
       !dalloc((void**)&d_jrate, sizeof(double) * std::max(1, njumps)) ||
-      !dalloc((void**)&d_action, sizeof(int) * ntraj) || !dalloc((void**)&d_dens_all, sizeof(double) * ntraj * (M + 1)) ||
+      !dalloc((void**)&d_action, sizeof(int) * ntraj) || !dalloc((void**)&d_slot2traj, sizeof(int) * ntraj) ||
+      !dalloc((void**)&d_dens_all, sizeof(double) * ntraj * (M + 1)) ||
       !dalloc((void**)&d_gtraj, sizeof(double) * ntraj) || !dalloc((void**)&d_dsum, sizeof(double) * M) ||
       !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_gslot, sizeof(double) * ntraj) ||
       !dalloc((void**)&d_scale, sizeof(double) * ntraj) ||
-      !dalloc((void**)&d_dpart, sizeof(double) * (size_t)max_chunks * (M + 1) * ntraj) ||
-      !dalloc((void**)&d_V, sizeof(cplx) * (size_t)(K + 1) * nmax * ntraj) ||
+      !dalloc((void**)&d_V, sizeof(cplx) * (size_t)K * nmax * ntraj) ||
       !dalloc((void**)&d_W, sizeof(cplx) * (size_t)nmax * ntraj) ||
-      !dalloc((void**)&d_part, sizeof(cplx) * (size_t)max_chunks * (K + 1) * ntraj) ||
-      !dalloc((void**)&d_h, sizeof(cplx) * (size_t)(K + 1) * ntraj) ||
+      !dalloc((void**)&d_W2, sizeof(cplx) * (size_t)nmax * ntraj) ||
+      !dalloc((void**)&d_part, sizeof(cplx) * (size_t)max_nch * PMAX * ntraj) ||
+      !dalloc((void**)&d_h, sizeof(cplx) * (size_t)K * ntraj) ||
       !dalloc((void**)&d_hess, sizeof(cplx) * (size_t)K * (K + 1) * ntraj) ||
       !dalloc((void**)&d_coef, sizeof(cplx) * (size_t)K * ntraj) ||
-      !dalloc((void**)&d_jr, sizeof(JumpRec) * ntraj) || !dalloc((void**)&d_pairs, sizeof(int2) * ntraj) ||
-      !dalloc((void**)&d_moves, sizeof(int2) * ntraj)) {
+      !dalloc((void**)&d_gsec, sizeof(GSec) * NSMAX) || !dalloc((void**)&d_jr, sizeof(JumpRec) * ntraj) ||
+      !dalloc((void**)&d_pairs, sizeof(int2) * ntraj) || !dalloc((void**)&d_moves, sizeof(int2) * ntraj)) {
     cleanup();
     return qd_set_error(ctx, QD_E_OOM, "trajectory workspace (%d trajectories, dim %lld, k %d)", ntraj,
                         (long long)nmax, k_rt);
@@ -512,93 +693,150 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     S.m = ntraj;
     S.slot2traj.resize(ntraj);
     for (int t = 0; t < ntraj; ++t) S.slot2traj[t] = t;
-    hipMemcpyAsync(S.d_slot2traj, S.slot2traj.data(), sizeof(int) * ntraj, hipMemcpyHostToDevice, st);
     broadcast_kernel<<<g1(S.n * ntraj, 256), 256, 0, st>>>(S.n, ntraj, S.ld, (const cplx*)d_psi0, S.X);
   }
   std::vector<int> action(ntraj);
-  // per-step jump lists; reserved for ntraj entries so that the async copies
-  // from them stay valid until the next step's synchronisation
+  std::vector<double> dens_host((size_t)ntraj * (M + 1));
+  // per-step host tables, reserved so that the async copies from them stay
+  // valid until the next synchronisation
   std::vector<JumpRec> h_jr;
   std::vector<int2> h_pairs, h_moves;
+  std::vector<GSec> h_gsec;
+  std::vector<int> h_slot2traj;
   h_jr.reserve(ntraj);
   h_pairs.reserve(ntraj);
   h_moves.reserve(ntraj);
-  std::vector<double> dens_host((size_t)ntraj * (M + 1));
+  h_gsec.reserve(NSMAX);
+  h_slot2traj.reserve(ntraj);
   int stg_cap = 0;
+  int ns = 0;
+  int64_t VS = 0, tiles = 0, blocks = 0, ngroups = 0;
+  int occupancy_target = 2048;  // waves per sector for the reduction tiles (tuned below)
 
-  // row chunks of the column reductions depend on the sector size only, so a
-  // trajectory's sums (and its result) do not depend on how many others share
-  // its sector
-  auto chunks_for = [&](int64_t n, int) {
-    int64_t c = std::min<int64_t>((n + 63) / 64, (int64_t)max_chunks);
-    return (int)std::max<int64_t>(1, c);
-  };
-
-  // densities of every active trajectory -> d_dens_all; returns the sums
-  auto densities = [&](int step) -> int {
+  // grouped-launch tables from the current sector membership
+  auto build_groups = [&]() {
+    h_gsec.clear();
+    h_slot2traj.clear();
+    int64_t vo = 0, co = 0;
+    tiles = blocks = ngroups = 0;
     for (auto& S : sec) {
       if (!S.basis || S.m == 0) continue;
-      int nch = chunks_for(S.n, S.m);
-      int rpc = (int)((S.n + nch - 1) / nch);
-      nch = (int)((S.n + rpc - 1) / rpc);
-      dim3 g((S.m + TT - 1) / TT, nch);
-      dens_partial_kernel<<<g, TT, 0, st>>>(S.n, S.m, S.ld, S.X, S.basis->d_states, M, rpc, d_dpart);
-      dens_reduce_kernel<<<g1(S.m, 256), 256, 0, st>>>(S.m, M, nch, d_dpart, S.d_slot2traj, d_dens_all);
+      GSec g{};
+      g.n = S.n;
+      g.m = S.m;
+      g.ldx = S.ld;
+      g.groups = (S.m + TT - 1) / TT;
+      g.rpc = rows_per_chunk(S.n, g.groups, occupancy_target);
+      g.nch = (int)((S.n + g.rpc - 1) / g.rpc);
+      g.so = (int)h_slot2traj.size();
+      g.vo = vo;
+      g.co = co;
+      g.tile0 = tiles;
+      g.blk0 = blocks;
+      g.grp0 = ngroups;
+      g.X = S.X;
+      g.outer = S.csr->d_outer;
+      g.inner = S.csr->d_inner;
+      g.vals = S.csr->d_vals;
+      g.D = S.D;
+      g.states = S.basis->d_states;
+      h_gsec.push_back(g);
+      h_slot2traj.insert(h_slot2traj.end(), S.slot2traj.begin(), S.slot2traj.begin() + S.m);
+      vo += S.n * S.m;
+      co += (int64_t)g.nch * S.m;
+      tiles += (int64_t)g.nch * g.groups;
+      blocks += (S.n + SPMM_RB - 1) / SPMM_RB * g.groups;
+      ngroups += g.groups;
     }
+    ns = (int)h_gsec.size();
+    VS = vo;
+    hipMemcpyAsync(d_gsec, h_gsec.data(), sizeof(GSec) * ns, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(d_slot2traj, h_slot2traj.data(), sizeof(int) * ntraj, hipMemcpyHostToDevice, st);
+  };
+
+  // densities of every trajectory -> d_dens_all; sums over trajectories -> host (step >= 0)
+  auto densities = [&](int step) -> int {
+    g_dens_partial_kernel<<<(unsigned)tiles, TT, 0, st>>>(d_gsec, ns, M, (double*)d_part);
+    g_dens_reduce_kernel<<<g1((int64_t)ntraj * (M + 1), 256), 256, 0, st>>>(d_gsec, ns, ntraj, M, (double*)d_part,
+                                                                           d_slot2traj, d_dens_all);
     dens_accumulate_kernel<<<M, 256, 0, st>>>(ntraj, M, d_dens_all, d_dsum, d_dsq);
     double hs[2 * MAXM];
     QD_HIP(ctx, hipMemcpyAsync(hs, d_dsum, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     QD_HIP(ctx, hipMemcpyAsync(hs + MAXM, d_dsq, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     QD_HIP(ctx, hipStreamSynchronize(st));
-    for (int q = 0; q < M; ++q) {
-      dsum[(size_t)step * M + q] = hs[q];
-      dsq[(size_t)step * M + q] = hs[MAXM + q];
-    }
+    if (step >= 0)
+      for (int q = 0; q < M; ++q) {
+        dsum[(size_t)step * M + q] = hs[q];
+        dsq[(size_t)step * M + q] = hs[MAXM + q];
+      }
     return QD_OK;
   };
 
-  // one Arnoldi step for every active slot of a sector (X updated in place)
-  auto evolve_sector = [&](Sector& S) -> int {
-    const int m = S.m;
-    if (m == 0) return QD_OK;
-    const int ldv = m;
-    const int64_t vstride = S.n * (int64_t)ldv;
-    int nch = chunks_for(S.n, m);
-    int rpc = (int)((S.n + nch - 1) / nch);
-    nch = (int)((S.n + rpc - 1) / rpc);
-    dim3 gd((m + TT - 1) / TT, nch);
-    gather_gamma_kernel<<<g1(m, 256), 256, 0, st>>>(m, S.d_slot2traj, d_gtraj, d_gslot);
-    // V_0 = X / |X|
-    col_dots_kernel<<<gd, TT, 0, st>>>(S.n, m, S.ld, S.X, 0, 1, S.X, rpc, d_part);
-    col_reduce_kernel<<<g1(m, 256), 256, 0, st>>>(m, 1, nch, d_part, d_h);
-    inv_norm_kernel<<<g1(m, 256), 256, 0, st>>>(m, d_h, d_scale);
-    col_scale_kernel<<<g1(S.n * m, 256), 256, 0, st>>>(S.n, m, S.ld, S.X, ldv, d_V, d_scale);
+  // one Arnoldi step (CGS2, K vectors) for every slot of every occupied sector, X updated in place
+  auto evolve_all = [&]() -> int {
+    const int Mtot = ntraj;
+    slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_gslot, d_scale);
+    cplx* Wa = d_W;
+    cplx* Wb = d_W2;
+    // V_0 = X / |X|, Wa = H_eff V_0
+    const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
+    g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 1, nullptr, d_scale, d_gslot, d_V, Wa);
     for (int j = 0; j < K; ++j) {
-      cplx* Vj = d_V + (int64_t)j * vstride;
-      dim3 gs((unsigned)S.n, (m + TT - 1) / TT);
-      spmm_tm_kernel<<<gs, TT, 0, st>>>(S.n, m, ldv, S.csr->d_outer, S.csr->d_inner, S.csr->d_vals, S.D, d_gslot,
-                                        Vj, d_W);
-      for (int pass = 0; pass < 2; ++pass) {
-        col_dots_kernel<<<gd, TT, 0, st>>>(S.n, m, ldv, d_V, vstride, j + 1, d_W, rpc, d_part);
-        col_reduce_kernel<<<g1((int64_t)m * (j + 1), 256), 256, 0, st>>>(m, j + 1, nch, d_part, d_h);
-        hess_acc_kernel<<<g1((int64_t)m * (j + 1), 256), 256, 0, st>>>(m, j, K, d_h, d_hess, pass);
-        col_update_kernel<<<g1(S.n * ldv, 256), 256, 0, st>>>(S.n, m, ldv, d_V, vstride, j + 1, d_h, d_W);
-      }
+      const int nq = j + 1;
+      const unsigned gr = (unsigned)(ngroups * nq);
+      launch_pass<0>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, nullptr, Mtot, d_part);
+      g_reduce_kernel<<<gr, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, nq, d_part, d_h, d_hess, j, K, 0, nullptr);
+      launch_pass<1>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, d_h, Mtot, d_part);
+      g_reduce_kernel<<<gr, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, nq, d_part, d_h, d_hess, j, K, 1, nullptr);
       if (j + 1 < K) {
-        col_dots_kernel<<<gd, TT, 0, st>>>(S.n, m, ldv, d_W, 0, 1, d_W, rpc, d_part);
-        col_reduce_kernel<<<g1(m, 256), 256, 0, st>>>(m, 1, nch, d_part, d_h);
-        beta_kernel<<<g1(m, 256), 256, 0, st>>>(m, j, K, d_h, d_hess, d_scale);
-        col_scale_kernel<<<g1(S.n * m, 256), 256, 0, st>>>(S.n, m, ldv, d_W, ldv, d_V + (int64_t)(j + 1) * vstride,
-                                                          d_scale);
+        launch_pass<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, d_h, Mtot, d_part);
+        g_reduce_kernel<<<(unsigned)ngroups, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, 1, d_part, nullptr, d_hess, j, K,
+                                                            2, d_scale);
+        // V_{j+1} = Wa / beta, Wb = H_eff V_{j+1}
+        g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 0, Wa, d_scale, d_gslot, d_V + (j + 1) * VS, Wb);
+        std::swap(Wa, Wb);
       }
     }
-    small_expm_kernel<<<g1(m, 64), 64, 0, st>>>(m, K, K, d_hess, dt, d_coef);
-    combine_tm_kernel<<<g1(S.n * m, 256), 256, 0, st>>>(S.n, m, ldv, d_V, vstride, K, d_coef, S.ld, S.X);
+    small_expm_kernel<<<(unsigned)Mtot, 64, 0, st>>>(Mtot, K, K, d_hess, dt, d_coef);
+    g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, d_V, VS, K, d_coef, Mtot);
     QD_CHECK_LAUNCH(ctx);
     return QD_OK;
   };
 
+  // Tile autotuner for the uneven sectors: the occupancy target of the
+  // reduction tiles (rows per chunk of every sector) is timed on the widest
+  // Gram-Schmidt pass over the current sector membership (a read-only pass,
+  // results discarded).  It runs at the start and again once the loss jumps
+  // have spread the trajectories over smaller sectors.
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEventCreate(&ev0);
+  hipEventCreate(&ev1);
+  auto tune_tiles = [&]() {
+    const int cand[5] = {512, 1024, 2048, 4096, 8192};
+    float best = 1e30f;
+    int best_c = occupancy_target;
+    for (int c : cand) {
+      occupancy_target = c;
+      build_groups();
+      launch_pass<0>(K, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, nullptr, ntraj, d_part);
+      hipEventRecord(ev0, st);
+      for (int r = 0; r < 2; ++r) launch_pass<0>(K, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, nullptr, ntraj, d_part);
+      hipEventRecord(ev1, st);
+      hipEventSynchronize(ev1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev0, ev1);
+      if (ms < best) {
+        best = ms;
+        best_c = c;
+      }
+    }
+    occupancy_target = best_c;
+  };
+  tune_tiles();
+
   for (int step = 0; step <= nsteps; ++step) {
+    if (step > 0 && step == nsteps / 4) tune_tiles();
+    build_groups();
     rc = densities(step);
     if (rc) break;
     if (step == nsteps) break;
@@ -634,8 +872,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
         JumpRec r{slot, mode, loss, (int)pend.size(), 1.0 / std::sqrt(nd)};
         int na = S.a, nb = S.b;
         if (loss) {
-          if (mode < nsite) --na;
-          else --nb;
+          if (mode < nsite)
+            --na;
+          else
+            --nb;
         }
         pend.push_back({tr, na, nb, (int)pend.size(), slot, loss});
         h_jr.push_back(r);
@@ -645,17 +885,17 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       // staging capacity grows on demand
       if (!d_staging || (int)pend.size() > stg_cap) {
         int ncap = std::max<int>((int)pend.size() * 2, 64);
-        cplx* ns = nullptr;
-        if (hipMalloc(&ns, sizeof(cplx) * (size_t)ncap * nmax) != hipSuccess) {
+        cplx* nsg = nullptr;
+        if (hipMalloc(&nsg, sizeof(cplx) * (size_t)ncap * nmax) != hipSuccess) {
           rc = qd_set_error(ctx, QD_E_OOM, "jump staging");
           break;
         }
         if (d_staging) {
-          hipMemcpyAsync(ns, d_staging, sizeof(cplx) * (size_t)stg_cap * nmax, hipMemcpyDeviceToDevice, st);
+          hipMemcpyAsync(nsg, d_staging, sizeof(cplx) * (size_t)stg_cap * nmax, hipMemcpyDeviceToDevice, st);
           hipStreamSynchronize(st);
           hipFree(d_staging);
         }
-        d_staging = ns;
+        d_staging = nsg;
         stg_cap = ncap;
       }
       // target sector descriptors for losses out of S
@@ -675,16 +915,11 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
                                                        nmax, d_staging);
     }
     if (rc) break;
-    // evolve every sector (jumpers too; their columns are replaced below)
-    for (auto& S : sec) {
-      if (!S.basis || S.m == 0) continue;
-      if ((rc = evolve_sector(S))) break;
-    }
-    if (rc) break;
+    // evolve every occupied sector (jumpers too; their columns are replaced below)
+    if ((rc = evolve_all())) break;
     // commit jumps: dephasing in place; losses move to the target sector
     if (!pend.empty()) {
       n_jumps += (int64_t)pend.size();
-      // removals per source sector (losses), in descending slot order
       std::vector<std::vector<int>> removed(sec.size());
       std::vector<std::vector<int2>> commits(sec.size());
       for (auto& S : sec) {
@@ -692,14 +927,12 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
         for (int slot = 0; slot < S.m; ++slot) {
           int tr = S.slot2traj[slot];
           if (action[tr] < 0) continue;
-          // find the pending record of tr
           for (const Pending& p : pend)
             if (p.traj == tr) {
-              if (!p.loss) {
+              if (!p.loss)
                 commits[sid(S.a, S.b)].push_back(make_int2(p.sidx, slot));
-              } else {
+              else
                 removed[sid(S.a, S.b)].push_back(slot);
-              }
               break;
             }
         }
@@ -742,9 +975,6 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
           move_cols_kernel<<<g1(S.n, 256), 256, 0, st>>>(S.n, S.ld, S.X, d_moves + m0, nmv);
         }
       }
-      for (auto& S : sec)
-        if (S.basis && S.m > 0)
-          hipMemcpyAsync(S.d_slot2traj, S.slot2traj.data(), sizeof(int) * S.m, hipMemcpyHostToDevice, st);
     }
     QD_CHECK_LAUNCH(ctx);
   }
@@ -763,7 +993,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       stats->sectors = used;
     }
   }
-  if (d_staging) hipFree(d_staging);
+  hipEventDestroy(ev0);
+  hipEventDestroy(ev1);
   cleanup();
   return rc;
 }

@@ -83,3 +83,38 @@ def hubbard_quench(ctx, Lx, Ly, nup, ndn, U0, U1, t=1.0, k_gs=120, k_rt=30, dt=0
 def op_expect(op, psi):
     v = op.expect(psi)
     return v.real
+
+
+def hubbard_trajectories(ctx, L=8, nup=4, ndn=4, U=4.0, t=1.0, dephase=0.1, loss=0.05, ntraj=4096, T=10.0,
+                         dt=0.05, k_gs=100, k_rt=13, seed=5, want_log=False, log=None):
+    """Dissipative Hubbard chain (open boundaries): ground state of the
+    (nup, ndn) sector by Lanczos, then ntraj quantum-jump trajectories with
+    dephasing sqrt(dephase) n_m and loss sqrt(loss) c_m on every mode
+    (programs/hubbard_diss.cpp:377-385,439-482 with loss channels added)."""
+    import torch
+    from .runtime import jump_channels, traj_run
+    bonds = Q.hubbard_bonds_chain(L, False)
+    idx, ham = Q.hubbard(L, bonds, t, U)
+    basis = Basis(ctx, Q.hubbard_sector(idx, L, nup, ndn))
+    op, keep = make_operator(ctx, basis, ham, 2 * L, False)
+    psi = ctx.seeded_state(basis.size, seed)
+    kr = Krylov(ctx, basis.size, k_gs, herm=True)
+    e0 = kr.ground_state(op, psi)
+    kr.close()
+    nrm = ctx.norm2(psi) ** 0.5
+    psi = psi / nrm
+    torch.cuda.synchronize()
+    nsteps = int(round(T / dt))
+    ch = jump_channels(L, dephase, loss)
+    t0 = time.perf_counter()
+    res = traj_run(ctx, ham, L, nup, ndn, psi, ch, ntraj, nsteps, dt, k_rt, seed, want_log=want_log)
+    torch.cuda.synchronize()
+    run_s = time.perf_counter() - t0
+    for h in [op] + keep + [basis]:
+        h.close()
+    res.update({"E0": e0, "psi0": psi, "run_s": run_s, "nsteps": nsteps, "channels": ch, "n": basis.size,
+                "ham": ham})
+    if log:
+        log(f"trajectories L={L} n={basis.size}: {ntraj} x {nsteps} steps in {run_s:.3f} s, "
+            f"{res['jumps']} jumps ({res['losses']} losses), {res['sectors']} sectors")
+    return res

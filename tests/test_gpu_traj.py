@@ -58,7 +58,7 @@ def test_traj_full_size_closed_form_and_oracle_prefix(ctx):
     ham = _setup(L)
     psi0 = oracle.seeded_state(4900, 15)
     ch = jump_channels(L, 0.1, 0.05)
-    ntraj, nsteps, dt, k = 512, 200, 0.05, 16
+    ntraj, nsteps, dt, k = 512, 200, 0.05, 13
     print(f"traj L=8 ntraj={ntraj} nsteps={nsteps}", flush=True)
     g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 5, want_log=True)
     is_loss = np.array([c[0] == 1 for c in ch] + [False])
