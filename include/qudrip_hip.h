@@ -165,6 +165,11 @@ int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);
  * the same H as `prog >> basis` (operator.hpp:215-232 is the lazy form). */
 int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out);
 int qd_linop_destroy(qd_linop* op);
+/* A view of `base` whose products add diag(d): (base + diag(d)) x, with
+ * d_diag n complex values on the device.  Borrows base's arrays and d_diag
+ * (both must outlive the view).  Lets many operators H = A + c D share one
+ * stored A (no reference counterpart; used by the solve server below). */
+int qd_linop_with_diagonal(qd_ctx* ctx, qd_linop* base, const double* d_diag, qd_linop** out);
 int64_t qd_linop_dim(const qd_linop* op);
 /* Matrix-free operators over a two-block sector (up bits low, down bits
  * high) whose terms are hops inside the low block, ops inside the high block
@@ -269,6 +274,68 @@ int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nterms, const 
                 int ndn0, const double* d_psi0, const qd_jump* jumps, int njumps, int ntraj, int nsteps,
                 double dt, int kry_dim_rt, uint64_t seed, double* h_dens_mean, double* h_dens_var,
                 int* h_jump_log, qd_traj_stats* stats);
+
+/* ------------------------------------------------------------ serving */
+/* A solve server: no reference counterpart (the reference runs one solve per
+ * process, programs/spin.cpp, programs/hubbard_diss.cpp); every request runs
+ * the reference's own solver calls (getGroundState with add_dim and the
+ * change of the lowest Ritz value, sparse_algorithm.hpp:564-585; evolve,
+ * :548-562) on the device.  A model is a sector and two operators, A and a
+ * diagonal D; a request solves H = A + coeff * D.  The server stores the
+ * basis, A's matrix and D's diagonal once per model; workers (one thread and
+ * stream each) admit queued requests in arrival order whenever a worker is
+ * free and the request's device memory fits in hbm_budget (later, smaller
+ * requests may pass a waiting one a bounded number of times; a request that
+ * can never fit is answered with QD_E_OOM).  A request's Krylov state stays on
+ * the device until it finishes; its result does not depend on what else
+ * runs. */
+typedef struct qd_server qd_server;
+enum { QD_REQ_GROUND_STATE = 0, QD_REQ_EVOLVE = 1 };
+typedef struct {
+  int32_t kind;   /* QD_REQ_GROUND_STATE or QD_REQ_EVOLVE */
+  int32_t model;  /* from qd_server_add_model */
+  double coeff;   /* H = A + coeff * D */
+  int32_t k;      /* ground state: Krylov dimension limit; evolve: kry_dim_rt */
+  int32_t chunk;  /* ground state: basis vectors added per getGroundState call */
+  double tol;     /* ground state: stop when E0 moves by <= tol |E0| over one chunk
+                   * (0: one call with the full dimension, like the plain solver) */
+  int32_t steps;  /* evolve: number of time steps */
+  double dt;      /* evolve: time step */
+  uint64_t seed;  /* start vector: qd_seeded_state(seed) */
+  int64_t tag;    /* caller's id, returned in the result */
+} qd_request;
+typedef struct {
+  int64_t tag;
+  int32_t status;      /* QD_OK or an error code (QD_E_OOM: never fits the budget) */
+  int32_t iterations;  /* ground state: Krylov dimension used; evolve: steps */
+  double energy;       /* ground state: E0; evolve: <H> / <psi|psi> at the end */
+  double energy_start; /* evolve: <H> / <psi|psi> of the start vector */
+  double observable;   /* <D> / <psi|psi> of the final state */
+  double t_submit, t_start, t_end; /* seconds on the server's clock */
+  int32_t worker;
+} qd_result;
+typedef struct {
+  int64_t submitted, finished, rejected, queued;
+  int32_t running, max_running;
+  int64_t bytes_in_use, peak_bytes;
+} qd_server_stats;
+int qd_server_create(int device, size_t hbm_budget, int workers, qd_server** out);
+/* Sector (nbits, cons) with operators A (tA, oA) and diagonal D (tD, oD;
+ * ntD = 0: none).  structured != 0: evolve requests use the matrix-free
+ * product of the combined program (two-block Kronecker form when present). */
+int qd_server_add_model(qd_server* s, int nbits, const qd_constraint* cons, int ncons, const qd_term* tA, int ntA,
+                        const qd_op* oA, int noA, const qd_term* tD, int ntD, const qd_op* oD, int noD,
+                        int structured, int* model_id);
+int qd_server_submit(qd_server* s, const qd_request* req);
+/* Moves up to cap finished results (in completion order) into out. */
+int qd_server_poll(qd_server* s, qd_result* out, int cap, int* count);
+/* Blocks until every submitted request has finished. */
+int qd_server_drain(qd_server* s);
+double qd_server_clock(qd_server* s);
+int qd_server_get_stats(qd_server* s, qd_server_stats* st);
+const char* qd_server_last_error(qd_server* s);
+/* Finishes the queued requests, then stops the workers and frees the models. */
+int qd_server_destroy(qd_server* s);
 
 /* ------------------------------------------------- FP64 host numerics */
 /* The small dense problems of the solvers, exposed for direct checks.

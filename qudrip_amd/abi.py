@@ -46,6 +46,30 @@ QD_JUMP_DEPHASE = 0
 QD_JUMP_LOSS = 1
 
 
+QD_REQ_GROUND_STATE = 0
+QD_REQ_EVOLVE = 1
+
+
+class qd_request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("model", ctypes.c_int32), ("coeff", ctypes.c_double),
+                ("k", ctypes.c_int32), ("chunk", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("steps", ctypes.c_int32), ("dt", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("tag", ctypes.c_int64)]
+
+
+class qd_result(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_int64), ("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("energy", ctypes.c_double), ("energy_start", ctypes.c_double), ("observable", ctypes.c_double),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("worker", ctypes.c_int32)]
+
+
+class qd_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("finished", ctypes.c_int64), ("rejected", ctypes.c_int64),
+                ("queued", ctypes.c_int64), ("running", ctypes.c_int32), ("max_running", ctypes.c_int32),
+                ("bytes_in_use", ctypes.c_int64), ("peak_bytes", ctypes.c_int64)]
+
+
 class qd_krylov_info(ctypes.Structure):
     _fields_ = [("act_dim", ctypes.c_int), ("breakdown", ctypes.c_int), ("error", ctypes.c_double),
                 ("beta_last", ctypes.c_double), ("n_spmv", ctypes.c_int64)]
@@ -86,6 +110,7 @@ SIGNATURES = {
     "qd_linop_from_csr": (I, [VP, VP, PP]),
     "qd_linop_matfree": (I, [VP, VP, VP, PP]),
     "qd_linop_destroy": (I, [VP]),
+    "qd_linop_with_diagonal": (I, [VP, VP, VP, PP]),
     "qd_linop_dim": (I64, [VP]),
     "qd_linop_set_lanes": (I, [VP, I]),
     "qd_linop_get_lanes": (I, [VP]),
@@ -108,6 +133,17 @@ SIGNATURES = {
     "qd_traj_run": (I, [VP, I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, I, I, VP,
                         ctypes.POINTER(qd_jump), I, I, I, ctypes.c_double, I, ctypes.c_uint64, VP, VP, VP,
                         ctypes.POINTER(qd_traj_stats)]),
+    "qd_server_create": (I, [I, ctypes.c_size_t, I, PP]),
+    "qd_server_add_model": (I, [VP, I, ctypes.POINTER(qd_constraint), I, ctypes.POINTER(qd_term), I,
+                                ctypes.POINTER(qd_op), I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, I,
+                                ctypes.POINTER(I)]),
+    "qd_server_submit": (I, [VP, ctypes.POINTER(qd_request)]),
+    "qd_server_poll": (I, [VP, ctypes.POINTER(qd_result), I, ctypes.POINTER(I)]),
+    "qd_server_drain": (I, [VP]),
+    "qd_server_clock": (ctypes.c_double, [VP]),
+    "qd_server_get_stats": (I, [VP, ctypes.POINTER(qd_server_stats)]),
+    "qd_server_last_error": (ctypes.c_char_p, [VP]),
+    "qd_server_destroy": (I, [VP]),
     "qd_host_tridiag_eig": (I, [I, DP, DP, DP, DP]),
     "qd_host_expm": (I, [I, DP, DP]),
 }

@@ -574,21 +574,19 @@ int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
   const KronDesc& d = k->desc;
   if (d.nA == 0 || d.nB == 0) return QD_OK;
   size_t lds = sizeof(cplx) * (size_t)d.nA;
-  static int optin = -1;
-  if (optin < 0) {
+  // opt-in LDS limit, and the kernels' attributes raised to it, once per
+  // process (thread-safe static initialisation; one device per process)
+  static const int optin = [dev = ctx->device]() {
     int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device) != hipSuccess) v = 65536;
-    optin = v;
-    if ((size_t)optin > 65536) hipFuncSetAttribute((const void*)kron_rows_kernel<true>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, optin);
-  }
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) v = 65536;
+    if (v > 65536) {
+      hipFuncSetAttribute((const void*)kron_rows_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+      hipFuncSetAttribute((const void*)kron_rows_packed_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    }
+    return v;
+  }();
   size_t lds2 = sizeof(cplx) * ((size_t)d.nvtab + 2 * (size_t)d.nA);
   if (d.ell && lds2 <= (size_t)optin) {
-    static bool attr2 = false;
-    if (!attr2 && (size_t)optin > 65536) {
-      hipFuncSetAttribute((const void*)kron_rows_packed_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, optin);
-      attr2 = true;
-    }
     kron_rows_packed_kernel<2><<<(unsigned)((d.nB + 1) / 2), 1024, lds2, ctx->stream>>>(d, x, y, shift);
   } else if (lds <= (size_t)optin) {
     kron_rows_kernel<true><<<(unsigned)d.nB, 1024, lds, ctx->stream>>>(d, x, y, shift);

@@ -1,0 +1,521 @@
+// serve.hip -- continuous-batching solve server over the Krylov solvers.
+//
+// No reference counterpart: the reference runs one solve per process
+// (programs/spin.cpp, programs/hubbard_diss.cpp).  The server keeps, per
+// model, the sector basis, the stored matrix of the model's first operator A
+// and the diagonal of its second operator D on the device; a request asks for
+// the ground state or a real-time evolution of H = A + c D for its own c
+// (Jz of an XXZ chain, U of a Hubbard lattice).  Its operator is a view of the
+// cached matrix plus its own diagonal c D, so no request builds a matrix.
+//
+// Workers (one thread and one HIP stream each) take requests in arrival
+// order as soon as a worker is free and the request's device memory fits in
+// the HBM budget; a request that does not fit waits while later, smaller
+// requests may pass it (at most QD_SERVE_MAX_BYPASS times), and a request that
+// could never fit is rejected.  The Krylov state of every running request
+// stays on the device until it finishes.  Each request runs the same kernels
+// with the same reduction order whatever else is running, so its result does
+// not depend on the mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "qd_internal.hpp"
+
+namespace {
+
+constexpr int QD_SERVE_MAX_BYPASS = 8;
+
+__global__ void scale_diag_kernel(int64_t n, double c, const cplx* __restrict__ src, cplx* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = cmake(c * src[i].x, c * src[i].y);
+}
+
+// diag[j] = value of column j's diagonal entry; flag = 1 if any entry is off the diagonal
+__global__ void extract_diag_kernel(int64_t n, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                                    const cplx* __restrict__ vals, cplx* __restrict__ diag, int* __restrict__ flag) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cplx d = cmake(0.0, 0.0);
+  for (int32_t e = outer[j]; e < outer[j + 1]; ++e) {
+    if (inner[e] == j)
+      d = cadd(d, vals[e]);
+    else
+      *flag = 1;
+  }
+  diag[j] = d;
+}
+
+// sum_s Re(d_s) |psi_s|^2 and sum_s |psi_s|^2: block partials, then one block in fixed order
+constexpr int DX_THREADS = 256;
+__global__ void __launch_bounds__(DX_THREADS)
+    diag_expect_partial(int64_t n, const cplx* __restrict__ d, const cplx* __restrict__ psi, double2* __restrict__ part) {
+  __shared__ double2 sh[DX_THREADS / 64];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * DX_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * DX_THREADS) {
+    const double p = psi[i].x * psi[i].x + psi[i].y * psi[i].y;
+    a += d[i].x * p;
+    b += p;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = make_double2(a, b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 s = sh[0];
+    for (int w = 1; w < DX_THREADS / 64; ++w) {
+      s.x += sh[w].x;
+      s.y += sh[w].y;
+    }
+    part[blockIdx.x] = s;
+  }
+}
+__global__ void diag_expect_final(int nb, const double2* __restrict__ part, double2* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double2 s = make_double2(0.0, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    s.x += part[b].x;
+    s.y += part[b].y;
+  }
+  *out = s;
+}
+
+unsigned blocks_for(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+struct Model {
+  int nbits = 0;
+  int64_t n = 0;
+  int structured = 0;
+  qd_basis* basis = nullptr;
+  qd_termprog* progA = nullptr;
+  qd_csr* csrA = nullptr;
+  qd_linop* opA = nullptr;
+  cplx* d_diagD = nullptr;
+  size_t bytes = 0;
+  std::vector<qd_term> tA, tD;
+  std::vector<qd_op> oA, oD;
+};
+
+struct Job {
+  qd_request req;
+  double t_submit = 0.0;
+  size_t footprint = 0;
+  int bypassed = 0;
+};
+
+}  // namespace
+
+struct qd_server {
+  int device = 0;
+  size_t budget = 0;
+  size_t used = 0;  // models + admitted requests
+  qd_ctx* setup = nullptr;
+  std::vector<Model*> models;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job> queue;
+  std::vector<qd_result> results;
+  int64_t submitted = 0, finished = 0, rejected = 0;
+  int running = 0, max_running = 0;
+  size_t peak_used = 0;
+  bool stopping = false;
+  std::vector<std::thread> threads;
+  std::chrono::steady_clock::time_point t0;
+  std::string err;
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+size_t request_footprint(const Model& M, const qd_request& r) {
+  const size_t vec = sizeof(cplx) * (size_t)M.n;
+  const size_t slack = 8u << 20;
+  if (r.kind == QD_REQ_GROUND_STATE) return (size_t)(r.k + 6) * vec + slack;  // basis k+1, scratch 2, psi, diag
+  return (size_t)(r.k + 8) * vec + slack;  // basis, scratch, psi, next, diag
+}
+
+// <D> in the normalised state psi and |psi|^2
+int diag_expect(qd_ctx* ctx, int64_t n, const cplx* d, const cplx* psi, double* dexp, double* nrm2) {
+  const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + DX_THREADS - 1) / DX_THREADS));
+  double2* part = nullptr;
+  QD_HIP(ctx, hipMalloc(&part, sizeof(double2) * (nb + 1)));
+  diag_expect_partial<<<nb, DX_THREADS, 0, ctx->stream>>>(n, d, psi, part);
+  diag_expect_final<<<1, 64, 0, ctx->stream>>>(nb, part, part + nb);
+  double2 h;
+  hipError_t e1 = hipMemcpyAsync(&h, part + nb, sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  hipFree(part);
+  if (e1 != hipSuccess || e2 != hipSuccess) return qd_set_error(ctx, QD_E_HIP, "diag_expect");
+  *dexp = h.y > 0.0 ? h.x / h.y : 0.0;
+  *nrm2 = h.y;
+  return QD_OK;
+}
+
+int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
+  const int64_t n = M.n;
+  cplx *diag = nullptr, *psi = nullptr;
+  qd_linop* op = nullptr;
+  qd_krylov* kr = nullptr;
+  int rc = QD_OK;
+  auto done = [&](int code) {
+    if (kr) qd_krylov_destroy(kr);
+    if (op) qd_linop_destroy(op);
+    hipStreamSynchronize(ctx->stream);
+    if (diag) hipFree(diag);
+    if (psi) hipFree(psi);
+    return code;
+  };
+  if (hipMalloc(&diag, sizeof(cplx) * n) != hipSuccess || hipMalloc(&psi, sizeof(cplx) * n) != hipSuccess)
+    return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
+  scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
+  if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
+  if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
+  if ((rc = qd_krylov_create(ctx, n, r.k, 1, &kr))) return done(rc);
+  double E = 0.0, Eprev = 0.0;
+  bool have_prev = false;
+  qd_krylov_info info{};
+  if (r.tol > 0.0) {
+    const int chunk = r.chunk > 0 ? r.chunk : 10;
+    for (;;) {
+      if ((rc = qd_krylov_ground_state(kr, op, (double*)psi, chunk, &E))) return done(rc);
+      qd_krylov_get_info(kr, &info);
+      if (have_prev && std::fabs(E - Eprev) <= r.tol * std::fabs(E)) break;
+      if (info.act_dim >= r.k || info.breakdown) break;
+      Eprev = E;
+      have_prev = true;
+    }
+  } else {
+    if ((rc = qd_krylov_ground_state(kr, op, (double*)psi, -1, &E))) return done(rc);
+    qd_krylov_get_info(kr, &info);
+  }
+  double dexp = 0.0, nrm2 = 0.0;
+  if ((rc = diag_expect(ctx, n, M.d_diagD, psi, &dexp, &nrm2))) return done(rc);
+  out->energy = E;
+  out->energy_start = E;
+  out->observable = dexp;
+  out->iterations = info.act_dim;
+  return done(QD_OK);
+}
+
+int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
+  const int64_t n = M.n;
+  cplx *diag = nullptr, *psi = nullptr, *nxt = nullptr;
+  qd_linop* op = nullptr;
+  qd_termprog* prog = nullptr;
+  qd_krylov* kr = nullptr;
+  int rc = QD_OK;
+  auto done = [&](int code) {
+    if (kr) qd_krylov_destroy(kr);
+    if (op) qd_linop_destroy(op);
+    if (prog) qd_termprog_destroy(prog);
+    hipStreamSynchronize(ctx->stream);
+    if (diag) hipFree(diag);
+    if (psi) hipFree(psi);
+    if (nxt) hipFree(nxt);
+    return code;
+  };
+  if (hipMalloc(&psi, sizeof(cplx) * n) != hipSuccess || hipMalloc(&nxt, sizeof(cplx) * n) != hipSuccess)
+    return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
+  if (M.structured) {
+    // H = A + c D as one program: the matrix-free product recognises the
+    // two-block (Kronecker) form of spinful lattice models
+    std::vector<qd_term> terms = M.tA;
+    std::vector<qd_op> ops = M.oA;
+    for (qd_term t : M.tD) {
+      t.coeff[0] *= r.coeff;
+      t.coeff[1] *= r.coeff;
+      t.op_begin += (int32_t)M.oA.size();
+      terms.push_back(t);
+    }
+    ops.insert(ops.end(), M.oD.begin(), M.oD.end());
+    if ((rc = qd_termprog_create(ctx, M.nbits, terms.data(), (int)terms.size(), ops.data(), (int)ops.size(), &prog)))
+      return done(rc);
+    if ((rc = qd_linop_matfree(ctx, M.basis, prog, &op))) return done(rc);
+  } else {
+    if (hipMalloc(&diag, sizeof(cplx) * n) != hipSuccess) return done(qd_set_error(ctx, QD_E_OOM, "request diagonal"));
+    scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
+    if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
+  }
+  if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
+  if ((rc = qd_krylov_create(ctx, n, r.k, 0, &kr))) return done(rc);
+  double h0[2] = {0.0, 0.0}, n0 = 0.0;
+  if ((rc = qd_expect(ctx, op, (const double*)psi, h0))) return done(rc);
+  if ((rc = qd_norm2(ctx, n, (const double*)psi, &n0))) return done(rc);
+  for (int s = 0; s < r.steps; ++s) {
+    if ((rc = qd_krylov_evolve(kr, op, (const double*)psi, (double*)nxt, r.dt, r.k))) return done(rc);
+    std::swap(psi, nxt);
+  }
+  double h1[2] = {0.0, 0.0}, n1 = 0.0, dexp = 0.0;
+  if ((rc = qd_expect(ctx, op, (const double*)psi, h1))) return done(rc);
+  if ((rc = diag_expect(ctx, n, M.d_diagD, psi, &dexp, &n1))) return done(rc);
+  out->energy_start = h0[0] / n0;
+  out->energy = h1[0] / n1;
+  out->observable = dexp;
+  out->iterations = r.steps;
+  return done(QD_OK);
+}
+
+// Takes the first queued request that fits (arrival order; the head may be
+// passed at most QD_SERVE_MAX_BYPASS times).  Requests that can never fit are
+// answered with QD_E_OOM.  Called with the lock held.
+bool pick(qd_server* S, Job* job) {
+  size_t model_bytes = 0;
+  for (Model* m : S->models) model_bytes += m->bytes;
+  bool head_waiting = false;
+  for (auto it = S->queue.begin(); it != S->queue.end();) {
+    if (it->footprint + model_bytes > S->budget) {
+      qd_result res{};
+      res.tag = it->req.tag;
+      res.status = QD_E_OOM;
+      res.t_submit = it->t_submit;
+      res.t_start = res.t_end = S->now();
+      res.worker = -1;
+      S->results.push_back(res);
+      ++S->finished;
+      ++S->rejected;
+      it = S->queue.erase(it);
+      S->cv.notify_all();
+      continue;
+    }
+    if (it->footprint <= S->budget - S->used) {
+      if (head_waiting) {
+        Job& head = S->queue.front();
+        if (head.bypassed >= QD_SERVE_MAX_BYPASS) return false;
+        ++head.bypassed;
+      }
+      *job = *it;
+      S->queue.erase(it);
+      S->used += job->footprint;
+      S->peak_used = std::max(S->peak_used, S->used);
+      return true;
+    }
+    if (it == S->queue.begin()) head_waiting = true;
+    ++it;
+  }
+  return false;
+}
+
+void worker_main(qd_server* S, int wi) {
+  qd_ctx* ctx = nullptr;
+  int crc = qd_ctx_create(S->device, &ctx);
+  for (;;) {
+    Job job;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (;;) {
+        if (crc == QD_OK && pick(S, &job)) break;
+        if (S->stopping && S->queue.empty()) {
+          lk.unlock();
+          if (ctx) qd_ctx_destroy(ctx);
+          return;
+        }
+        if (crc != QD_OK && !S->queue.empty()) {
+          // no device context on this worker: answer with the error
+          job = S->queue.front();
+          S->queue.pop_front();
+          qd_result res{};
+          res.tag = job.req.tag;
+          res.status = crc;
+          res.t_submit = job.t_submit;
+          res.t_start = res.t_end = S->now();
+          res.worker = wi;
+          S->results.push_back(res);
+          ++S->finished;
+          S->cv.notify_all();
+          continue;
+        }
+        S->cv.wait(lk);
+      }
+      ++S->running;
+      S->max_running = std::max(S->max_running, S->running);
+    }
+    qd_result res{};
+    res.tag = job.req.tag;
+    res.t_submit = job.t_submit;
+    res.t_start = S->now();
+    res.worker = wi;
+    const Model& M = *S->models[job.req.model];
+    res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(ctx, M, job.req, &res)
+                                                     : run_evolve(ctx, M, job.req, &res);
+    res.t_end = S->now();
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->used -= job.footprint;
+      --S->running;
+      S->results.push_back(res);
+      ++S->finished;
+    }
+    S->cv.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" int qd_server_create(int device, size_t hbm_budget, int workers, qd_server** out) {
+  if (!out || workers < 1 || workers > 64 || hbm_budget == 0) return QD_E_ARG;
+  *out = nullptr;
+  qd_server* S = new qd_server();
+  S->device = device;
+  S->budget = hbm_budget;
+  S->t0 = std::chrono::steady_clock::now();
+  int rc = qd_ctx_create(device, &S->setup);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  for (int w = 0; w < workers; ++w) S->threads.emplace_back(worker_main, S, w);
+  *out = S;
+  return QD_OK;
+}
+
+extern "C" const char* qd_server_last_error(qd_server* s) { return s ? s->err.c_str() : "null server"; }
+
+extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint* cons, int ncons, const qd_term* tA,
+                                   int ntA, const qd_op* oA, int noA, const qd_term* tD, int ntD, const qd_op* oD,
+                                   int noD, int structured, int* model_id) {
+  if (!S || !model_id || !tA || !oA || (ntD && (!tD || !oD))) return QD_E_ARG;
+  qd_ctx* ctx = S->setup;
+  Model* M = new Model();
+  M->nbits = nbits;
+  M->structured = structured;
+  M->tA.assign(tA, tA + ntA);
+  M->oA.assign(oA, oA + noA);
+  if (ntD) {
+    M->tD.assign(tD, tD + ntD);
+    M->oD.assign(oD, oD + noD);
+  }
+  auto fail = [&](int code) {
+    S->err = qd_last_error(ctx);
+    if (M->opA) qd_linop_destroy(M->opA);
+    if (M->csrA) qd_csr_destroy(M->csrA);
+    if (M->progA) qd_termprog_destroy(M->progA);
+    if (M->basis) qd_basis_destroy(M->basis);
+    hipStreamSynchronize(ctx->stream);
+    if (M->d_diagD) hipFree(M->d_diagD);
+    delete M;
+    return code;
+  };
+  int rc = qd_basis_create(ctx, nbits, cons, ncons, &M->basis);
+  if (rc) return fail(rc);
+  M->n = qd_basis_size(M->basis);
+  if ((rc = qd_termprog_create(ctx, nbits, tA, ntA, oA, noA, &M->progA))) return fail(rc);
+  if ((rc = qd_csr_build(ctx, M->basis, M->progA, &M->csrA))) return fail(rc);
+  if ((rc = qd_linop_from_csr(ctx, M->csrA, &M->opA))) return fail(rc);
+  if (hipMalloc(&M->d_diagD, sizeof(cplx) * M->n) != hipSuccess) return fail(qd_set_error(ctx, QD_E_OOM, "model diagonal"));
+  if (ntD) {
+    qd_termprog* pD = nullptr;
+    qd_csr* cD = nullptr;
+    if ((rc = qd_termprog_create(ctx, nbits, tD, ntD, oD, noD, &pD))) return fail(rc);
+    rc = qd_csr_build(ctx, M->basis, pD, &cD);
+    if (rc) {
+      qd_termprog_destroy(pD);
+      return fail(rc);
+    }
+    int* flag = nullptr;
+    int hflag = 0;
+    if (hipMalloc(&flag, sizeof(int)) == hipSuccess) {
+      hipMemsetAsync(flag, 0, sizeof(int), ctx->stream);
+      extract_diag_kernel<<<blocks_for(M->n, 256), 256, 0, ctx->stream>>>(M->n, cD->d_outer, cD->d_inner, cD->d_vals,
+                                                                          M->d_diagD, flag);
+      hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+      hipStreamSynchronize(ctx->stream);
+      hipFree(flag);
+    } else {
+      hflag = 2;
+    }
+    qd_csr_destroy(cD);
+    qd_termprog_destroy(pD);
+    if (hflag) return fail(qd_set_error(ctx, hflag == 1 ? QD_E_ARG : QD_E_OOM, "the second operator must be diagonal"));
+  } else {
+    hipMemsetAsync(M->d_diagD, 0, sizeof(cplx) * M->n, ctx->stream);
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(qd_set_error(ctx, QD_E_HIP, "model set-up"));
+  const qd_csr* c = M->csrA;
+  M->bytes = sizeof(uint64_t) * (size_t)M->n + sizeof(int32_t) * (size_t)(M->n + 1) +
+             (sizeof(int32_t) + sizeof(cplx)) * (size_t)c->nnz + sizeof(cplx) * (size_t)M->n;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *model_id = (int)S->models.size();
+  S->models.push_back(M);
+  S->used += M->bytes;
+  S->peak_used = std::max(S->peak_used, S->used);
+  return QD_OK;
+}
+
+extern "C" int qd_server_submit(qd_server* S, const qd_request* r) {
+  if (!S || !r) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (r->model < 0 || r->model >= (int)S->models.size() || r->k < 2 ||
+      (r->kind != QD_REQ_GROUND_STATE && r->kind != QD_REQ_EVOLVE) || (r->kind == QD_REQ_EVOLVE && r->steps < 0)) {
+    S->err = "qd_server_submit: bad request";
+    return QD_E_ARG;
+  }
+  Job j;
+  j.req = *r;
+  j.t_submit = S->now();
+  j.footprint = request_footprint(*S->models[r->model], *r);
+  S->queue.push_back(j);
+  ++S->submitted;
+  S->cv.notify_all();
+  return QD_OK;
+}
+
+extern "C" int qd_server_poll(qd_server* S, qd_result* out, int cap, int* count) {
+  if (!S || !count || (cap > 0 && !out)) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  int k = std::min<int>(cap, (int)S->results.size());
+  for (int i = 0; i < k; ++i) out[i] = S->results[i];
+  S->results.erase(S->results.begin(), S->results.begin() + k);
+  *count = k;
+  return QD_OK;
+}
+
+extern "C" int qd_server_drain(qd_server* S) {
+  if (!S) return QD_E_ARG;
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->cv.wait(lk, [&] { return S->finished == S->submitted; });
+  return QD_OK;
+}
+
+extern "C" double qd_server_clock(qd_server* S) { return S ? S->now() : 0.0; }
+
+extern "C" int qd_server_get_stats(qd_server* S, qd_server_stats* st) {
+  if (!S || !st) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  st->submitted = S->submitted;
+  st->finished = S->finished;
+  st->rejected = S->rejected;
+  st->queued = (int64_t)S->queue.size();
+  st->running = S->running;
+  st->max_running = S->max_running;
+  st->bytes_in_use = (int64_t)S->used;
+  st->peak_bytes = (int64_t)S->peak_used;
+  return QD_OK;
+}
+
+extern "C" int qd_server_destroy(qd_server* S) {
+  if (!S) return QD_E_ARG;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+  }
+  S->cv.notify_all();
+  for (auto& t : S->threads) t.join();
+  for (Model* M : S->models) {
+    if (M->opA) qd_linop_destroy(M->opA);
+    if (M->csrA) qd_csr_destroy(M->csrA);
+    if (M->progA) qd_termprog_destroy(M->progA);
+    if (M->basis) qd_basis_destroy(M->basis);
+    if (M->d_diagD) hipFree(M->d_diagD);
+    delete M;
+  }
+  if (S->setup) qd_ctx_destroy(S->setup);
+  delete S;
+  return QD_OK;
+}

@@ -209,6 +209,10 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
   qd_ctx* ctx = op->ctx;
   int64_t n = op->n;
   if (n == 0) return QD_OK;
+  if (op->diag0) {
+    if (d_diag) return qd_set_error(ctx, QD_E_UNSUPPORTED, "two diagonals on one product");
+    d_diag = op->diag0;
+  }
   if (op->kind == QD_LINOP_CSR && op->lanes_per_row <= 0) {
     // CSR-stream: 4 waves x 64 rows per block; variant = -lanes_per_row
     unsigned grid = grid8((n + 255) / 256);
@@ -325,6 +329,21 @@ extern "C" int qd_linop_set_structured(qd_linop* op, int on, int* has_structured
 extern "C" int qd_linop_destroy(qd_linop* op) {
   if (op && op->kron) qd_kron_destroy(op->kron);
   delete op;
+  return QD_OK;
+}
+
+// A view of `base` that adds a diagonal to every product: (base + diag(d)) x.
+// It borrows base's arrays and d_diag (n complex values); both must outlive it.
+extern "C" int qd_linop_with_diagonal(qd_ctx* ctx, qd_linop* base, const double* d_diag, qd_linop** out) {
+  if (!ctx || !base || !d_diag || !out) return qd_set_error(ctx, QD_E_ARG, "qd_linop_with_diagonal: bad arguments");
+  if (base->diag0) return qd_set_error(ctx, QD_E_UNSUPPORTED, "the base operator already carries a diagonal");
+  qd_linop* op = new qd_linop(*base);
+  op->ctx = ctx;
+  op->kron = nullptr;  // the Kronecker kernels take no diagonal; the generic kernels do
+  op->use_kron = 0;
+  op->diag0 = reinterpret_cast<const cplx*>(d_diag);
+  op->view = true;
+  *out = op;
   return QD_OK;
 }
 

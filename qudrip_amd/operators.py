@@ -363,3 +363,35 @@ def hubbard(nsite, bonds, t=1.0, U=4.0):
 
 def hubbard_sector(idx, nsite, nup, ndn):
     return Constrain(idx, getNup(idx, nsite) == nup, getNdo(idx, nsite) == ndn)
+
+
+def xxz_parts(N, Jxy=1.0, pbc=True):
+    """XXZ chain split as H(Jz) = A + Jz D: A = sum_i Jxy/2 (S+_{i+1} S-_i + h.c.),
+    D = sum_i Sz_{i+1} Sz_i (diagonal), with the terms of xxz_chain."""
+    idx = getQbits(N)
+    gc, ga, gz = getBoseGate(idx), getBoseGate(idx), getBoseGate(idx)
+    gc << 0.5 * (pauli_x + II * pauli_y)
+    ga << 0.5 * (pauli_x - II * pauli_y)
+    gz << 0.5 * pauli_z
+    A = getEmptyOperator()
+    D = getEmptyOperator()
+    for i in range(N if pbc else N - 1):
+        j = (i + 1) % N
+        A = A + (0.5 * Jxy) * (gc(j) * ga(i) + gc(i) * ga(j))
+        D = D + gz(j) * gz(i)
+    return idx, A, D
+
+
+def hubbard_parts(nsite, bonds, t=1.0):
+    """Hubbard model split as H(U) = T + U D: T = -t sum_<ij>,s (c+_is c_js + h.c.),
+    D = sum_i n_i,up n_i,dn (diagonal), with the terms of hubbard()."""
+    idx = getQbits(2 * nsite)
+    g = getHopFermiGate(idx)
+    T = getEmptyOperator()
+    for (a, b) in bonds:
+        for off in (0, nsite):
+            T = T + (-t) * g(a + off, b + off) + (-t) * g(b + off, a + off)
+    D = getEmptyOperator()
+    for i in range(nsite):
+        D = D + g(i, i) * g(i + nsite, i + nsite)
+    return idx, T, D

@@ -1,0 +1,92 @@
+"""Solve server (qd_server_*) on the GPU: results against the plain one-at-a-
+time path and exact diagonalisation, identity alone vs in the mix, and
+admission by the HBM budget (deferral and rejection)."""
+import numpy as np
+import pytest
+
+from qudrip_amd import abi
+from qudrip_amd import operators as Q
+from qudrip_amd import serving as SV
+from tests import physics as P
+
+pytestmark = pytest.mark.gpu
+
+LATTICE = (6, Q.hubbard_bonds_square(2, 3), 3, 3)  # 2 x 3 periodic, N_up = N_dn = 3 (n = 400)
+
+
+def _reqs():
+    out = []
+    for i, (N, jz) in enumerate([(10, 0.3), (12, 1.0), (10, 1.7), (12, 0.05), (10, 1.0)]):
+        out.append({"tag": i, "arrival": 0.002 * i, "kind": "gs", "N": N, "coeff": jz, "seed": 100 + i})
+    for i, u in enumerate([3.0, 6.5, 2.2]):
+        out.append({"tag": 10 + i, "arrival": 0.001 + 0.002 * i, "kind": "rt", "coeff": u, "seed": 200 + i})
+    return out
+
+
+KW = dict(gs_k=80, gs_chunk=10, gs_tol=1e-13, rt_k=20, rt_steps=25, dt=0.05)
+
+
+def _dense_e0(N, jz):
+    H = P.dense_xxz_full(N, 1.0, jz, True)
+    st = P.sector_states(N, [((1 << N) - 1, N // 2)])
+    return np.linalg.eigvalsh(P.restrict(H, st))[0]
+
+
+def test_served_matches_plain_and_exact(ctx):
+    reqs = _reqs()
+    print("serving mix of", len(reqs), flush=True)
+    served = SV.run_served(reqs, lattice=LATTICE, workers=4, hbm_budget=8 << 30, **KW)
+    plain = SV.run_plain(reqs, ctx, lattice=LATTICE, gs_k=KW["gs_k"], rt_k=KW["rt_k"], rt_steps=KW["rt_steps"],
+                         dt=KW["dt"])
+    for r in reqs:
+        s, p = served["results"][r["tag"]], plain["results"][r["tag"]]
+        assert s["status"] == 0
+        if r["kind"] == "gs":
+            e = _dense_e0(r["N"], r["coeff"])
+            assert abs(s["energy"] - e) <= 1e-10 * abs(e), (r, s["energy"], e)
+            assert abs(p["energy"] - e) <= 1e-10 * abs(e)
+            assert s["iterations"] < KW["gs_k"] or s["iterations"] == KW["gs_k"]
+        else:
+            assert abs(s["energy"] - p["energy"]) <= 1e-10 * abs(p["energy"])
+            assert abs(s["observable"] - p["observable"]) <= 1e-10
+            assert abs(s["energy"] - s["energy_start"]) <= 1e-9 * abs(s["energy_start"])
+    assert served["stats"]["max_running"] >= 2
+
+
+def test_result_same_alone_and_in_the_mix(ctx):
+    reqs = _reqs()
+    mix = SV.run_served(reqs, lattice=LATTICE, workers=4, hbm_budget=8 << 30, **KW)["results"]
+    for r in reqs:
+        alone = SV.run_served([dict(r, arrival=0.0)], lattice=LATTICE, workers=1, hbm_budget=8 << 30,
+                              **KW)["results"][r["tag"]]
+        for key in ("energy", "energy_start", "observable"):
+            assert abs(alone[key] - mix[r["tag"]][key]) <= 1e-12 * max(1.0, abs(alone[key])), (r, key)
+        assert alone["iterations"] == mix[r["tag"]]["iterations"]
+
+
+def test_admission_defers_and_rejects(ctx):
+    idx, A, D = Q.xxz_parts(12, 1.0, pbc=True)
+    sec = Q.sz_sector(idx, 12, 6)
+    probe = SV.Server(hbm_budget=1 << 30, workers=1)
+    probe.add_model(sec, A, D)
+    model_bytes = probe.stats()["bytes_in_use"]
+    probe.close()
+    n = 924
+    fp = (60 + 6) * n * 16 + (8 << 20)  # footprint of a k = 60 ground-state request
+    srv = SV.Server(hbm_budget=model_bytes + fp + fp // 2, workers=4)  # room for one request at a time
+    try:
+        m = srv.add_model(sec, A, D)
+        for i in range(5):
+            srv.submit(abi.QD_REQ_GROUND_STATE, m, 0.5 + 0.1 * i, 60, chunk=10, tol=1e-12, seed=7, tag=i)
+        srv.submit(abi.QD_REQ_GROUND_STATE, m, 1.0, 200000, seed=7, tag=99)  # can never fit
+        srv.drain()
+        res = {x["tag"]: x for x in srv.poll()}
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert res[99]["status"] == 2  # QD_E_OOM
+    assert all(res[i]["status"] == 0 for i in range(5))
+    assert st["max_running"] == 1 and st["rejected"] == 1 and st["finished"] == 6
+    for i in range(5):
+        e = _dense_e0(12, 0.5 + 0.1 * i)
+        assert abs(res[i]["energy"] - e) <= 1e-10 * abs(e)
