@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--no-quench", action="store_true", help="skip the Hubbard 4x4 quench workload")
     ap.add_argument("--quench-steps", type=int, default=200)
     ap.add_argument("--no-traj", action="store_true", help="skip the dissipative trajectory workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests in the Poisson stream")
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--traj", type=int, default=4096, help="number of trajectories")
     ap.add_argument("--traj-cpu", type=int, default=8, help="trajectories in the oracle's CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -171,6 +174,9 @@ def main():
     if not args.no_traj:
         result["hubbard_l8_trajectories"] = trajectories(ctx, args)
 
+    if not args.no_serve:
+        result["serving"] = serving(ctx, args)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(csr, n, nnz, B, x, args.cpu_seconds)
 
@@ -212,6 +218,65 @@ def trajectories(ctx, args):
                                          "restatement with operators precomputed per sector, 1 thread"}
     log(f"trajectories: N(T) = {out['N_T']} vs {out['N_T_closed_form']} (z={out['closed_form_z']})")
     return out
+
+
+def serving(ctx, args):
+    """Poisson stream (200 req/s, seed 4): 70 % XXZ ground states (N in
+    16..26, Jz ~ U[0,2]), 30 % 3x4 Hubbard real-time solves (U ~ U[2,8], 100
+    steps of 0.05).  Served: the solve server (structure cache, concurrent
+    workers, ground states stopped when E0 moves by <= 1e-12 |E0| over 10
+    Krylov vectors, matrix-free real-time operator).  Plain: each request
+    alone through the drop-in path (basis + stored matrix per request,
+    Lanczos k = 150, evolve with kry_dim_rt = 30).  Accuracy: served vs plain
+    energies per request."""
+    import numpy as np
+
+    from qudrip_amd import serving as SV
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+    reqs = SV.request_stream(args.serve_requests, rate=200.0, seed=4)
+    served = SV.run_served(reqs, workers=args.serve_workers, log=log)
+    log(f"served: {len(reqs)} requests in {served['span_s']:.2f} s ({served['solves_per_s']:.2f} /s)")
+    plain = SV.run_plain(reqs, ctx)
+    log(f"plain: {plain['busy_s']:.2f} s busy ({plain['solves_per_s']:.2f} /s)")
+    rel_gs, d_rt, drift = [], [], []
+    for r in reqs:
+        s, p = served["results"][r["tag"]], plain["results"][r["tag"]]
+        if s["status"] != 0:
+            continue
+        if r["kind"] == "gs":
+            rel_gs.append(abs(s["energy"] - p["energy"]) / abs(p["energy"]))
+        else:
+            d_rt.append(abs(s["observable"] - p["observable"]))
+            drift.append(abs(s["energy"] - s["energy_start"]) / abs(s["energy_start"]))
+    sp50, sp99 = SV.latency_percentiles(served["results"])
+    pp50, pp99 = SV.latency_percentiles(plain["results"])
+    its = [served["results"][r["tag"]]["iterations"] for r in reqs if r["kind"] == "gs"]
+    rt_m = [served["results"][r["tag"]]["krylov_mean"] for r in reqs if r["kind"] == "rt"]
+    busy = {}
+    for r in reqs:
+        x = served["results"][r["tag"]]
+        key = f"gs_N{r['N']}" if r["kind"] == "gs" else "rt_3x4"
+        b = busy.setdefault(key, [0, 0.0, 0.0])
+        b[0] += 1
+        b[1] += x["t_end"] - x["t_start"]
+        b[2] += plain["results"][r["tag"]]["time_s"]
+    return {"requests": len(reqs), "rate_per_s": 200.0, "gs_fraction": 0.7,
+            "served": {"solves_per_s": round(served["solves_per_s"], 3), "span_s": round(served["span_s"], 3),
+                       "p50_latency_s": round(sp50, 4), "p99_latency_s": round(sp99, 4),
+                       "workers": args.serve_workers, "max_running": served["stats"]["max_running"],
+                       "peak_hbm_gb": round(served["stats"]["peak_bytes"] / 1e9, 2),
+                       "model_setup_s": round(served["setup_s"], 3),
+                       "gs_krylov_dim_mean": round(float(np.mean(its)), 1) if its else None,
+                       "rt_krylov_dim_mean": round(float(np.mean(rt_m)), 2) if rt_m else None,
+                       "per_kind_count_served_busy_s_plain_s": {k: [v[0], round(v[1], 3), round(v[2], 3)]
+                                                               for k, v in sorted(busy.items())},
+                       "failed": sum(1 for x in served["results"].values() if x["status"] != 0)},
+            "plain": {"solves_per_s": round(plain["solves_per_s"], 3), "busy_s": round(plain["busy_s"], 3),
+                      "p50_latency_s": round(pp50, 4), "p99_latency_s": round(pp99, 4)},
+            "speedup": round(served["solves_per_s"] / plain["solves_per_s"], 3),
+            "max_rel_dE_gs": max(rel_gs) if rel_gs else None,
+            "max_abs_d_docc_rt": max(d_rt) if d_rt else None,
+            "max_rel_energy_drift_rt": max(drift) if drift else None}
 
 
 def cpu_baseline(csr, n, nnz, B, x, budget_s):

@@ -225,12 +225,24 @@ int qd_krylov_ground_state(qd_krylov* kr, qd_linop* op, double* d_psi, int add_d
  * d_psi_out may equal d_psi_in. */
 int qd_krylov_evolve(qd_krylov* kr, qd_linop* op, const double* d_psi_in, double* d_psi_out,
                      double h, int kry_dim_rt);
+/* evolve with the Krylov dimension chosen per step (no reference
+ * counterpart; the reference fixes kry_dim_rt): the basis grows from m_min by
+ * 2 vectors until Saad's a-posteriori estimate of the step error,
+ * h_{m+1,m} |[expm(-i h K_m)]_{m,1}|, is <= tol, or it holds kry_max - 1
+ * vectors.  m_used (may be NULL) receives the dimension used, m_enough (may
+ * be NULL) the smallest leading dimension whose estimate met tol, a start for
+ * the next step. */
+int qd_krylov_evolve_tol(qd_krylov* kr, qd_linop* op, const double* d_psi_in, double* d_psi_out, double h,
+                         int m_min, int kry_max, double tol, int* m_used, int* m_enough);
 typedef struct {
   int act_dim;      /* ArnoldiSolver::act_dim() */
   int breakdown;    /* 1 if the last expansion stopped on |w| < 1e-10 */
   double error;     /* ArnoldiSolver::error() */
   double beta_last; /* trailing |w| of the last expansion */
   int64_t n_spmv;   /* operator applications so far */
+  double ritz_resid; /* |(H - E0) psi| of the last ground-state Ritz pair,
+                        beta_m |z_m| (no reference counterpart) */
+  double ritz_gap;   /* second minus lowest Ritz value of that solve */
 } qd_krylov_info;
 int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info);
 /* Ritz values of the last ground-state solve, ascending (storeE, :587-593).
@@ -297,8 +309,11 @@ typedef struct {
   double coeff;   /* H = A + coeff * D */
   int32_t k;      /* ground state: Krylov dimension limit; evolve: kry_dim_rt */
   int32_t chunk;  /* ground state: basis vectors added per getGroundState call */
-  double tol;     /* ground state: stop when E0 moves by <= tol |E0| over one chunk
-                   * (0: one call with the full dimension, like the plain solver) */
+  double tol;     /* ground state: stop when the Ritz-residual bound resid^2 / gap
+                   * or the move of E0 over one chunk is <= tol |E0|
+                   * (0: one call with the full dimension, like the plain solver);
+                   * evolve: step-error bound of qd_krylov_evolve_tol with m_min =
+                   * chunk (0: qd_krylov_evolve with kry_dim_rt = k) */
   int32_t steps;  /* evolve: number of time steps */
   double dt;      /* evolve: time step */
   uint64_t seed;  /* start vector: qd_seeded_state(seed) */
@@ -312,6 +327,7 @@ typedef struct {
   double energy_start; /* evolve: <H> / <psi|psi> of the start vector */
   double observable;   /* <D> / <psi|psi> of the final state */
   double t_submit, t_start, t_end; /* seconds on the server's clock */
+  double krylov_mean;  /* Krylov dimension used (mean over steps for evolve) */
   int32_t worker;
 } qd_result;
 typedef struct {

@@ -61,7 +61,7 @@ class qd_result(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_int64), ("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
                 ("energy", ctypes.c_double), ("energy_start", ctypes.c_double), ("observable", ctypes.c_double),
                 ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
-                ("worker", ctypes.c_int32)]
+                ("krylov_mean", ctypes.c_double), ("worker", ctypes.c_int32)]
 
 
 class qd_server_stats(ctypes.Structure):
@@ -72,7 +72,8 @@ class qd_server_stats(ctypes.Structure):
 
 class qd_krylov_info(ctypes.Structure):
     _fields_ = [("act_dim", ctypes.c_int), ("breakdown", ctypes.c_int), ("error", ctypes.c_double),
-                ("beta_last", ctypes.c_double), ("n_spmv", ctypes.c_int64)]
+                ("beta_last", ctypes.c_double), ("n_spmv", ctypes.c_int64), ("ritz_resid", ctypes.c_double),
+                ("ritz_gap", ctypes.c_double)]
 
 
 VP = ctypes.c_void_p
@@ -125,6 +126,8 @@ SIGNATURES = {
     "qd_krylov_destroy": (I, [VP]),
     "qd_krylov_ground_state": (I, [VP, VP, VP, I, DP]),
     "qd_krylov_evolve": (I, [VP, VP, VP, VP, ctypes.c_double, I]),
+    "qd_krylov_evolve_tol": (I, [VP, VP, VP, VP, ctypes.c_double, I, I, ctypes.c_double, ctypes.POINTER(I),
+                                 ctypes.POINTER(I)]),
     "qd_krylov_get_info": (I, [VP, ctypes.POINTER(qd_krylov_info)]),
     "qd_krylov_eigenvalues": (I, [VP, DP, I, ctypes.POINTER(I)]),
     "qd_krylov_converge_ev": (I, [VP, I, ctypes.c_double, ctypes.c_double]),

@@ -753,11 +753,20 @@ static int kr_expand_cgs2(qd_krylov* kr, qd_linop* op, int add) {
 
 extern "C" int qd_krylov_ground_state(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, double* h_E0) {
   if (!kr || !op || !d_psi || op->n != kr->n) return qd_set_error(kr ? kr->ctx : nullptr, QD_E_ARG, "qd_krylov_ground_state: bad arguments");
-  qd_ctx* ctx = kr->ctx;
   if (kr->act_dim >= kr->kry_dim && !kr->evals.empty()) {  // :567
     if (h_E0) *h_E0 = kr->evals[0];
     return QD_OK;
   }
+  return qd_krylov_gs_step(kr, op, d_psi, add_dim, h_E0, true);
+}
+
+// Extends the basis by add_dim (-1: to kry_dim; clamped) and solves the Ritz
+// problem; the lowest Ritz vector overwrites d_psi only if want_vector (a
+// caller that checks convergence between extensions asks for it once, at the
+// end, instead of paying a pass over the basis per check).
+int qd_krylov_gs_step(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, double* h_E0, bool want_vector) {
+  if (!kr || !op || !d_psi || op->n != kr->n) return qd_set_error(kr ? kr->ctx : nullptr, QD_E_ARG, "qd_krylov_gs_step: bad arguments");
+  qd_ctx* ctx = kr->ctx;
   int rc;
   if (kr->act_dim == 0) {
     rc = kr_init(kr, (const cplx*)d_psi);
@@ -777,6 +786,13 @@ extern "C" int qd_krylov_ground_state(qd_krylov* kr, qd_linop* op, double* d_psi
   rc = qd_tridiag_eig(m, dg.data(), e.data(), w.data(), z.data(), 1);
   if (rc) return qd_set_error(ctx, QD_E_EIGEN, "failed to solve the Krylov matrix");
   kr->evals = w;
+  // residual of the lowest Ritz pair: beta_m |z_{m-1,0}| (exact for the Krylov relation)
+  kr->ritz_resid = kr->beta_last * std::fabs(z[(size_t)m - 1]);
+  kr->ritz_gap = m > 1 ? w[1] - w[0] : 0.0;
+  if (!want_vector) {
+    if (h_E0) *h_E0 = w[0];
+    return QD_OK;
+  }
   // psi = V[:, :m] y0  (getState, :450-473)
   KrylovDev d = kr_layout(kr);
   std::vector<cplx> y(m);
@@ -844,6 +860,78 @@ extern "C" int qd_krylov_evolve(qd_krylov* kr, qd_linop* op, const double* d_psi
   return QD_OK;
 }
 
+// Real-time step with the Krylov dimension chosen per step: the basis grows
+// from m_min by 2 vectors until the a-posteriori estimate of the step error,
+// h_{m+1,m} |[expm(-i h K_m)]_{m,1}| (Saad, SIAM J. Numer. Anal. 29 (1992)
+// 209, section 5), is at most tol, or the basis reaches kry_max - 1 vectors
+// or an invariant subspace.  psi_out = V_m expm(-i h K_m) e_1 as in evolve.
+// m_enough receives the smallest leading dimension whose estimate met tol
+// (the next step's m_min for a caller that steps repeatedly), or m_used.
+static double step_error_estimate(const qd_krylov* kr, int mp, double h, std::vector<cplx>& A, std::vector<cplx>& E,
+                                  double beta) {
+  const int K = kr->kry_dim;
+  A.assign((size_t)mp * mp, cmake(0.0, 0.0));
+  E.assign((size_t)mp * mp, cmake(0.0, 0.0));
+  for (int j = 0; j < mp; ++j)
+    for (int i = 0; i < mp; ++i) {
+      cplx kv = kr->KR[(size_t)j * K + i];
+      A[(size_t)j * mp + i] = cmake(h * kv.y, -h * kv.x);
+    }
+  if (qd_expm_pade13(mp, A.data(), E.data())) return -1.0;
+  const cplx em = E[(size_t)mp - 1];  // row mp-1 of column 0
+  return beta * std::sqrt(em.x * em.x + em.y * em.y);
+}
+
+extern "C" int qd_krylov_evolve_tol(qd_krylov* kr, qd_linop* op, const double* d_psi_in, double* d_psi_out,
+                                    double h, int m_min, int kry_max, double tol, int* m_used, int* m_enough) {
+  if (!kr || !op || !d_psi_in || !d_psi_out || op->n != kr->n || m_min < 1 || kry_max < 2 || !(tol >= 0.0))
+    return qd_set_error(kr ? kr->ctx : nullptr, QD_E_ARG, "qd_krylov_evolve_tol: bad arguments");
+  qd_ctx* ctx = kr->ctx;
+  int rc = kr_init(kr, (const cplx*)d_psi_in);
+  if (rc) return rc;
+  const int mmax = std::min(kry_max - 1, kr->kry_dim);
+  int target = std::min(m_min, mmax);
+  const int K = kr->kry_dim;
+  std::vector<cplx> A, E;
+  int m = 0;
+  for (;;) {
+    rc = kr_expand(kr, op, target - kr->act_dim);
+    if (rc) return rc;
+    m = kr->act_dim;
+    if (m < 1) return qd_set_error(ctx, QD_E_BREAKDOWN, "Krylov space is empty");
+    const double est = step_error_estimate(kr, m, h, A, E, kr->beta_last);
+    if (est < 0.0) return qd_set_error(ctx, QD_E_EIGEN, "expm of the Krylov matrix failed");
+    if (kr->breakdown || est <= tol || m >= mmax) break;
+    target = std::min(mmax, m + 2);
+  }
+  if (m_used) *m_used = m;
+  if (m_enough) {
+    // smallest leading block whose own estimate meets tol (E of the full m kept)
+    std::vector<cplx> A2, E2;
+    int me = m;
+    for (int mp = m - 1; mp >= 2; --mp) {
+      const double beta = kr->KR[(size_t)(mp - 1) * K + mp].x;  // h_{mp+1, mp}
+      const double est = step_error_estimate(kr, mp, h, A2, E2, beta);
+      if (est < 0.0 || est > tol) break;
+      me = mp;
+    }
+    *m_enough = me;
+  }
+  KrylovDev d = kr_layout(kr);
+  QD_HIP(ctx, hipMemcpyAsync(d.y, E.data(), sizeof(cplx) * m, hipMemcpyHostToDevice, ctx->stream));
+  int rpt = pick_rpt(kr->n, ctx->num_cus);
+  int nb = (int)kr_nb(kr, rpt);
+  if (rpt == 8)
+    combine_kernel<8><<<nb, KT, 0, ctx->stream>>>(kr->n, kr->d_V, kr->ld, m, d.y, (cplx*)d_psi_out);
+  else if (rpt == 4)
+    combine_kernel<4><<<nb, KT, 0, ctx->stream>>>(kr->n, kr->d_V, kr->ld, m, d.y, (cplx*)d_psi_out);
+  else
+    combine_kernel<1><<<nb, KT, 0, ctx->stream>>>(kr->n, kr->d_V, kr->ld, m, d.y, (cplx*)d_psi_out);
+  QD_CHECK_LAUNCH(ctx);
+  QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QD_OK;
+}
+
 extern "C" int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info) {
   if (!kr || !info) return QD_E_ARG;
   info->act_dim = kr->act_dim;
@@ -851,6 +939,8 @@ extern "C" int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info) {
   info->error = kr->err;
   info->beta_last = kr->beta_last;
   info->n_spmv = kr->n_spmv;
+  info->ritz_resid = kr->ritz_resid;
+  info->ritz_gap = kr->ritz_gap;
   return QD_OK;
 }
 

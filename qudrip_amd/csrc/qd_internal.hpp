@@ -264,6 +264,8 @@ struct qd_krylov {
   std::vector<cplx> evs_prev;  // convergeEV targets
   double err = 0.0;
   double beta_last = 0.0;
+  double ritz_resid = 0.0;  // |(H - E0) psi| of the last ground-state Ritz pair
+  double ritz_gap = 0.0;    // second minus lowest Ritz value
   int breakdown = 0;
   int64_t n_spmv = 0;
   int method = 1;  // 0 = CGS2, 1 = DCGS2
@@ -296,6 +298,7 @@ int qd_dev_dot_many(qd_ctx* ctx, int64_t n, const cplx* V, int64_t ld, int ncols
 // FP64 host numerics (host_linalg.cpp)
 int qd_tridiag_eig(int m, const double* d, const double* e, double* w, double* z, int want_z);
 int qd_expm_pade13(int m, const cplx* A, cplx* E);
+int qd_krylov_gs_step(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, double* h_E0, bool want_vector);
 
 // grid helpers
 static inline unsigned qd_grid(int64_t work, int per_block) {

@@ -184,13 +184,18 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result
   if (r.tol > 0.0) {
     const int chunk = r.chunk > 0 ? r.chunk : 10;
     for (;;) {
-      if ((rc = qd_krylov_ground_state(kr, op, (double*)psi, chunk, &E))) return done(rc);
+      if ((rc = qd_krylov_gs_step(kr, op, (double*)psi, chunk, &E, false))) return done(rc);
       qd_krylov_get_info(kr, &info);
+      // Ritz-residual bound of the eigenvalue error, r^2 / gap (Kato-Temple
+      // with the second Ritz value for the gap), or the move of E0
+      const double bound = info.ritz_gap > 0.0 ? info.ritz_resid * info.ritz_resid / info.ritz_gap : 1e300;
+      if (info.act_dim >= 2 * chunk && bound <= r.tol * std::fabs(E)) break;
       if (have_prev && std::fabs(E - Eprev) <= r.tol * std::fabs(E)) break;
       if (info.act_dim >= r.k || info.breakdown) break;
       Eprev = E;
       have_prev = true;
     }
+    if ((rc = qd_krylov_gs_step(kr, op, (double*)psi, 0, &E, true))) return done(rc);  // the Ritz vector, once
   } else {
     if ((rc = qd_krylov_ground_state(kr, op, (double*)psi, -1, &E))) return done(rc);
     qd_krylov_get_info(kr, &info);
@@ -201,6 +206,7 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result
   out->energy_start = E;
   out->observable = dexp;
   out->iterations = info.act_dim;
+  out->krylov_mean = info.act_dim;
   return done(QD_OK);
 }
 
@@ -248,10 +254,22 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
   double h0[2] = {0.0, 0.0}, n0 = 0.0;
   if ((rc = qd_expect(ctx, op, (const double*)psi, h0))) return done(rc);
   if ((rc = qd_norm2(ctx, n, (const double*)psi, &n0))) return done(rc);
+  int64_t msum = 0;
+  int m_start = r.chunk > 0 ? r.chunk : 12;
   for (int s = 0; s < r.steps; ++s) {
-    if ((rc = qd_krylov_evolve(kr, op, (const double*)psi, (double*)nxt, r.dt, r.k))) return done(rc);
+    if (r.tol > 0.0) {
+      int mu = 0, me = 0;
+      rc = qd_krylov_evolve_tol(kr, op, (const double*)psi, (double*)nxt, r.dt, m_start, r.k, r.tol, &mu, &me);
+      msum += mu;
+      m_start = std::max(2, me);  // the next step starts where this one's estimate was met
+    } else {
+      rc = qd_krylov_evolve(kr, op, (const double*)psi, (double*)nxt, r.dt, r.k);
+      msum += r.k - 1;
+    }
+    if (rc) return done(rc);
     std::swap(psi, nxt);
   }
+  out->krylov_mean = r.steps > 0 ? (double)msum / r.steps : 0.0;
   double h1[2] = {0.0, 0.0}, n1 = 0.0, dexp = 0.0;
   if ((rc = qd_expect(ctx, op, (const double*)psi, h1))) return done(rc);
   if ((rc = diag_expect(ctx, n, M.d_diagD, psi, &dexp, &n1))) return done(rc);

@@ -208,6 +208,15 @@ class Krylov:
                                                      ctypes.c_void_p(psi_out.data_ptr()), float(h), kry_dim_rt),
                        "evolve")
 
+    def evolve_tol(self, op, psi_in, psi_out, h, m_min, kry_max, tol):
+        """Returns (dimension used, smallest dimension whose estimate met tol)."""
+        m, me = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.ctx.lib.qd_krylov_evolve_tol(self.ptr, op.ptr, ctypes.c_void_p(psi_in.data_ptr()),
+                                                         ctypes.c_void_p(psi_out.data_ptr()), float(h), m_min,
+                                                         kry_max, float(tol), ctypes.byref(m), ctypes.byref(me)),
+                       "evolve_tol")
+        return m.value, me.value
+
     def info(self):
         inf = abi.qd_krylov_info()
         self.ctx.check(self.ctx.lib.qd_krylov_get_info(self.ptr, ctypes.byref(inf)), "info")

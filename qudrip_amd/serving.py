@@ -62,7 +62,8 @@ class Server:
             r = buf[i]
             out.append({"tag": r.tag, "status": r.status, "iterations": r.iterations, "energy": r.energy,
                         "energy_start": r.energy_start, "observable": r.observable, "t_submit": r.t_submit,
-                        "t_start": r.t_start, "t_end": r.t_end, "worker": r.worker})
+                        "t_start": r.t_start, "t_end": r.t_end, "krylov_mean": r.krylov_mean,
+                        "worker": r.worker})
         return out
 
     def drain(self):
@@ -109,7 +110,7 @@ def hubbard_lattice(Lx=3, Ly=4, nup=6, ndn=6):
 
 
 def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=10, gs_tol=1e-12,
-               rt_k=30, rt_steps=100, dt=0.05, log=None):
+               rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, log=None):
     """Registers the models (timed apart: the structure cache is built once at
     start-up), then submits every request at its arrival time and waits."""
     nsite, bonds, nup, ndn = lattice or hubbard_lattice()
@@ -135,8 +136,8 @@ def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs
                 srv.submit(abi.QD_REQ_GROUND_STATE, models[("gs", r["N"])], r["coeff"], gs_k, chunk=gs_chunk,
                            tol=gs_tol, seed=r["seed"], tag=r["tag"])
             else:
-                srv.submit(abi.QD_REQ_EVOLVE, models["rt"], r["coeff"], rt_k, steps=rt_steps, dt=dt,
-                           seed=r["seed"], tag=r["tag"])
+                srv.submit(abi.QD_REQ_EVOLVE, models["rt"], r["coeff"], rt_k, chunk=rt_mmin, tol=rt_tol,
+                           steps=rt_steps, dt=dt, seed=r["seed"], tag=r["tag"])
         srv.drain()
         res = {x["tag"]: x for x in srv.poll(cap=max(16, len(reqs)))}
         stats = srv.stats()

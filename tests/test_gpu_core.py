@@ -320,6 +320,38 @@ def test_arnoldi_evolve_matches_oracle(ctx):
         _close(kr, mf, op, c, p, b)
 
 
+def test_evolve_tol_matches_dense_propagator(ctx):
+    """Adaptive Krylov dimension: the step agrees with the dense exponential
+    to the requested bound, and a loose bound picks a smaller basis."""
+    import scipy.linalg
+    ham, sec = hub(8, Q.hubbard_bonds_chain(8, False), 3, 3, U=6.0)[1:]
+    ref_basis = oracle.basis_scan(sec)
+    colptr, rowidx, vals = oracle.build_csc(ref_basis, ham)
+    n = len(ref_basis)
+    H = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        H[rowidx[colptr[j]:colptr[j + 1]], j] = vals.view(np.complex128)[colptr[j]:colptr[j + 1]]
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.from_csr(ctx, c)
+    kr = Krylov(ctx, b.size, 40, herm=False)
+    try:
+        psi0 = oracle.seeded_state(b.size, 9)
+        exact = scipy.linalg.expm(-1j * 0.05 * H) @ psi0
+        xin = to_dev(ctx, psi0)
+        out = ctx.zeros(b.size)
+        m_tight, me_tight = kr.evolve_tol(op, xin, out, 0.05, 8, 40, 1e-14)
+        assert np.max(np.abs(out.cpu().numpy() - exact)) < 1e-12
+        m_loose, me_loose = kr.evolve_tol(op, xin, out, 0.05, 8, 40, 1e-6)
+        assert np.max(np.abs(out.cpu().numpy() - exact)) < 1e-5
+        assert 8 <= m_loose < m_tight < 39
+        assert me_tight <= m_tight and me_loose <= m_loose and me_loose < me_tight
+        # starting at the suggested dimension reaches the bound again
+        m2, _ = kr.evolve_tol(op, xin, out, 0.05, me_tight, 40, 1e-14)
+        assert m2 <= m_tight and np.max(np.abs(out.cpu().numpy() - exact)) < 1e-12
+    finally:
+        _close(kr, op, c, p, b)
+
+
 @pytest.mark.parametrize("case", ["chain8_44", "sq3x4_33", "sq3x4_66", "chain6_nonherm", "phase_hops"])
 def test_matfree_structured_matches_oracle(ctx, case):
     """Two-block Kronecker form of the matrix-free product (Hubbard layout)."""
