@@ -295,7 +295,48 @@ __global__ void __launch_bounds__(KT)
   }
   for (int c0 = 0; c0 < ncols; c0 += 64) {
     const int cend = min(ncols, c0 + 64);
-    for (int c = c0; c < cend; ++c) {
+    int c = c0;
+    // two columns at a time: twice the loads in flight per wave and two
+    // independent reduction chains (each wave's columns run in sequence)
+    for (; c + 1 < cend; c += 2) {
+      const cplx* __restrict__ Vc = V + (int64_t)c * ld;
+      const cplx* __restrict__ Vd = Vc + ld;
+      cplx a0 = cmake(0.0, 0.0), b0 = cmake(0.0, 0.0), a1 = cmake(0.0, 0.0), b1 = cmake(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        int64_t r = base + (int64_t)q * KT + threadIdx.x;
+        if (r < n) {
+          const cplx v0 = Vc[r], v1 = Vd[r];
+          a0 = cadd(a0, cdotmul(v0, ur[q]));
+          a1 = cadd(a1, cdotmul(v1, ur[q]));
+          if (NR == 2) {
+            b0 = cadd(b0, cdotmul(v0, wr[q]));
+            b1 = cadd(b1, cdotmul(v1, wr[q]));
+          }
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        a0.x += __shfl_xor(a0.x, o, 64);
+        a0.y += __shfl_xor(a0.y, o, 64);
+        a1.x += __shfl_xor(a1.x, o, 64);
+        a1.y += __shfl_xor(a1.y, o, 64);
+        if (NR == 2) {
+          b0.x += __shfl_xor(b0.x, o, 64);
+          b0.y += __shfl_xor(b0.y, o, 64);
+          b1.x += __shfl_xor(b1.x, o, 64);
+          b1.y += __shfl_xor(b1.y, o, 64);
+        }
+      }
+      if (lane == 0) {
+        sh[wv][c - c0][0] = a0;
+        sh[wv][c + 1 - c0][0] = a1;
+        if (NR == 2) {
+          sh[wv][c - c0][NR - 1] = b0;
+          sh[wv][c + 1 - c0][NR - 1] = b1;
+        }
+      }
+    }
+    for (; c < cend; ++c) {
       const cplx* __restrict__ Vc = V + (int64_t)c * ld;
       cplx a = cmake(0.0, 0.0), bb = cmake(0.0, 0.0);
 #pragma unroll
