@@ -18,9 +18,9 @@
 // again once the trajectories have spread over smaller sectors.
 //
 // Each trajectory runs its own Arnoldi step (the reference's evolve with
-// H_eff, hubbard_diss.cpp:467-469; sparse_algorithm.hpp:548-562) with CGS2
-// fused into three passes per Krylov vector, a per-trajectory Hessenberg and a
-// per-trajectory exponential on the device.  Jumps are drawn on the device from
+// H_eff, hubbard_diss.cpp:467-469; sparse_algorithm.hpp:548-562) with delayed
+// CGS2 (two passes over the basis per Krylov vector), a per-trajectory
+// Hessenberg and a per-trajectory exponential on the device.  Jumps are drawn on the device from
 // a counter-based stream per trajectory, applied from the pre-step state into
 // a staging area, and committed after the evolution (moving lost trajectories
 // to their new sector).  All reductions have a fixed order for a given tile
@@ -204,22 +204,26 @@ __global__ void __launch_bounds__(256)
   *p = rmode ? cadd(*p, s) : s;
 }
 
-// per global slot: Gamma_t and 1/|psi_t| from the recorded densities and norm^2
+// per global slot: the diagonal shift of H_eff,t = H - i/2 (D - Gamma_t) as
+// alpha D + beta with alpha = -i/2, beta = i Gamma_t / 2, and 1/|psi_t| from the
+// recorded densities and norm^2
 __global__ void slot_params_kernel(int Mtot, int M, const int* __restrict__ slot2traj, const double* __restrict__ gtraj,
-                                   const double* __restrict__ dens_all, double* __restrict__ g,
-                                   double* __restrict__ scale) {
+                                   const double* __restrict__ dens_all, cplx* __restrict__ alpha,
+                                   cplx* __restrict__ beta, double* __restrict__ scale) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Mtot) return;
   const int tr = slot2traj[t];
-  g[t] = gtraj[tr];
+  alpha[t] = cmake(0.0, -0.5);
+  beta[t] = cmake(0.0, 0.5 * gtraj[tr]);
   const double n2 = dens_all[(int64_t)tr * (M + 1) + M];
   scale[t] = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
 }
 
 // Multi-RHS product with the normalisation of the source folded in, all sectors:
-//   Vout[s,t] = f_t src[s,t],   Wout[s,t] = (H_eff,t Vout)[s,t]
-//   H_eff,t = H - i/2 (D - Gamma_t),  H from the ColMajor arrays of a Hermitian
-//   operator (pull form: row s of H is column s conjugated).
+//   x = f_t src[s,t] (f = NULL: 1),  Vout[s,t] = x (Vout may be NULL),
+//   Wout[s,t] = (H x)[s] + (alpha_t D[s] + beta_t) x[s]
+// H from the ColMajor arrays of a Hermitian operator (pull form: row s of H is
+// column s conjugated); alpha, beta per slot.
 // src is the sector's X (from_x) or the W pool.  One wave per (SPMM_RB rows,
 // 64 slots): the wave loads the rows' pointers and up to 64 entries of a row
 // with one vector load each and broadcasts them with v_readlane, so all of a
@@ -239,8 +243,8 @@ __device__ inline double readlane_d(double v, int k) {
 }
 __global__ void __launch_bounds__(TT)
     g_spmm_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
-                  const double* __restrict__ f, const double* __restrict__ gamma, cplx* __restrict__ Vout,
-                  cplx* __restrict__ Wout) {
+                  const double* __restrict__ f, const cplx* __restrict__ alpha, const cplx* __restrict__ beta,
+                  cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
   const int64_t per_xcd = (int64_t)gridDim.x / 8;
   const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (b >= nblocks) return;
@@ -257,8 +261,9 @@ __global__ void __launch_bounds__(TT)
   const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
   const int64_t lds = from_x ? S.ldx : m;
   const int g = S.so + (active ? t : 0);
-  const double ft = active ? f[g] : 0.0;
-  const double gam = active ? gamma[g] : 0.0;
+  const double ft = active ? (f ? f[g] : 1.0) : 0.0;
+  const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
+  const cplx be = active ? beta[g] : cmake(0.0, 0.0);
   const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
   for (int r = 0; r < nr; ++r) {
     const int64_t s = s0 + r;
@@ -293,10 +298,188 @@ __global__ void __launch_bounds__(TT)
     if (active) {
       const cplx x0 = src[s * lds + t];
       const cplx x = cmake(x0.x * ft, x0.y * ft);
-      const double d = -0.5 * (S.D[s] - gam);  // (0, d) * x
-      Vout[S.vo + s * m + t] = x;
-      Wout[S.vo + s * m + t] = cmake(sx * ft - d * x.y, sy * ft + d * x.x);
+      const cplx dg = cmake(al.x * S.D[s] + be.x, al.y * S.D[s] + be.y);
+      const cplx dx = cmul(dg, x);
+      if (Vout) Vout[S.vo + s * m + t] = x;
+      Wout[S.vo + s * m + t] = cmake(sx * ft + dx.x, sy * ft + dx.y);
     }
+  }
+}
+
+// ---------------------------------------------------------- delayed CGS2
+// Batched form of the delayed classical Gram-Schmidt with one
+// reorthogonalisation (the single-vector form and its derivation are in
+// krylov.hip): step j holds, per slot, the pending vector u (A q_{j-1}
+// projected once) and w = A u.  One pass over Q_{j-1} gives c = Q^H u,
+// d = Q^H w, |u|^2 and u^H w (g_red_kernel); the small algebra of each slot
+// (g_dc_small_kernel) finishes column j-1 of its Hessenberg and the first pass
+// of column j; a second pass writes q_j and the next pending vector
+// (g_dc_update_kernel).  Two passes over the basis per vector instead of
+// CGS2's three fused ones.
+
+// part[P co + (chunk P + p) m + t], P = NR NQ + 2:
+//   p < NQ: conj(V_p) u;  NR == 2: NQ + p: conj(V_p) w;  P-2: |u|^2;  P-1: conj(u) w
+template <int NQ, int NR>
+__global__ void __launch_bounds__(TT)
+    g_red_kernel(const GSec* __restrict__ gs, int ns, const cplx* __restrict__ V, int64_t VS,
+                 const cplx* __restrict__ U, const cplx* __restrict__ W, cplx* __restrict__ part) {
+  constexpr int P = NR * NQ + 2;
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const cplx* __restrict__ Vb = V + S.vo;
+  const cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+  cplx ac[NQ], ad[NR == 2 ? NQ : 1];
+  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) ac[q] = cmake(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < (NR == 2 ? NQ : 1); ++q) ad[q] = cmake(0.0, 0.0);
+#pragma unroll 2
+  for (int64_t s = r0; s < r1; ++s) {
+    const int64_t o = s * m + t;
+    const cplx u = Ub[o];
+    const cplx w = (NR == 2) ? Wb[o] : cmake(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const cplx v = Vb[q * VS + o];
+      ac[q] = cadd(ac[q], cdotmul(v, u));
+      if (NR == 2) ad[q] = cadd(ad[q], cdotmul(v, w));
+    }
+    uu = cadd(uu, cdotmul(u, u));
+    if (NR == 2) uw = cadd(uw, cdotmul(u, w));
+  }
+  cplx* pb = part + P * S.co + (int64_t)tl.chunk * P * m + t;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) pb[(int64_t)q * m] = ac[q];
+  if (NR == 2) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pb[(int64_t)(NQ + q) * m] = ad[q];
+  }
+  pb[(int64_t)(P - 2) * m] = uu;
+  pb[(int64_t)(P - 1) * m] = uw;
+}
+
+// red[p Mtot + slot] = sum over chunks (fixed order) of the partials, P per slot.
+// One 256-thread block per (p, 64-slot group); wave w sums chunks w, w+4, ...
+__global__ void __launch_bounds__(256)
+    g_reduce_p_kernel(const GSec* __restrict__ gs, int ns, int64_t ngroups, int Mtot, int P,
+                      const cplx* __restrict__ part, cplx* __restrict__ red) {
+  __shared__ cplx sh[4][TT];
+  const int p = (int)(blockIdx.x / ngroups);
+  const int64_t gi = blockIdx.x % ngroups;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  cplx s = cmake(0.0, 0.0);
+  if (t < S.m) {
+    const cplx* pb = part + (int64_t)P * S.co;
+    for (int c = wv; c < S.nch; c += 4) s = cadd(s, pb[((int64_t)c * P + p) * S.m + t]);
+  }
+  sh[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  red[(int64_t)p * Mtot + S.so + t] = cadd(cadd(sh[0][lane], sh[1][lane]), cadd(sh[2][lane], sh[3][lane]));
+}
+
+// Small algebra of step j for every slot (one thread each; Hess[(col (K+1) + row) Mtot + slot]).
+// last: finalisation (red holds c and |u|^2 only).  coef[q Mtot + slot]: a (= c) at
+// q < j, b at K + q; sc[slot] = {1/alpha, f_j/alpha}.  Below eps the column ends
+// the Krylov space: every coefficient is zero, so the later vectors vanish.
+__global__ void g_dc_small_kernel(int Mtot, int j, int K, int last, const cplx* __restrict__ red,
+                                  cplx* __restrict__ hess, cplx* __restrict__ coef, cplx* __restrict__ sc, double eps) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= Mtot) return;
+  const int NR = last ? 1 : 2;
+  const int P = NR * j + 2;
+  auto R = [&](int p) { return red[(int64_t)p * Mtot + g]; };
+  auto H = [&](int row, int col) -> cplx& { return hess[((int64_t)col * (K + 1) + row) * Mtot + g]; };
+  double cc = 0.0;
+  cplx chd = cmake(0.0, 0.0);
+  for (int i = 0; i < j; ++i) {
+    const cplx c = R(i);
+    cc += c.x * c.x + c.y * c.y;
+    if (!last) chd = cadd(chd, cdotmul(c, R(j + i)));
+  }
+  const double a2 = R(P - 2).x - cc;
+  const double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
+  // finish column j-1
+  for (int i = 0; i < j; ++i) H(i, j - 1) = cadd(H(i, j - 1), R(i));
+  if (j <= K) H(j, j - 1) = cmake(alpha, 0.0);
+  if (last) return;
+  if (!(alpha >= eps)) {
+    for (int i = 0; i < j; ++i) {
+      coef[(int64_t)i * Mtot + g] = cmake(0.0, 0.0);
+      coef[(int64_t)(K + i) * Mtot + g] = cmake(0.0, 0.0);
+    }
+    for (int i = 0; i <= j; ++i) H(i, j) = cmake(0.0, 0.0);
+    sc[2 * g] = cmake(0.0, 0.0);
+    sc[2 * g + 1] = cmake(0.0, 0.0);
+    return;
+  }
+  const double inv = 1.0 / alpha;
+  const cplx fj = crmul(inv * inv, csub(R(P - 1), chd));
+  // e = H[0:j+1, 0:j] c;  column j first pass = f - e / alpha
+  for (int i = 0; i <= j; ++i) {
+    cplx e = cmake(0.0, 0.0);
+    for (int k = (i > 0 ? i - 1 : 0); k < j; ++k) e = cadd(e, cmul(H(i, k), R(k)));
+    const cplx f = (i < j) ? crmul(inv, R(j + i)) : fj;
+    H(i, j) = csub(f, crmul(inv, e));
+    if (i < j) {
+      coef[(int64_t)i * Mtot + g] = R(i);
+      coef[(int64_t)(K + i) * Mtot + g] = csub(f, crmul(inv, cmul(R(i), fj)));
+    }
+  }
+  sc[2 * g] = cmake(inv, 0.0);
+  sc[2 * g + 1] = crmul(inv, fj);
+}
+
+// q_j = (u - sum_q V_q a_q) / alpha -> V_j;  u <- w / alpha - sum_q V_q b_q - u f_j / alpha
+template <int NQ>
+__global__ void __launch_bounds__(TT)
+    g_dc_update_kernel(const GSec* __restrict__ gs, int ns, cplx* __restrict__ V, int64_t VS, int j,
+                       cplx* __restrict__ U, const cplx* __restrict__ W, const cplx* __restrict__ coef,
+                       const cplx* __restrict__ sc, int Mtot, int K) {
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int g = S.so + t;
+  cplx a[NQ], b[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    a[q] = coef[(int64_t)q * Mtot + g];
+    b[q] = coef[(int64_t)(K + q) * Mtot + g];
+  }
+  const double inv = sc[2 * g].x;
+  const cplx f = sc[2 * g + 1];
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  cplx* __restrict__ Vb = V + S.vo;
+  cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+#pragma unroll 2
+  for (int64_t s = r0; s < r1; ++s) {
+    const int64_t o = s * m + t;
+    cplx s1 = cmake(0.0, 0.0), s2 = cmake(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const cplx v = Vb[q * VS + o];
+      s1 = cadd(s1, cmul(v, a[q]));
+      s2 = cadd(s2, cmul(v, b[q]));
+    }
+    const cplx u = Ub[o];
+    const cplx w = Wb[o];
+    Vb[(int64_t)j * VS + o] = crmul(inv, csub(u, s1));
+    const cplx uf = cmul(u, f);
+    Ub[o] = cmake(w.x * inv - s2.x - uf.x, w.y * inv - s2.y - uf.y);
   }
 }
 
@@ -525,21 +708,40 @@ __global__ void broadcast_kernel(int64_t n, int m, int ld, const cplx* __restric
   if (s < n) X[s * ld + t] = psi0[s];
 }
 
-template <int MODE>
-void launch_pass(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, const cplx* V, int64_t VS, cplx* W,
-                 const cplx* h, int Mtot, cplx* part) {
+template <int NR>
+void launch_red(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, const cplx* V, int64_t VS,
+                const cplx* U, const cplx* W, cplx* part) {
   switch (nq) {
-#define QD_PASS_CASE(N)                                                            \
-  case N:                                                                          \
-    g_pass_kernel<N, MODE><<<tiles, TT, 0, st>>>(gs, ns, V, VS, W, h, Mtot, part); \
+#define QD_RED_CASE(N)                                                            \
+  case N:                                                                         \
+    g_red_kernel<N, NR><<<tiles, TT, 0, st>>>(gs, ns, V, VS, U, W, part);         \
     break;
-    QD_PASS_CASE(1) QD_PASS_CASE(2) QD_PASS_CASE(3) QD_PASS_CASE(4) QD_PASS_CASE(5) QD_PASS_CASE(6)
-    QD_PASS_CASE(7) QD_PASS_CASE(8) QD_PASS_CASE(9) QD_PASS_CASE(10) QD_PASS_CASE(11) QD_PASS_CASE(12)
-    QD_PASS_CASE(13) QD_PASS_CASE(14) QD_PASS_CASE(15) QD_PASS_CASE(16) QD_PASS_CASE(17) QD_PASS_CASE(18)
-    QD_PASS_CASE(19) QD_PASS_CASE(20) QD_PASS_CASE(21) QD_PASS_CASE(22) QD_PASS_CASE(23) QD_PASS_CASE(24)
-    QD_PASS_CASE(25) QD_PASS_CASE(26) QD_PASS_CASE(27) QD_PASS_CASE(28) QD_PASS_CASE(29) QD_PASS_CASE(30)
-    QD_PASS_CASE(31) QD_PASS_CASE(32)
-#undef QD_PASS_CASE
+    QD_RED_CASE(1) QD_RED_CASE(2) QD_RED_CASE(3) QD_RED_CASE(4) QD_RED_CASE(5) QD_RED_CASE(6)
+    QD_RED_CASE(7) QD_RED_CASE(8) QD_RED_CASE(9) QD_RED_CASE(10) QD_RED_CASE(11) QD_RED_CASE(12)
+    QD_RED_CASE(13) QD_RED_CASE(14) QD_RED_CASE(15) QD_RED_CASE(16) QD_RED_CASE(17) QD_RED_CASE(18)
+    QD_RED_CASE(19) QD_RED_CASE(20) QD_RED_CASE(21) QD_RED_CASE(22) QD_RED_CASE(23) QD_RED_CASE(24)
+    QD_RED_CASE(25) QD_RED_CASE(26) QD_RED_CASE(27) QD_RED_CASE(28) QD_RED_CASE(29) QD_RED_CASE(30)
+    QD_RED_CASE(31) QD_RED_CASE(32)
+#undef QD_RED_CASE
+    default:
+      break;
+  }
+}
+
+void launch_dc_update(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, cplx* V, int64_t VS, int j,
+                      cplx* U, const cplx* W, const cplx* coef, const cplx* sc, int Mtot, int K) {
+  switch (nq) {
+#define QD_UPD_CASE(N)                                                                         \
+  case N:                                                                                      \
+    g_dc_update_kernel<N><<<tiles, TT, 0, st>>>(gs, ns, V, VS, j, U, W, coef, sc, Mtot, K);    \
+    break;
+    QD_UPD_CASE(1) QD_UPD_CASE(2) QD_UPD_CASE(3) QD_UPD_CASE(4) QD_UPD_CASE(5) QD_UPD_CASE(6)
+    QD_UPD_CASE(7) QD_UPD_CASE(8) QD_UPD_CASE(9) QD_UPD_CASE(10) QD_UPD_CASE(11) QD_UPD_CASE(12)
+    QD_UPD_CASE(13) QD_UPD_CASE(14) QD_UPD_CASE(15) QD_UPD_CASE(16) QD_UPD_CASE(17) QD_UPD_CASE(18)
+    QD_UPD_CASE(19) QD_UPD_CASE(20) QD_UPD_CASE(21) QD_UPD_CASE(22) QD_UPD_CASE(23) QD_UPD_CASE(24)
+    QD_UPD_CASE(25) QD_UPD_CASE(26) QD_UPD_CASE(27) QD_UPD_CASE(28) QD_UPD_CASE(29) QD_UPD_CASE(30)
+    QD_UPD_CASE(31)
+#undef QD_UPD_CASE
     default:
       break;
   }
@@ -610,12 +812,13 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   const int NSMAX = (int)sec.size();
   // chunk counts are at most min(QD_TRAJ_MAX_CHUNKS, ceil(n / 16)) per sector
   const int max_nch = (int)std::min<int64_t>(QD_TRAJ_MAX_CHUNKS, (nmax + 15) / 16);
-  const int PMAX = std::max(K, M + 1);
+  const int PMAX = std::max(2 * K + 2, M + 1);
 
   // device buffers
   int *d_jmode = nullptr, *d_action = nullptr, *d_slot2traj = nullptr;
   double *d_jrate = nullptr, *d_dens_all = nullptr, *d_gtraj = nullptr, *d_dsum = nullptr, *d_dsq = nullptr;
-  double *d_gslot = nullptr, *d_scale = nullptr;
+  double *d_scale = nullptr;
+  cplx *d_alpha = nullptr, *d_beta = nullptr, *d_red = nullptr, *d_coef2 = nullptr, *d_sc = nullptr;
   cplx *d_V = nullptr, *d_W = nullptr, *d_W2 = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr;
   cplx *d_coef = nullptr, *d_staging = nullptr;
   GSec* d_gsec = nullptr;
@@ -664,8 +867,11 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&d_action, sizeof(int) * ntraj) || !dalloc((void**)&d_slot2traj, sizeof(int) * ntraj) ||
       !dalloc((void**)&d_dens_all, sizeof(double) * ntraj * (M + 1)) ||
       !dalloc((void**)&d_gtraj, sizeof(double) * ntraj) || !dalloc((void**)&d_dsum, sizeof(double) * M) ||
-      !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_gslot, sizeof(double) * ntraj) ||
-      !dalloc((void**)&d_scale, sizeof(double) * ntraj) ||
+      !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_scale, sizeof(double) * ntraj) ||
+      !dalloc((void**)&d_alpha, sizeof(cplx) * ntraj) || !dalloc((void**)&d_beta, sizeof(cplx) * ntraj) ||
+      !dalloc((void**)&d_red, sizeof(cplx) * (size_t)(2 * K + 2) * ntraj) ||
+      !dalloc((void**)&d_coef2, sizeof(cplx) * (size_t)(2 * K) * ntraj) ||
+      !dalloc((void**)&d_sc, sizeof(cplx) * 2 * (size_t)ntraj) ||
       !dalloc((void**)&d_V, sizeof(cplx) * (size_t)K * nmax * ntraj) ||
       !dalloc((void**)&d_W, sizeof(cplx) * (size_t)nmax * ntraj) ||
       !dalloc((void**)&d_W2, sizeof(cplx) * (size_t)nmax * ntraj) ||
@@ -772,31 +978,33 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     return QD_OK;
   };
 
-  // one Arnoldi step (CGS2, K vectors) for every slot of every occupied sector, X updated in place
+  // one Arnoldi step (delayed CGS2, K vectors) for every slot of every occupied sector, X updated in place
   auto evolve_all = [&]() -> int {
     const int Mtot = ntraj;
-    slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_gslot, d_scale);
-    cplx* Wa = d_W;
-    cplx* Wb = d_W2;
-    // V_0 = X / |X|, Wa = H_eff V_0
+    slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
+                                                       d_scale);
     const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
-    g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 1, nullptr, d_scale, d_gslot, d_V, Wa);
-    for (int j = 0; j < K; ++j) {
-      const int nq = j + 1;
-      const unsigned gr = (unsigned)(ngroups * nq);
-      launch_pass<0>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, nullptr, Mtot, d_part);
-      g_reduce_kernel<<<gr, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, nq, d_part, d_h, d_hess, j, K, 0, nullptr);
-      launch_pass<1>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, d_h, Mtot, d_part);
-      g_reduce_kernel<<<gr, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, nq, d_part, d_h, d_hess, j, K, 1, nullptr);
-      if (j + 1 < K) {
-        launch_pass<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, Wa, d_h, Mtot, d_part);
-        g_reduce_kernel<<<(unsigned)ngroups, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, 1, d_part, nullptr, d_hess, j, K,
-                                                            2, d_scale);
-        // V_{j+1} = Wa / beta, Wb = H_eff V_{j+1}
-        g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 0, Wa, d_scale, d_gslot, d_V + (j + 1) * VS, Wb);
-        std::swap(Wa, Wb);
-      }
+    const unsigned tl = (unsigned)tiles;
+    cplx* U = d_W2;
+    cplx* W = d_W;
+    // V_0 = X / |X|, U = H_eff V_0, then its first projection: H[0][0] = V_0^H U, U -= V_0 H[0][0]
+    g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 1, nullptr, d_scale, d_alpha, d_beta, d_V, U);
+    g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, d_V, VS, U, nullptr, Mtot, d_part);
+    g_reduce_kernel<<<(unsigned)ngroups, 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, 1, d_part, d_h, d_hess, 0, K, 0,
+                                                        nullptr);
+    g_pass_kernel<1, 2><<<tl, TT, 0, st>>>(d_gsec, ns, d_V, VS, U, d_h, Mtot, d_part);
+    for (int j = 1; j < K; ++j) {
+      const int P = 2 * j + 2;
+      g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 0, U, nullptr, d_alpha, d_beta, nullptr, W);
+      launch_red<2>(j, tl, st, d_gsec, ns, d_V, VS, U, W, d_part);
+      g_reduce_p_kernel<<<(unsigned)(ngroups * P), 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, P, d_part, d_red);
+      g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, j, K, 0, d_red, d_hess, d_coef2, d_sc, 1e-10);
+      launch_dc_update(j, tl, st, d_gsec, ns, d_V, VS, j, U, W, d_coef2, d_sc, Mtot, K);
     }
+    // the pending vector's second projection completes column K-1
+    launch_red<1>(K, tl, st, d_gsec, ns, d_V, VS, U, nullptr, d_part);
+    g_reduce_p_kernel<<<(unsigned)(ngroups * (K + 2)), 256, 0, st>>>(d_gsec, ns, ngroups, Mtot, K + 2, d_part, d_red);
+    g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, K, K, 1, d_red, d_hess, d_coef2, d_sc, 1e-10);
     small_expm_kernel<<<(unsigned)Mtot, 64, 0, st>>>(Mtot, K, K, d_hess, dt, d_coef);
     g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, d_V, VS, K, d_coef, Mtot);
     QD_CHECK_LAUNCH(ctx);
@@ -805,7 +1013,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
 
   // Tile autotuner for the uneven sectors: the occupancy target of the
   // reduction tiles (rows per chunk of every sector) is timed on the widest
-  // Gram-Schmidt pass over the current sector membership (a read-only pass,
+  // reduction pass over the current sector membership (a read-only pass,
   // results discarded).  It runs at the start and again once the loss jumps
   // have spread the trajectories over smaller sectors.
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -818,9 +1026,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     for (int c : cand) {
       occupancy_target = c;
       build_groups();
-      launch_pass<0>(K, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, nullptr, ntraj, d_part);
+      const int nq = std::max(1, K - 1);
+      launch_red<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, d_W2, d_part);
       hipEventRecord(ev0, st);
-      for (int r = 0; r < 2; ++r) launch_pass<0>(K, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, nullptr, ntraj, d_part);
+      for (int r = 0; r < 2; ++r) launch_red<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, d_W2, d_part);
       hipEventRecord(ev1, st);
       hipEventSynchronize(ev1);
       float ms = 0.f;
