@@ -205,6 +205,15 @@ def trajectories(ctx, args):
            "closed_form_z": round(abs(N.mean() - expect) / sigma, 3),
            "density_site0_up_T": round(float(r["mean"][-1][0]), 6),
            "density_site0_up_T_sigma": round(float(np.sqrt(max(r["var"][-1][0], 0.0) / args.traj)), 6)}
+    # plain: one trajectory at a time through the single-vector evolve (same
+    # random streams, so the jump sequences must agree), on a sample
+    ns_plain = 8
+    _, plog, psecs = WL.plain_trajectories(ctx, r["ham"], 8, 4, 4, r["psi0"], ch, ns_plain, nsteps, 0.05, 13, 5)
+    out["plain_gpu"] = {"traj_steps_per_s": round(ns_plain * nsteps / psecs, 1), "sample_trajectories": ns_plain,
+                        "logs_equal_batched": bool((plog == r["log"][:ns_plain]).all())}
+    out["batched_vs_plain"] = round(out["traj_steps_per_s"] / out["plain_gpu"]["traj_steps_per_s"], 1)
+    log(f"trajectories plain: {out['plain_gpu']['traj_steps_per_s']} traj-steps/s; batched "
+        f"{out['batched_vs_plain']}x")
     if args.traj_cpu > 0 and not args.no_cpu:
         import oracle
         psi0 = r["psi0"].cpu().numpy()

@@ -148,6 +148,14 @@ class LinOp:
         ctx.check(ctx.lib.qd_linop_matfree(ctx.ptr, basis.ptr, prog.ptr, ctypes.byref(p)), "linop_matfree")
         return cls(ctx, p, (basis, prog))
 
+    @classmethod
+    def with_diagonal(cls, ctx, base, diag):
+        """(base + diag(d)) as a view (qd_linop_with_diagonal); diag is a device complex128 vector."""
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_linop_with_diagonal(ctx.ptr, base.ptr, ctypes.c_void_p(diag.data_ptr()),
+                                                 ctypes.byref(p)), "linop_with_diagonal")
+        return cls(ctx, p, (base, diag))
+
     def set_lanes(self, lanes):
         self.ctx.check(self.ctx.lib.qd_linop_set_lanes(self.ptr, lanes), "set_lanes")
 

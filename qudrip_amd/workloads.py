@@ -118,3 +118,125 @@ def hubbard_trajectories(ctx, L=8, nup=4, ndn=4, U=4.0, t=1.0, dephase=0.1, loss
         log(f"trajectories L={L} n={basis.size}: {ntraj} x {nsteps} steps in {run_s:.3f} s, "
             f"{res['jumps']} jumps ({res['losses']} losses), {res['sectors']} sectors")
     return res
+
+
+_M64 = (1 << 64) - 1
+
+
+def _sm64(z):
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def _traj_uniform(seed, traj, step, k):
+    """The trajectory stream of qd_traj_run: key = sm(seed * golden + traj), r = sm(key + 4 step + k) >> 11."""
+    key = _sm64((seed * 0x9E3779B97F4A7C15 + traj) & _M64)
+    return (_sm64((key + 4 * step + k) & _M64) >> 11) * (1.0 / 9007199254740992.0)
+
+
+def plain_trajectories(ctx, ham, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, seed, traj_offset=0):
+    """One trajectory at a time on the single-vector GPU path, the loop of
+    programs/hubbard_diss.cpp:440-482 with the channels of qd_traj_run: per
+    step the densities of the state, the jump test on the same random stream,
+    then either the jump or evolve(H_eff, psi, dt, k_rt) with
+    H_eff = H - i/2 (D - Gamma) as the stored H plus a diagonal, and a
+    renormalisation.  Returns (sum of densities [(nsteps+1), 2 nsite],
+    jump log [ntraj, nsteps], seconds)."""
+    import numpy as np
+    import torch
+    dev = f"cuda:{ctx.device}"
+    M = 2 * nsite
+    idx = Q.getQbits(M)
+    sectors = {}
+
+    def sector(a, b):
+        if (a, b) in sectors:
+            return sectors[(a, b)]
+        basis = Basis(ctx, Q.hubbard_sector(idx, nsite, a, b))
+        prog = TermProgram(ctx, ham, M)
+        csr = CSR(ctx, basis, prog)
+        op = LinOp.from_csr(ctx, csr)
+        st = basis.states()
+        occ = ((st[:, None] >> np.arange(M, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(np.float64)
+        D = np.zeros(len(st))
+        for (kind, m, g) in channels:
+            D += g * occ[:, m]
+        S = {"basis": basis, "prog": prog, "csr": csr, "op": op, "states": st, "n": len(st),
+             "occ": torch.from_numpy(occ).to(dev), "D": torch.from_numpy(D).to(dev),
+             "kr": Krylov(ctx, max(1, len(st)), k_rt, herm=False), "loss": {}}
+        sectors[(a, b)] = S
+        return S
+
+    def loss_map(a, b, m):
+        S = sector(a, b)
+        if m in S["loss"]:
+            return S["loss"][m]
+        na, nb = (a - 1, b) if m < nsite else (a, b - 1)
+        T = sector(na, nb)
+        st = S["states"]
+        src = np.nonzero((st >> np.uint64(m)) & np.uint64(1))[0]
+        y = st[src] & ~np.uint64(1 << m)
+        dst = np.searchsorted(T["states"], y)
+        par = np.array([bin(int(x) >> (m + 1)).count("1") & 1 for x in st[src]])
+        sign = torch.from_numpy(np.where(par == 1, -1.0, 1.0)).to(dev).to(torch.complex128)
+        S["loss"][m] = (na, nb, torch.from_numpy(src).to(dev), torch.from_numpy(dst).to(dev), sign)
+        return S["loss"][m]
+
+    rates = np.array([c[2] for c in channels])
+    modes = np.array([c[1] for c in channels])
+    dsum = np.zeros((nsteps + 1, M))
+    log = np.full((ntraj, max(nsteps, 1)), -1, dtype=np.int64)
+    psi0 = torch.as_tensor(psi0).to(dev).to(torch.complex128)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for tr in range(ntraj):
+        a, b = nup0, ndn0
+        S = sector(a, b)
+        psi = psi0 / torch.linalg.vector_norm(psi0)
+        for step in range(nsteps + 1):
+            p = psi.real ** 2 + psi.imag ** 2
+            dens = ((p @ S["occ"]) / p.sum()).cpu().numpy()
+            dsum[step] += dens
+            if step == nsteps:
+                break
+            w = rates * dens[modes]
+            G = float(w.sum())
+            r0 = _traj_uniform(seed, traj_offset + tr, step, 0)
+            r1 = _traj_uniform(seed, traj_offset + tr, step, 1)
+            ch = -1
+            if r0 < G * dt:
+                cum = 0.0
+                for c in range(len(channels)):
+                    cum += w[c]
+                    if w[c] > 0.0:
+                        ch = c
+                    if r1 * G < cum and w[c] > 0.0:
+                        break
+            log[tr, step] = ch
+            if ch >= 0:
+                kind, m, _ = channels[ch]
+                if kind == 0:
+                    psi = psi * S["occ"][:, m]
+                else:
+                    na, nb, src, dst, sign = loss_map(a, b, m)
+                    T = sector(na, nb)
+                    out = torch.zeros(T["n"], dtype=torch.complex128, device=dev)
+                    out[dst] = psi[src] * sign
+                    psi, S, a, b = out, T, na, nb
+                psi = psi / torch.linalg.vector_norm(psi)
+            else:
+                diag = (-0.5j) * (S["D"] - G).to(torch.complex128)
+                heff = LinOp.with_diagonal(ctx, S["op"], diag)
+                nxt = torch.empty_like(psi)
+                S["kr"].evolve(heff, psi, nxt, dt, k_rt)
+                heff.close()
+                psi = nxt / torch.linalg.vector_norm(nxt)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    for S in sectors.values():
+        S["kr"].close()
+        for h in (S["op"], S["csr"], S["prog"], S["basis"]):
+            h.close()
+    return dsum, log, secs

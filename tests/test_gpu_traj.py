@@ -69,3 +69,18 @@ def test_traj_full_size_closed_form_and_oracle_prefix(ctx):
     assert abs(N.mean() - expect) < 5 * N.std() / np.sqrt(ntraj)
     _, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, 4, nsteps, dt, k, 5)
     assert (g["log"][:4] == log).all()
+
+
+def test_traj_batched_matches_plain_one_at_a_time(ctx):
+    """The batched engine and the plain loop (one trajectory at a time through
+    the single-vector evolve) draw the same jumps and give the same densities."""
+    from qudrip_amd.workloads import plain_trajectories
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 17)
+    ch = jump_channels(L, 0.5, 0.3)
+    ntraj, nsteps, dt, k = 12, 40, 0.05, 10
+    g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 8, want_log=True)
+    dsum, log, _ = plain_trajectories(ctx, ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 8)
+    assert (g["log"] == log).all()
+    np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
