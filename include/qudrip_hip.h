@@ -343,6 +343,11 @@ int qd_server_add_model(qd_server* s, int nbits, const qd_constraint* cons, int 
                         const qd_op* oA, int noA, const qd_term* tD, int ntD, const qd_op* oD, int noD,
                         int structured, int* model_id);
 int qd_server_submit(qd_server* s, const qd_request* req);
+/* Evolve requests of one model with equal kry_dim_rt and dt run in lockstep
+ * batches of up to `slots` (one multi-RHS Arnoldi step per time step over all
+ * of them, on the model's stored matrix; arrivals join at the next step).
+ * 1 (default): each request alone on its worker. */
+int qd_server_set_batch(qd_server* s, int slots);
 /* Moves up to cap finished results (in completion order) into out. */
 int qd_server_poll(qd_server* s, qd_result* out, int cap, int* count);
 /* Blocks until every submitted request has finished. */

@@ -1,0 +1,681 @@
+// mrhs.hpp -- grouped multi-RHS Krylov machinery shared by the batched
+// trajectory engine (traj.hip) and the solve server (serve.hip).
+//
+// A batch is a set of "sectors" (blocks of slots that share one operator and
+// one basis); a slot is one right-hand side (a trajectory, a request).  Slots
+// are stored trajectory-minor, X[s * ld + t], and every kernel runs as one
+// grouped launch over all sectors: a GSec table gives each sector's rows,
+// slots, pool offsets and tile ranges, and each workgroup finds its sector by
+// binary search.  One Arnoldi step (delayed CGS2, per-slot Hessenberg and
+// exponential) advances every slot at once.
+#pragma once
+#include <algorithm>
+#include <cmath>
+
+#include "qd_internal.hpp"
+
+namespace {
+
+constexpr int TT = 64;    // trajectories per wave
+constexpr int MAXM = 32;  // modes (2 * nsite) supported
+constexpr int MAXK = 32;  // Krylov dimension limit for the batched step
+
+// One occupied sector in a grouped launch.
+struct GSec {
+  int64_t n;      // rows (sector dimension)
+  int m;          // active slots
+  int ldx;        // leading dimension of X
+  int rpc, nch;   // rows per reduction chunk, chunks
+  int groups;     // ceil(m / 64)
+  int so;         // first global slot
+  int64_t vo;     // offset of the sector's n x m block in the V / W pools
+  int64_t co;     // sum of nch * m over the earlier sectors (partials unit)
+  int64_t tile0;  // first reduction tile (tiles = nch * groups)
+  int64_t blk0;   // first product block (blocks = ceil(n / SPMM_RB) * groups)
+  int64_t grp0;   // first 64-slot group (sum of groups over the earlier sectors)
+  cplx* X;
+  const int32_t* outer;
+  const int32_t* inner;
+  const cplx* vals;
+  const double* D;
+  const uint64_t* states;
+};
+
+// index of the last sector whose key <= v (keys ascending, first key 0)
+template <typename F>
+__device__ inline int find_sec(int ns, int64_t v, F key) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+
+// Reduction tile -> (sector, chunk, slot of this lane); one wave per tile.
+struct Tile {
+  int sec, chunk, t;
+};
+__device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
+  Tile r;
+  const int64_t b = blockIdx.x;
+  r.sec = find_sec(ns, b, [&](int i) { return gs[i].tile0; });
+  const GSec& S = gs[r.sec];
+  const int64_t local = b - S.tile0;
+  r.chunk = (int)(local / S.groups);
+  r.t = (int)(local % S.groups) * TT + threadIdx.x;
+  return r;
+}
+
+// One pass over a row chunk of NQ basis vectors and W for every sector
+// (trajectory-minor blocks in the V / W pools, lds = m).
+//   MODE 0: part_q = sum_s conj(V_q[s,t]) W[s,t]                  (first Gram-Schmidt pass)
+//   MODE 1: W <- W - sum_q V_q h_q, then part_q as in MODE 0      (update + second pass)
+//   MODE 2: W <- W - sum_q V_q h_q, then part_0 = sum_s |W[s,t]|^2 (update + norm)
+// part[P co + (chunk P + q) m + t], P = NQ (modes 0, 1) or 1 (mode 2); h[q Mtot + slot].
+// NQ is a template parameter so the per-lane accumulators stay in registers.
+template <int NQ, int MODE>
+__global__ void __launch_bounds__(TT)
+    g_pass_kernel(const GSec* __restrict__ gs, int ns, const cplx* __restrict__ V, int64_t VS, cplx* __restrict__ W,
+                  const cplx* __restrict__ h, int Mtot, cplx* __restrict__ part) {
+  constexpr int NP = MODE == 2 ? 1 : NQ;
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const cplx* __restrict__ Vb = V + S.vo;
+  cplx* __restrict__ Wb = W + S.vo;
+  cplx hc[NQ];
+  if (MODE != 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) hc[q] = h[(int64_t)q * Mtot + S.so + t];
+  }
+  cplx acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = cmake(0.0, 0.0);
+#pragma unroll 2
+  for (int64_t s = r0; s < r1; ++s) {
+    const int64_t o = s * m + t;
+    cplx v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = Vb[q * VS + o];
+    cplx w = Wb[o];
+    if (MODE != 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        w.x -= v[q].x * hc[q].x - v[q].y * hc[q].y;
+        w.y -= v[q].x * hc[q].y + v[q].y * hc[q].x;
+      }
+      Wb[o] = w;
+    }
+    if (MODE == 2) {
+      acc[0].x += w.x * w.x + w.y * w.y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = cadd(acc[q], cdotmul(v[q], w));
+    }
+  }
+  cplx* pb = part + NP * S.co;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) pb[((int64_t)tl.chunk * NP + q) * m + t] = acc[q];
+}
+
+// Chunk partials are summed by RW waves per (value, 64-slot group): wave w
+// adds chunks w, w + RW, ... and the wave sums are added in wave order.
+constexpr int RW = 16;
+
+// Sums the chunk partials of every slot (fixed order) and files them:
+//   RMODE 0: h = sum, Hess[0..nq-1][j] = sum;  RMODE 1: h = sum, Hess[..][j] += sum;
+//   RMODE 2: beta = sqrt(sum), Hess[j+1][j] = beta, scale = 1/beta (0 below 1e-10).
+// One block per (q, 64-slot group).  Hess[(j (K+1) + i) Mtot + slot].
+__global__ void __launch_bounds__(RW * TT)
+    g_reduce_kernel(const GSec* __restrict__ gs, int ns, int64_t ngroups, int Mtot, int nq,
+                    const cplx* __restrict__ part, cplx* __restrict__ h, cplx* __restrict__ hess, int j, int K,
+                    int rmode, double* __restrict__ scale) {
+  __shared__ cplx sh[RW][TT];
+  const int q = (int)(blockIdx.x / ngroups);
+  const int64_t gi = blockIdx.x % ngroups;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  cplx s = cmake(0.0, 0.0);
+  if (t < S.m) {
+    const cplx* pb = part + (int64_t)nq * S.co;
+#pragma unroll 4
+    for (int c = wv; c < S.nch; c += RW) s = cadd(s, pb[((int64_t)c * nq + q) * S.m + t]);
+  }
+  sh[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  s = sh[0][lane];
+  for (int w = 1; w < RW; ++w) s = cadd(s, sh[w][lane]);
+  const int g = S.so + t;
+  if (rmode == 2) {
+    const double b = sqrt(s.x);
+    hess[((int64_t)(j * (K + 1) + j + 1)) * Mtot + g] = cmake(b, 0.0);
+    scale[g] = (b >= 1e-10) ? 1.0 / b : 0.0;
+    return;
+  }
+  h[(int64_t)q * Mtot + g] = s;
+  cplx* p = hess + ((int64_t)(j * (K + 1) + q)) * Mtot + g;
+  *p = rmode ? cadd(*p, s) : s;
+}
+
+// Multi-RHS product with the normalisation of the source folded in, all sectors:
+//   x = f_t src[s,t] (f = NULL: 1),  Vout[s,t] = x (Vout may be NULL),
+//   Wout[s,t] = (H x)[s] + (alpha_t D[s] + beta_t) x[s]
+// H from the ColMajor arrays of a Hermitian operator (pull form: row s of H is
+// column s conjugated); alpha, beta per slot.
+// src is the sector's X (from_x) or the W pool.  One wave per (SPMM_RB rows,
+// 64 slots): the wave loads the rows' pointers and up to 64 entries of a row
+// with one vector load each and broadcasts them with v_readlane, so all of a
+// row's gathers (64 consecutive trajectories each) are in flight together.
+//
+// Block order: workgroups are dealt to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8; it is given work item (b % 8) * (nb / 8) + b / 8, which
+// hands every XCD a contiguous range of (sector, group, rows) items.  Within a
+// sector the items run group-major (all rows of one 64-slot group, then the
+// next), so the rows an XCD gathers from belong to the one or two groups it is
+// working on and stay in its L2.
+constexpr int SPMM_RB = 4;
+__device__ inline double readlane_d(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+__global__ void __launch_bounds__(TT)
+    g_spmm_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
+                  const double* __restrict__ f, const cplx* __restrict__ alpha, const cplx* __restrict__ beta,
+                  cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
+  const int64_t per_xcd = (int64_t)gridDim.x / 8;
+  const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (b >= nblocks) return;
+  const int si = find_sec(ns, b, [&](int i) { return gs[i].blk0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x;
+  const int64_t nrb = (S.n + SPMM_RB - 1) / SPMM_RB;
+  const int64_t local = b - S.blk0;
+  const int64_t s0 = (local % nrb) * SPMM_RB;
+  const int nr = (int)min<int64_t>(SPMM_RB, S.n - s0);
+  const int t = (int)(local / nrb) * TT + lane;
+  const int m = S.m;
+  const bool active = t < m;
+  const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
+  const int64_t lds = from_x ? S.ldx : m;
+  const int g = S.so + (active ? t : 0);
+  const double ft = active ? (f ? f[g] : 1.0) : 0.0;
+  const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
+  const cplx be = active ? beta[g] : cmake(0.0, 0.0);
+  const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
+  for (int r = 0; r < nr; ++r) {
+    const int64_t s = s0 + r;
+    const int e0 = __builtin_amdgcn_readlane(optr, r);
+    const int e1 = __builtin_amdgcn_readlane(optr, r + 1);
+    double sx = 0.0, sy = 0.0;
+    for (int base = e0; base < e1; base += TT) {
+      const int cnt = min(TT, e1 - base);
+      const int ci = (lane < cnt) ? S.inner[base + lane] : 0;
+      const cplx cv = (lane < cnt) ? S.vals[base + lane] : cmake(0.0, 0.0);
+      // batches of 8 entries: all 8 gathers are issued before the sums; the
+      // padding entries of the last batch repeat a valid column with weight 0
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        cplx x[8];
+        double vx[8], vy[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = min(k0 + i, cnt - 1);
+          const int col = __builtin_amdgcn_readlane(ci, k);
+          const bool real = k0 + i < cnt;
+          vx[i] = real ? readlane_d(cv.x, k) : 0.0;
+          vy[i] = real ? readlane_d(cv.y, k) : 0.0;
+          x[i] = active ? src[(int64_t)col * lds + t] : cmake(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          sx += vx[i] * x[i].x + vy[i] * x[i].y;
+          sy += vx[i] * x[i].y - vy[i] * x[i].x;
+        }
+      }
+    }
+    if (active) {
+      const cplx x0 = src[s * lds + t];
+      const cplx x = cmake(x0.x * ft, x0.y * ft);
+      const cplx dg = cmake(al.x * S.D[s] + be.x, al.y * S.D[s] + be.y);
+      const cplx dx = cmul(dg, x);
+      if (Vout) Vout[S.vo + s * m + t] = x;
+      Wout[S.vo + s * m + t] = cmake(sx * ft + dx.x, sy * ft + dx.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------- delayed CGS2
+// Batched form of the delayed classical Gram-Schmidt with one
+// reorthogonalisation (the single-vector form and its derivation are in
+// krylov.hip): step j holds, per slot, the pending vector u (A q_{j-1}
+// projected once) and w = A u.  One pass over Q_{j-1} gives c = Q^H u,
+// d = Q^H w, |u|^2 and u^H w (g_red_kernel); the small algebra of each slot
+// (g_dc_small_kernel) finishes column j-1 of its Hessenberg and the first pass
+// of column j; a second pass writes q_j and the next pending vector
+// (g_dc_update_kernel).  Two passes over the basis per vector instead of
+// CGS2's three fused ones.
+
+// part[P co + (chunk P + p) m + t], P = NR NQ + 2:
+//   p < NQ: conj(V_p) u;  NR == 2: NQ + p: conj(V_p) w;  P-2: |u|^2;  P-1: conj(u) w
+template <int NQ, int NR>
+__global__ void __launch_bounds__(TT)
+    g_red_kernel(const GSec* __restrict__ gs, int ns, const cplx* __restrict__ V, int64_t VS,
+                 const cplx* __restrict__ U, const cplx* __restrict__ W, cplx* __restrict__ part) {
+  constexpr int P = NR * NQ + 2;
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const cplx* __restrict__ Vb = V + S.vo;
+  const cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+  cplx ac[NQ], ad[NR == 2 ? NQ : 1];
+  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) ac[q] = cmake(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < (NR == 2 ? NQ : 1); ++q) ad[q] = cmake(0.0, 0.0);
+#pragma unroll 2
+  for (int64_t s = r0; s < r1; ++s) {
+    const int64_t o = s * m + t;
+    const cplx u = Ub[o];
+    const cplx w = (NR == 2) ? Wb[o] : cmake(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const cplx v = Vb[q * VS + o];
+      ac[q] = cadd(ac[q], cdotmul(v, u));
+      if (NR == 2) ad[q] = cadd(ad[q], cdotmul(v, w));
+    }
+    uu = cadd(uu, cdotmul(u, u));
+    if (NR == 2) uw = cadd(uw, cdotmul(u, w));
+  }
+  cplx* pb = part + P * S.co + (int64_t)tl.chunk * P * m + t;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) pb[(int64_t)q * m] = ac[q];
+  if (NR == 2) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pb[(int64_t)(NQ + q) * m] = ad[q];
+  }
+  pb[(int64_t)(P - 2) * m] = uu;
+  pb[(int64_t)(P - 1) * m] = uw;
+}
+
+// red[p Mtot + slot] = sum over chunks (fixed order) of the partials, P per slot.
+// One block per (p, 64-slot group).
+__global__ void __launch_bounds__(RW * TT)
+    g_reduce_p_kernel(const GSec* __restrict__ gs, int ns, int64_t ngroups, int Mtot, int P,
+                      const cplx* __restrict__ part, cplx* __restrict__ red) {
+  __shared__ cplx sh[RW][TT];
+  const int p = (int)(blockIdx.x / ngroups);
+  const int64_t gi = blockIdx.x % ngroups;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  cplx s = cmake(0.0, 0.0);
+  if (t < S.m) {
+    const cplx* pb = part + (int64_t)P * S.co;
+#pragma unroll 4
+    for (int c = wv; c < S.nch; c += RW) s = cadd(s, pb[((int64_t)c * P + p) * S.m + t]);
+  }
+  sh[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  s = sh[0][lane];
+  for (int w = 1; w < RW; ++w) s = cadd(s, sh[w][lane]);
+  red[(int64_t)p * Mtot + S.so + t] = s;
+}
+
+// Small algebra of step j for every slot (one thread each; Hess[(col (K+1) + row) Mtot + slot]).
+// last: finalisation (red holds c and |u|^2 only).  coef[q Mtot + slot]: a (= c) at
+// q < j, b at K + q; sc[slot] = {1/alpha, f_j/alpha}.  Below eps the column ends
+// the Krylov space: every coefficient is zero, so the later vectors vanish.
+__global__ void g_dc_small_kernel(int Mtot, int j, int K, int last, const cplx* __restrict__ red,
+                                  cplx* __restrict__ hess, cplx* __restrict__ coef, cplx* __restrict__ sc, double eps) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= Mtot) return;
+  const int NR = last ? 1 : 2;
+  const int P = NR * j + 2;
+  auto R = [&](int p) { return red[(int64_t)p * Mtot + g]; };
+  auto H = [&](int row, int col) -> cplx& { return hess[((int64_t)col * (K + 1) + row) * Mtot + g]; };
+  double cc = 0.0;
+  cplx chd = cmake(0.0, 0.0);
+  for (int i = 0; i < j; ++i) {
+    const cplx c = R(i);
+    cc += c.x * c.x + c.y * c.y;
+    if (!last) chd = cadd(chd, cdotmul(c, R(j + i)));
+  }
+  const double a2 = R(P - 2).x - cc;
+  const double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
+  // finish column j-1
+  for (int i = 0; i < j; ++i) H(i, j - 1) = cadd(H(i, j - 1), R(i));
+  if (j <= K) H(j, j - 1) = cmake(alpha, 0.0);
+  if (last) return;
+  if (!(alpha >= eps)) {
+    for (int i = 0; i < j; ++i) {
+      coef[(int64_t)i * Mtot + g] = cmake(0.0, 0.0);
+      coef[(int64_t)(K + i) * Mtot + g] = cmake(0.0, 0.0);
+    }
+    for (int i = 0; i <= j; ++i) H(i, j) = cmake(0.0, 0.0);
+    sc[2 * g] = cmake(0.0, 0.0);
+    sc[2 * g + 1] = cmake(0.0, 0.0);
+    return;
+  }
+  const double inv = 1.0 / alpha;
+  const cplx fj = crmul(inv * inv, csub(R(P - 1), chd));
+  // e = H[0:j+1, 0:j] c;  column j first pass = f - e / alpha
+  for (int i = 0; i <= j; ++i) {
+    cplx e = cmake(0.0, 0.0);
+    for (int k = (i > 0 ? i - 1 : 0); k < j; ++k) e = cadd(e, cmul(H(i, k), R(k)));
+    const cplx f = (i < j) ? crmul(inv, R(j + i)) : fj;
+    H(i, j) = csub(f, crmul(inv, e));
+    if (i < j) {
+      coef[(int64_t)i * Mtot + g] = R(i);
+      coef[(int64_t)(K + i) * Mtot + g] = csub(f, crmul(inv, cmul(R(i), fj)));
+    }
+  }
+  sc[2 * g] = cmake(inv, 0.0);
+  sc[2 * g + 1] = crmul(inv, fj);
+}
+
+// q_j = (u - sum_q V_q a_q) / alpha -> V_j;  u <- w / alpha - sum_q V_q b_q - u f_j / alpha
+template <int NQ>
+__global__ void __launch_bounds__(TT)
+    g_dc_update_kernel(const GSec* __restrict__ gs, int ns, cplx* __restrict__ V, int64_t VS, int j,
+                       cplx* __restrict__ U, const cplx* __restrict__ W, const cplx* __restrict__ coef,
+                       const cplx* __restrict__ sc, int Mtot, int K) {
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int g = S.so + t;
+  cplx a[NQ], b[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    a[q] = coef[(int64_t)q * Mtot + g];
+    b[q] = coef[(int64_t)(K + q) * Mtot + g];
+  }
+  const double inv = sc[2 * g].x;
+  const cplx f = sc[2 * g + 1];
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  cplx* __restrict__ Vb = V + S.vo;
+  cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+#pragma unroll 2
+  for (int64_t s = r0; s < r1; ++s) {
+    const int64_t o = s * m + t;
+    cplx s1 = cmake(0.0, 0.0), s2 = cmake(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const cplx v = Vb[q * VS + o];
+      s1 = cadd(s1, cmul(v, a[q]));
+      s2 = cadd(s2, cmul(v, b[q]));
+    }
+    const cplx u = Ub[o];
+    const cplx w = Wb[o];
+    Vb[(int64_t)j * VS + o] = crmul(inv, csub(u, s1));
+    const cplx uf = cmul(u, f);
+    Ub[o] = cmake(w.x * inv - s2.x - uf.x, w.y * inv - s2.y - uf.y);
+  }
+}
+
+// y_t = expm(-i dt K_t) e_1 for the k x k Hessenberg of each trajectory:
+// scaling and squaring with a truncated Taylor series applied to vectors
+// (Moler & Van Loan, "Nineteen dubious ways to compute the exponential of a
+// matrix", methods 1 and 3): A = -i dt K / 2^s with |A|_1 <= 1/2, then
+// y <- T_16(A) y, 2^s times (remainder below 1e-19 relative).
+// One wave per trajectory; lane i owns row i, the matrix and the current
+// Taylor term live in LDS.
+__global__ void __launch_bounds__(64)
+    small_expm_kernel(int m, int k, int K, const cplx* __restrict__ hess, double dt, cplx* __restrict__ coef) {
+  __shared__ cplx Hs[MAXK][MAXK + 1];
+  __shared__ cplx tv[MAXK];
+  __shared__ double cn[MAXK];
+  const int t = blockIdx.x;
+  const int i = threadIdx.x;
+  for (int e = i; e < k * k; e += 64) {
+    const int r = e % k, c = e / k;
+    Hs[r][c] = (r <= c + 1) ? hess[((int64_t)(c * (K + 1) + r)) * m + t] : cmake(0.0, 0.0);
+  }
+  __syncthreads();
+  if (i < k) {
+    double c = 0.0;
+    for (int r = 0; r <= min(i + 1, k - 1); ++r) c += sqrt(Hs[r][i].x * Hs[r][i].x + Hs[r][i].y * Hs[r][i].y);
+    cn[i] = c * fabs(dt);
+  }
+  __syncthreads();
+  double nrm = 0.0;
+  for (int c = 0; c < k; ++c) nrm = fmax(nrm, cn[c]);
+  int sq = 0;
+  while (nrm > 0.5 && sq < 16) {
+    nrm *= 0.5;
+    ++sq;
+  }
+  const double sc = ldexp(dt, -sq);
+  cplx y = cmake(i == 0 ? 1.0 : 0.0, 0.0);
+  const int reps = 1 << sq;
+  for (int rep = 0; rep < reps; ++rep) {
+    cplx term = y;
+    for (int p = 1; p <= 16; ++p) {
+      __syncthreads();
+      if (i < k) tv[i] = term;
+      __syncthreads();
+      cplx a = cmake(0.0, 0.0);
+      if (i < k)
+        for (int j = (i > 0 ? i - 1 : 0); j < k; ++j) a = cadd(a, cmul(Hs[i][j], tv[j]));
+      term = cmake(sc * a.y / p, -sc * a.x / p);  // -i sc a / p
+      y = cadd(y, term);
+    }
+  }
+  if (i < k) coef[i * m + t] = y;
+}
+
+// X[s,t] = sum_j V_j[s,t] coef[j Mtot + slot], all sectors (one thread per element of the pools)
+__global__ void g_combine_kernel(const GSec* __restrict__ gs, int ns, int64_t total, const cplx* __restrict__ V,
+                                 int64_t VS, int k, const cplx* __restrict__ coef, int Mtot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int si = find_sec(ns, e, [&](int i) { return gs[i].vo; });
+  const GSec& S = gs[si];
+  const int64_t local = e - S.vo;
+  const int64_t s = local / S.m;
+  const int t = (int)(local % S.m);
+  const int g = S.so + t;
+  cplx a = cmake(0.0, 0.0);
+  for (int j = 0; j < k; ++j) a = cadd(a, cmul(V[j * VS + e], coef[(int64_t)j * Mtot + g]));
+  S.X[s * S.ldx + t] = a;
+}
+
+// densities: part[(M+1) co + (chunk (M+1) + q) m + t] = sum_{s in chunk} |X[s,t]|^2 [bit q of state s]
+// (q = M: plain norm^2), all sectors
+__global__ void __launch_bounds__(TT)
+    g_dens_partial_kernel(const GSec* __restrict__ gs, int ns, int M, double* __restrict__ part) {
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  if (t >= m) return;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  double acc[MAXM + 1];
+#pragma unroll
+  for (int q = 0; q <= MAXM; ++q) acc[q] = 0.0;
+  for (int64_t s = r0; s < r1; ++s) {
+    const cplx v = S.X[s * S.ldx + t];
+    const double p = v.x * v.x + v.y * v.y;
+    const uint64_t x = S.states[s];
+#pragma unroll
+    for (int q = 0; q < MAXM; ++q)
+      if ((x >> q) & 1ull) acc[q] += p;
+    acc[MAXM] += p;
+  }
+  double* pb = part + (int64_t)(M + 1) * S.co;
+#pragma unroll
+  for (int q = 0; q < MAXM; ++q)
+    if (q < M) pb[((int64_t)tl.chunk * (M + 1) + q) * m + t] = acc[q];
+  pb[((int64_t)tl.chunk * (M + 1) + M) * m + t] = acc[MAXM];
+}
+
+// dens_all[traj (M+1) + q] = density q / norm^2 (q < M), norm^2 (q = M).
+// One block per (q, 64-slot group), chunks summed as in g_reduce_kernel.
+__global__ void __launch_bounds__(RW * TT)
+    g_dens_reduce_kernel(const GSec* __restrict__ gs, int ns, int64_t ngroups, int M, const double* __restrict__ part,
+                         const int* __restrict__ slot2traj, double* __restrict__ dens_all) {
+  __shared__ double sh[2][RW][TT];
+  const int q = (int)(blockIdx.x / ngroups);
+  const int64_t gi = blockIdx.x % ngroups;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  double v = 0.0, nn = 0.0;
+  if (t < S.m) {
+    const double* pb = part + (int64_t)(M + 1) * S.co;
+#pragma unroll 4
+    for (int c = wv; c < S.nch; c += RW) {
+      v += pb[((int64_t)c * (M + 1) + q) * S.m + t];
+      if (q < M) nn += pb[((int64_t)c * (M + 1) + M) * S.m + t];
+    }
+  }
+  sh[0][wv][lane] = v;
+  sh[1][wv][lane] = nn;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  v = sh[0][0][lane];
+  nn = sh[1][0][lane];
+  for (int w = 1; w < RW; ++w) {
+    v += sh[0][w][lane];
+    nn += sh[1][w][lane];
+  }
+  dens_all[(int64_t)slot2traj[S.so + t] * (M + 1) + q] = (q < M) ? v / nn : v;
+}
+
+template <int NR>
+void launch_red(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, const cplx* V, int64_t VS,
+                const cplx* U, const cplx* W, cplx* part) {
+  switch (nq) {
+#define QD_RED_CASE(N)                                                            \
+  case N:                                                                         \
+    g_red_kernel<N, NR><<<tiles, TT, 0, st>>>(gs, ns, V, VS, U, W, part);         \
+    break;
+    QD_RED_CASE(1) QD_RED_CASE(2) QD_RED_CASE(3) QD_RED_CASE(4) QD_RED_CASE(5) QD_RED_CASE(6)
+    QD_RED_CASE(7) QD_RED_CASE(8) QD_RED_CASE(9) QD_RED_CASE(10) QD_RED_CASE(11) QD_RED_CASE(12)
+    QD_RED_CASE(13) QD_RED_CASE(14) QD_RED_CASE(15) QD_RED_CASE(16) QD_RED_CASE(17) QD_RED_CASE(18)
+    QD_RED_CASE(19) QD_RED_CASE(20) QD_RED_CASE(21) QD_RED_CASE(22) QD_RED_CASE(23) QD_RED_CASE(24)
+    QD_RED_CASE(25) QD_RED_CASE(26) QD_RED_CASE(27) QD_RED_CASE(28) QD_RED_CASE(29) QD_RED_CASE(30)
+    QD_RED_CASE(31) QD_RED_CASE(32)
+#undef QD_RED_CASE
+    default:
+      break;
+  }
+}
+
+void launch_dc_update(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, cplx* V, int64_t VS, int j,
+                      cplx* U, const cplx* W, const cplx* coef, const cplx* sc, int Mtot, int K) {
+  switch (nq) {
+#define QD_UPD_CASE(N)                                                                         \
+  case N:                                                                                      \
+    g_dc_update_kernel<N><<<tiles, TT, 0, st>>>(gs, ns, V, VS, j, U, W, coef, sc, Mtot, K);    \
+    break;
+    QD_UPD_CASE(1) QD_UPD_CASE(2) QD_UPD_CASE(3) QD_UPD_CASE(4) QD_UPD_CASE(5) QD_UPD_CASE(6)
+    QD_UPD_CASE(7) QD_UPD_CASE(8) QD_UPD_CASE(9) QD_UPD_CASE(10) QD_UPD_CASE(11) QD_UPD_CASE(12)
+    QD_UPD_CASE(13) QD_UPD_CASE(14) QD_UPD_CASE(15) QD_UPD_CASE(16) QD_UPD_CASE(17) QD_UPD_CASE(18)
+    QD_UPD_CASE(19) QD_UPD_CASE(20) QD_UPD_CASE(21) QD_UPD_CASE(22) QD_UPD_CASE(23) QD_UPD_CASE(24)
+    QD_UPD_CASE(25) QD_UPD_CASE(26) QD_UPD_CASE(27) QD_UPD_CASE(28) QD_UPD_CASE(29) QD_UPD_CASE(30)
+    QD_UPD_CASE(31)
+#undef QD_UPD_CASE
+    default:
+      break;
+  }
+}
+
+unsigned g1(int64_t work, int b) { return (unsigned)std::max<int64_t>(1, (work + b - 1) / b); }
+
+// Rows per reduction chunk of a sector: about `target` waves for the sector
+// alone (target / groups chunks), 16 to 1024 rows, at most QD_TRAJ_MAX_CHUNKS
+// chunks.
+constexpr int QD_TRAJ_MAX_CHUNKS = 2048;
+int rows_per_chunk(int64_t n, int groups, int target) {
+  const int64_t want = std::max<int64_t>(1, target / std::max(1, groups));
+  int64_t rpc = std::max<int64_t>(16, (n + want - 1) / want);
+  rpc = std::min<int64_t>(rpc, 1024);
+  rpc = std::max<int64_t>(rpc, (n + QD_TRAJ_MAX_CHUNKS - 1) / QD_TRAJ_MAX_CHUNKS);
+  return (int)rpc;
+}
+
+
+// per-slot a-posteriori error estimate of the step (Saad 1992, section 5):
+// est = h_{K+1,K} |y_K| with y = expm(-i dt K_K) e_1 (coef) and h_{K+1,K} the
+// norm the finalisation left in Hess[K][K-1]
+__global__ void g_step_estimate_kernel(int Mtot, int K, const cplx* __restrict__ hess, const cplx* __restrict__ coef,
+                                       double* __restrict__ est) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= Mtot) return;
+  const cplx h = hess[((int64_t)(K - 1) * (K + 1) + K) * Mtot + g];
+  const cplx y = coef[(int64_t)(K - 1) * Mtot + g];
+  est[g] = sqrt(h.x * h.x + h.y * h.y) * sqrt(y.x * y.x + y.y * y.y);
+}
+
+// Work buffers of one batched Arnoldi step, sized for the largest batch.
+struct MrhsBufs {
+  cplx *V, *U, *W;      // K vector pools (VS each), pending vector, product
+  cplx* part;           // chunk partials
+  cplx *h, *hess;       // first projection; Hessenbergs [(col (K+1) + row) Mtot + slot]
+  cplx *red, *coef2;    // reduced partials; update coefficients
+  cplx *sc, *coef;      // per-slot scalars; y = expm(...) e_1 [q Mtot + slot]
+};
+
+// K Arnoldi vectors (delayed CGS2) for every slot: V_0 = f X (f = per-slot
+// 1/|x|), H_eff = H + alpha D + beta per slot, then y = expm(-i dt K) e_1.
+// X is not touched; mrhs_combine writes V y into it.
+inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t blocks, int64_t ngroups,
+                         int64_t VS, int Mtot, int K, double dt, const double* f, const cplx* alpha, const cplx* beta,
+                         const MrhsBufs& b) {
+  const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
+  const unsigned tl = (unsigned)tiles;
+  cplx* U = b.U;
+  cplx* W = b.W;
+  // V_0 = f X, U = H_eff V_0, then its first projection: H[0][0] = V_0^H U, U -= V_0 H[0][0]
+  g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 1, nullptr, f, alpha, beta, b.V, U);
+  g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, U, nullptr, Mtot, b.part);
+  g_reduce_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, ngroups, Mtot, 1, b.part, b.h, b.hess, 0, K,
+                                                          0, nullptr);
+  g_pass_kernel<1, 2><<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, U, b.h, Mtot, b.part);
+  for (int j = 1; j < K; ++j) {
+    const int P = 2 * j + 2;
+    g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 0, U, nullptr, alpha, beta, nullptr, W);
+    launch_red<2>(j, tl, st, d_gsec, ns, b.V, VS, U, W, b.part);
+    g_reduce_p_kernel<<<(unsigned)(ngroups * P), RW * TT, 0, st>>>(d_gsec, ns, ngroups, Mtot, P, b.part, b.red);
+    g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, j, K, 0, b.red, b.hess, b.coef2, b.sc, 1e-10);
+    launch_dc_update(j, tl, st, d_gsec, ns, b.V, VS, j, U, W, b.coef2, b.sc, Mtot, K);
+  }
+  // the pending vector's second projection completes column K-1 (and h_{K+1,K})
+  launch_red<1>(K, tl, st, d_gsec, ns, b.V, VS, U, nullptr, b.part);
+  g_reduce_p_kernel<<<(unsigned)(ngroups * (K + 2)), RW * TT, 0, st>>>(d_gsec, ns, ngroups, Mtot, K + 2, b.part,
+                                                                         b.red);
+  g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, K, K, 1, b.red, b.hess, b.coef2, b.sc, 1e-10);
+  small_expm_kernel<<<(unsigned)Mtot, 64, 0, st>>>(Mtot, K, K, b.hess, dt, b.coef);
+}
+
+inline void mrhs_combine(hipStream_t st, const GSec* d_gsec, int ns, int64_t VS, int Mtot, int K, const MrhsBufs& b) {
+  g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, b.V, VS, K, b.coef, Mtot);
+}
+
+}  // namespace

@@ -25,7 +25,7 @@
 #include <thread>
 #include <vector>
 
-#include "qd_internal.hpp"
+#include "mrhs.hpp"
 
 namespace {
 
@@ -34,6 +34,29 @@ constexpr int QD_SERVE_MAX_BYPASS = 8;
 __global__ void scale_diag_kernel(int64_t n, double c, const cplx* __restrict__ src, cplx* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = cmake(c * src[i].x, c * src[i].y);
+}
+
+__global__ void real_part_kernel(int64_t n, const cplx* __restrict__ d, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = d[i].x;
+}
+
+// columns of a trajectory-minor block X[s * ld + t]
+__global__ void col_gather_kernel(int64_t n, int ld, int t, const cplx* __restrict__ X, cplx* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) out[s] = X[s * ld + t];
+}
+__global__ void col_scatter_kernel(int64_t n, int ld, int t, const cplx* __restrict__ in, cplx* __restrict__ X) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) X[s * ld + t] = in[s];
+}
+__global__ void col_copy_kernel(int64_t n, int ld, int src, int dst, cplx* __restrict__ X) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) X[s * ld + dst] = X[s * ld + src];
+}
+__global__ void inv_sqrt_kernel(int m, const double* __restrict__ n2, double* __restrict__ f) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) f[t] = n2[t] > 0.0 ? 1.0 / sqrt(n2[t]) : 0.0;
 }
 
 // diag[j] = value of column j's diagonal entry; flag = 1 if any entry is off the diagonal
@@ -98,6 +121,7 @@ struct Model {
   qd_csr* csrA = nullptr;
   qd_linop* opA = nullptr;
   cplx* d_diagD = nullptr;
+  double* d_diagR = nullptr;  // its real part (the diagonal of a Hermitian D)
   size_t bytes = 0;
   std::vector<qd_term> tA, tD;
   std::vector<qd_op> oA, oD;
@@ -126,6 +150,8 @@ struct qd_server {
   int running = 0, max_running = 0;
   size_t peak_used = 0;
   bool stopping = false;
+  int batch_slots = 1;             // evolve requests of one model stepped together (1: no batching)
+  std::vector<char> batch_active;  // per model: a batch owner admits its evolve requests
   std::vector<std::thread> threads;
   std::chrono::steady_clock::time_point t0;
   std::string err;
@@ -302,6 +328,10 @@ bool pick(qd_server* S, Job* job) {
       S->cv.notify_all();
       continue;
     }
+    if (it->req.kind == QD_REQ_EVOLVE && S->batch_active[it->req.model]) {
+      ++it;  // its model's batch owner takes it at the next step
+      continue;
+    }
     if (it->footprint <= S->budget - S->used) {
       if (head_waiting) {
         Job& head = S->queue.front();
@@ -318,6 +348,260 @@ bool pick(qd_server* S, Job* job) {
     ++it;
   }
   return false;
+}
+
+// ---------------------------------------------------------- evolve batches
+// Evolve requests of one model with the same kry_dim_rt and dt advance
+// together: one multi-RHS Arnoldi step (delayed CGS2 over all slots, the
+// stored A read once per product for every slot, each slot's own diagonal
+// coeff * D) per time step.  Requests that arrive while the batch runs join it
+// at the next step; finished ones leave.  The Krylov dimension is shared: it
+// grows by 2 (the step is redone) while the largest per-slot estimate of the
+// step error exceeds the smallest tol, and shrinks by 1 when all are 100 times
+// below it.  The batch reserves `footprint` per slot of the HBM budget.
+// The product of the batch gathers one row of all slots per stored entry; for
+// large sectors with few slots that moves more bytes than the matrix-free
+// single-vector products of the same requests, so batching is off by default.
+struct BatchSlot {
+  Job job;
+  int steps_done = 0;
+  double e_start = 0.0;
+  double t_start = 0.0;
+  int64_t msum = 0;
+};
+
+void finish_result(qd_server* S, const qd_result& res) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  --S->running;
+  S->results.push_back(res);
+  ++S->finished;
+}
+
+int view_energy(qd_ctx* ctx, const Model& M, double coeff, const cplx* psi, cplx* diagc, double* e) {
+  const int64_t n = M.n;
+  scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, coeff, M.d_diagD, diagc);
+  qd_linop* op = nullptr;
+  int rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diagc, &op);
+  if (rc) return rc;
+  double h[2] = {0.0, 0.0}, n2 = 0.0;
+  rc = qd_expect(ctx, op, (const double*)psi, h);
+  if (!rc) rc = qd_norm2(ctx, n, (const double*)psi, &n2);
+  qd_linop_destroy(op);
+  *e = n2 > 0.0 ? h[0] / n2 : 0.0;
+  return rc;
+}
+
+void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
+  const int model = first.req.model;
+  const Model& M = *S->models[model];
+  const int64_t n = M.n;
+  const int Kmax = std::max(1, std::min(first.req.k - 1, MAXK));
+  const double dt = first.req.dt;
+  hipStream_t st = ctx->stream;
+  int C = 1;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    while (C < S->batch_slots && S->used + first.footprint <= S->budget) {
+      S->used += first.footprint;
+      ++C;
+    }
+    S->peak_used = std::max(S->peak_used, S->used);
+    S->batch_active[model] = 1;
+  }
+  const int max_nch = (int)std::min<int64_t>(QD_TRAJ_MAX_CHUNKS, (n + 15) / 16);
+  const int PMAX = 2 * Kmax + 2;
+  std::vector<void*> allocs;
+  bool ok = true;
+  auto dalloc = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    allocs.push_back(p);
+    return p;
+  };
+  const size_t vecC = sizeof(cplx) * (size_t)n * C;
+  cplx* X = (cplx*)dalloc(vecC);
+  MrhsBufs mb{};
+  mb.V = (cplx*)dalloc(vecC * Kmax);
+  mb.U = (cplx*)dalloc(vecC);
+  mb.W = (cplx*)dalloc(vecC);
+  mb.part = (cplx*)dalloc(sizeof(cplx) * (size_t)max_nch * PMAX * C);
+  mb.h = (cplx*)dalloc(sizeof(cplx) * C);
+  mb.hess = (cplx*)dalloc(sizeof(cplx) * (size_t)Kmax * (Kmax + 1) * C);
+  mb.red = (cplx*)dalloc(sizeof(cplx) * (size_t)PMAX * C);
+  mb.coef2 = (cplx*)dalloc(sizeof(cplx) * (size_t)2 * Kmax * C);
+  mb.sc = (cplx*)dalloc(sizeof(cplx) * 2 * C);
+  mb.coef = (cplx*)dalloc(sizeof(cplx) * (size_t)Kmax * C);
+  cplx* d_alpha = (cplx*)dalloc(sizeof(cplx) * C);
+  cplx* d_beta = (cplx*)dalloc(sizeof(cplx) * C);
+  double* d_scale = (double*)dalloc(sizeof(double) * C);
+  double* d_est = (double*)dalloc(sizeof(double) * C);
+  double* d_n2 = (double*)dalloc(sizeof(double) * C);
+  int* d_ids = (int*)dalloc(sizeof(int) * C);
+  GSec* d_gsec = (GSec*)dalloc(sizeof(GSec));
+  cplx* tmp = (cplx*)dalloc(sizeof(cplx) * n);
+  cplx* diagc = (cplx*)dalloc(sizeof(cplx) * n);
+  std::vector<BatchSlot> slots;
+  std::vector<int> ids(C);
+  for (int i = 0; i < C; ++i) ids[i] = i;
+  std::vector<cplx> h_alpha(C), h_beta(C, cmake(0.0, 0.0));
+  std::vector<double> h_est(C);
+  if (ok) {
+    hipMemcpyAsync(d_ids, ids.data(), sizeof(int) * C, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(d_beta, h_beta.data(), sizeof(cplx) * C, hipMemcpyHostToDevice, st);
+  }
+  auto fail_slot = [&](BatchSlot& sl, int code) {
+    qd_result res{};
+    res.tag = sl.job.req.tag;
+    res.status = code;
+    res.t_submit = sl.job.t_submit;
+    res.t_start = sl.t_start;
+    res.t_end = S->now();
+    res.worker = wi;
+    finish_result(S, res);
+  };
+  std::vector<Job> pending{first};
+  int Kcur = std::min(Kmax, first.req.chunk > 0 ? first.req.chunk : 12);
+  int rc = ok ? QD_OK : QD_E_OOM;
+  for (;;) {
+    // admit: this model's queued evolve requests with the same k and dt
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (auto it = S->queue.begin(); it != S->queue.end() && (int)(slots.size() + pending.size()) < C;) {
+        if (it->req.kind == QD_REQ_EVOLVE && it->req.model == model && it->req.k == first.req.k &&
+            it->req.dt == dt) {
+          pending.push_back(*it);
+          it = S->queue.erase(it);
+          ++S->running;
+          S->max_running = std::max(S->max_running, S->running);
+        } else {
+          ++it;
+        }
+      }
+      if (pending.empty() && slots.empty()) {
+        S->batch_active[model] = 0;  // new arrivals start a new batch
+        break;
+      }
+    }
+    for (const Job& j : pending) {
+      BatchSlot sl;
+      sl.job = j;
+      sl.t_start = S->now();
+      if (rc == QD_OK) rc = qd_seeded_state(ctx, n, j.req.seed, (double*)tmp);
+      if (rc == QD_OK) rc = view_energy(ctx, M, j.req.coeff, tmp, diagc, &sl.e_start);
+      if (rc != QD_OK) {
+        fail_slot(sl, rc);
+        continue;
+      }
+      col_scatter_kernel<<<blocks_for(n, 256), 256, 0, st>>>(n, C, (int)slots.size(), tmp, X);
+      slots.push_back(sl);
+    }
+    pending.clear();
+    // requests with nothing (left) to do
+    for (int t = (int)slots.size() - 1; rc == QD_OK && t >= 0; --t) {
+      BatchSlot& sl = slots[t];
+      if (sl.steps_done < sl.job.req.steps) continue;
+      qd_result res{};
+      res.tag = sl.job.req.tag;
+      res.t_submit = sl.job.t_submit;
+      res.t_start = sl.t_start;
+      res.worker = wi;
+      res.iterations = sl.steps_done;
+      res.krylov_mean = sl.steps_done ? (double)sl.msum / sl.steps_done : 0.0;
+      res.energy_start = sl.e_start;
+      col_gather_kernel<<<blocks_for(n, 256), 256, 0, st>>>(n, C, t, X, tmp);
+      double e = 0.0, dexp = 0.0, n2 = 0.0;
+      rc = view_energy(ctx, M, sl.job.req.coeff, tmp, diagc, &e);
+      if (rc == QD_OK) rc = diag_expect(ctx, n, M.d_diagD, tmp, &dexp, &n2);
+      res.status = rc;
+      res.energy = e;
+      res.observable = dexp;
+      res.t_end = S->now();
+      finish_result(S, res);
+      const int last = (int)slots.size() - 1;
+      if (t != last) {
+        col_copy_kernel<<<blocks_for(n, 256), 256, 0, st>>>(n, C, last, t, X);
+        slots[t] = slots[last];
+      }
+      slots.pop_back();
+    }
+    if (rc != QD_OK) break;
+    const int m = (int)slots.size();
+    if (m == 0) continue;
+    // one time step for every slot
+    GSec g{};
+    g.n = n;
+    g.m = m;
+    g.ldx = C;
+    g.groups = (m + TT - 1) / TT;
+    g.rpc = rows_per_chunk(n, g.groups, 2048);
+    g.nch = (int)((n + g.rpc - 1) / g.rpc);
+    g.X = X;
+    g.outer = M.csrA->d_outer;
+    g.inner = M.csrA->d_inner;
+    g.vals = M.csrA->d_vals;
+    g.D = M.d_diagR;
+    g.states = M.basis->d_states;
+    const int64_t tiles = (int64_t)g.nch * g.groups;
+    const int64_t blocks = (n + SPMM_RB - 1) / SPMM_RB * g.groups;
+    const int64_t ngroups = g.groups;
+    const int64_t VS = n * (int64_t)m;
+    double tol = 1e300;
+    for (int t = 0; t < m; ++t) {
+      h_alpha[t] = cmake(slots[t].job.req.coeff, 0.0);
+      if (slots[t].job.req.tol > 0.0) tol = std::min(tol, slots[t].job.req.tol);
+    }
+    hipMemcpyAsync(d_gsec, &g, sizeof(GSec), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(d_alpha, h_alpha.data(), sizeof(cplx) * m, hipMemcpyHostToDevice, st);
+    // 1 / |x| per slot (the density kernels with no modes give |x|^2)
+    g_dens_partial_kernel<<<(unsigned)tiles, TT, 0, st>>>(d_gsec, 1, 0, (double*)mb.part);
+    g_dens_reduce_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, 1, ngroups, 0, (const double*)mb.part, d_ids,
+                                                                d_n2);
+    inv_sqrt_kernel<<<g1(m, 256), 256, 0, st>>>(m, d_n2, d_scale);
+    double maxest = 0.0;
+    for (;;) {
+      mrhs_arnoldi(st, d_gsec, 1, tiles, blocks, ngroups, VS, m, Kcur, dt, d_scale, d_alpha, d_beta, mb);
+      g_step_estimate_kernel<<<g1(m, 256), 256, 0, st>>>(m, Kcur, mb.hess, mb.coef, d_est);
+      hipMemcpyAsync(h_est.data(), d_est, sizeof(double) * m, hipMemcpyDeviceToHost, st);
+      if (hipStreamSynchronize(st) != hipSuccess) {
+        rc = QD_E_HIP;
+        break;
+      }
+      maxest = 0.0;
+      for (int t = 0; t < m; ++t) maxest = std::max(maxest, h_est[t]);
+      if (tol < 1e300 && maxest > tol && Kcur < Kmax) {
+        Kcur = std::min(Kmax, Kcur + 2);  // redo the step with a larger basis (X is untouched)
+        continue;
+      }
+      break;
+    }
+    if (rc != QD_OK) break;
+    mrhs_combine(st, d_gsec, 1, VS, m, Kcur, mb);
+    for (auto& sl : slots) {
+      ++sl.steps_done;
+      sl.msum += Kcur;
+    }
+    if (tol < 1e300 && maxest < 0.01 * tol && Kcur > 2) --Kcur;
+  }
+  if (rc != QD_OK) {
+    for (auto& sl : slots) fail_slot(sl, rc);
+    for (auto& j : pending) {
+      BatchSlot sl;
+      sl.job = j;
+      fail_slot(sl, rc);
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->batch_active[model] = 0;
+  }
+  hipStreamSynchronize(st);
+  for (void* p : allocs) hipFree(p);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->used -= (size_t)C * first.footprint;
+  }
+  S->cv.notify_all();
 }
 
 void worker_main(qd_server* S, int wi) {
@@ -353,6 +637,10 @@ void worker_main(qd_server* S, int wi) {
       }
       ++S->running;
       S->max_running = std::max(S->max_running, S->running);
+    }
+    if (job.req.kind == QD_REQ_EVOLVE && S->batch_slots > 1) {
+      run_evolve_batch(S, ctx, job, wi);
+      continue;
     }
     qd_result res{};
     res.tag = job.req.tag;
@@ -417,6 +705,7 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
     if (M->basis) qd_basis_destroy(M->basis);
     hipStreamSynchronize(ctx->stream);
     if (M->d_diagD) hipFree(M->d_diagD);
+    if (M->d_diagR) hipFree(M->d_diagR);
     delete M;
     return code;
   };
@@ -454,13 +743,17 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
   } else {
     hipMemsetAsync(M->d_diagD, 0, sizeof(cplx) * M->n, ctx->stream);
   }
+  if (hipMalloc(&M->d_diagR, sizeof(double) * M->n) != hipSuccess)
+    return fail(qd_set_error(ctx, QD_E_OOM, "model diagonal"));
+  real_part_kernel<<<blocks_for(M->n, 256), 256, 0, ctx->stream>>>(M->n, M->d_diagD, M->d_diagR);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(qd_set_error(ctx, QD_E_HIP, "model set-up"));
   const qd_csr* c = M->csrA;
   M->bytes = sizeof(uint64_t) * (size_t)M->n + sizeof(int32_t) * (size_t)(M->n + 1) +
-             (sizeof(int32_t) + sizeof(cplx)) * (size_t)c->nnz + sizeof(cplx) * (size_t)M->n;
+             (sizeof(int32_t) + sizeof(cplx)) * (size_t)c->nnz + (sizeof(cplx) + sizeof(double)) * (size_t)M->n;
   std::lock_guard<std::mutex> lk(S->mu);
   *model_id = (int)S->models.size();
   S->models.push_back(M);
+  S->batch_active.push_back(0);
   S->used += M->bytes;
   S->peak_used = std::max(S->peak_used, S->used);
   return QD_OK;
@@ -501,6 +794,13 @@ extern "C" int qd_server_drain(qd_server* S) {
   return QD_OK;
 }
 
+extern "C" int qd_server_set_batch(qd_server* S, int slots) {
+  if (!S || slots < 1 || slots > 4096) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->batch_slots = slots;
+  return QD_OK;
+}
+
 extern "C" double qd_server_clock(qd_server* S) { return S ? S->now() : 0.0; }
 
 extern "C" int qd_server_get_stats(qd_server* S, qd_server_stats* st) {
@@ -531,6 +831,7 @@ extern "C" int qd_server_destroy(qd_server* S) {
     if (M->progA) qd_termprog_destroy(M->progA);
     if (M->basis) qd_basis_destroy(M->basis);
     if (M->d_diagD) hipFree(M->d_diagD);
+    if (M->d_diagR) hipFree(M->d_diagR);
     delete M;
   }
   if (S->setup) qd_ctx_destroy(S->setup);

@@ -64,6 +64,24 @@ def test_result_same_alone_and_in_the_mix(ctx):
         assert alone["iterations"] == mix[r["tag"]]["iterations"]
 
 
+@pytest.mark.parametrize("batch", [1, 16])
+def test_evolve_batching_matches_single(ctx, batch):
+    """Lockstep batches of evolve requests (joining at step boundaries) give the
+    plain path's results; batch = 1 runs each request alone."""
+    reqs = [{"tag": i, "arrival": 0.003 * i, "kind": "rt", "coeff": u, "seed": 300 + i}
+            for i, u in enumerate([2.5, 7.0, 4.1, 5.5, 3.3])]
+    served = SV.run_served(reqs, lattice=LATTICE, workers=2, hbm_budget=8 << 30, batch=batch, **KW)
+    plain = SV.run_plain(reqs, ctx, lattice=LATTICE, gs_k=KW["gs_k"], rt_k=KW["rt_k"], rt_steps=KW["rt_steps"],
+                         dt=KW["dt"])
+    for r in reqs:
+        s, p = served["results"][r["tag"]], plain["results"][r["tag"]]
+        assert s["status"] == 0 and s["iterations"] == KW["rt_steps"]
+        assert abs(s["energy_start"] - p["energy_start"]) <= 1e-12 * abs(p["energy_start"])
+        assert abs(s["energy"] - p["energy"]) <= 1e-10 * abs(p["energy"])
+        assert abs(s["observable"] - p["observable"]) <= 1e-10
+        assert 2 <= s["krylov_mean"] <= KW["rt_k"] - 1
+
+
 def test_admission_defers_and_rejects(ctx):
     idx, A, D = Q.xxz_parts(12, 1.0, pbc=True)
     sec = Q.sz_sector(idx, 12, 6)
