@@ -129,6 +129,7 @@ struct Model {
 
 struct Job {
   qd_request req;
+  const Model* model = nullptr;  // resolved under the server lock at submission
   double t_submit = 0.0;
   size_t footprint = 0;
   int bypassed = 0;
@@ -393,7 +394,7 @@ int view_energy(qd_ctx* ctx, const Model& M, double coeff, const cplx* psi, cplx
 
 void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
   const int model = first.req.model;
-  const Model& M = *S->models[model];
+  const Model& M = *first.model;
   const int64_t n = M.n;
   const int Kmax = std::max(1, std::min(first.req.k - 1, MAXK));
   const double dt = first.req.dt;
@@ -647,7 +648,7 @@ void worker_main(qd_server* S, int wi) {
     res.t_submit = job.t_submit;
     res.t_start = S->now();
     res.worker = wi;
-    const Model& M = *S->models[job.req.model];
+    const Model& M = *job.model;
     res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(ctx, M, job.req, &res)
                                                      : run_evolve(ctx, M, job.req, &res);
     res.t_end = S->now();
@@ -769,8 +770,9 @@ extern "C" int qd_server_submit(qd_server* S, const qd_request* r) {
   }
   Job j;
   j.req = *r;
+  j.model = S->models[r->model];
   j.t_submit = S->now();
-  j.footprint = request_footprint(*S->models[r->model], *r);
+  j.footprint = request_footprint(*j.model, *r);
   S->queue.push_back(j);
   ++S->submitted;
   S->cv.notify_all();
