@@ -243,6 +243,8 @@ typedef struct {
   double ritz_resid; /* |(H - E0) psi| of the last ground-state Ritz pair,
                         beta_m |z_m| (no reference counterpart) */
   double ritz_gap;   /* second minus lowest Ritz value of that solve */
+  int full_steps;    /* method 2: steps that projected against the whole
+                        basis since the basis was started */
 } qd_krylov_info;
 int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info);
 /* Ritz values of the last ground-state solve, ascending (storeE, :587-593).
@@ -253,7 +255,15 @@ int qd_krylov_converge_ev(qd_krylov* kr, int N, double iv_re, double iv_im);
 /* Forget the held basis (act_dim = 0). */
 int qd_krylov_reset(qd_krylov* kr);
 /* Reorthogonalisation scheme: 1 = delayed CGS2 (two passes over the basis
- * per step; default), 0 = CGS2 (four passes). */
+ * per step; default), 0 = CGS2 (four passes).  For Hermitian operators only
+ * (no reference counterpart): 2 = delayed CGS2 over the last two basis
+ * vectors with partial reorthogonalisation (Simon 1984): a recurrence
+ * estimates the loss of orthogonality and, when it passes
+ * sqrt(eps / kry_dim) / 4, the next two steps project against the whole
+ * basis, so the basis stays semi-orthogonal and the tridiagonal matrix
+ * agrees with a fully reorthogonalised one to O(eps |H|); 3 = the local
+ * window only (no estimate; Ritz values may acquire spurious copies).
+ * Takes effect when a basis is started. */
 int qd_krylov_set_method(qd_krylov* kr, int method);
 
 /* ------------------------------------------------ batched trajectories */

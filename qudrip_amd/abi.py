@@ -73,7 +73,7 @@ class qd_server_stats(ctypes.Structure):
 class qd_krylov_info(ctypes.Structure):
     _fields_ = [("act_dim", ctypes.c_int), ("breakdown", ctypes.c_int), ("error", ctypes.c_double),
                 ("beta_last", ctypes.c_double), ("n_spmv", ctypes.c_int64), ("ritz_resid", ctypes.c_double),
-                ("ritz_gap", ctypes.c_double)]
+                ("ritz_gap", ctypes.c_double), ("full_steps", ctypes.c_int)]
 
 
 VP = ctypes.c_void_p

@@ -37,11 +37,26 @@ __device__ inline cplx wave_sum(cplx s) {
   return s;
 }
 
+// Columns [lo, ncols) of the basis that a projection covers.  arr == nullptr:
+// the whole basis.  Otherwise arr[j] is the first column for step j: >= 0 as
+// set by the step before (0 when partial reorthogonalisation asks for a full
+// pass), -1 for the local window max(0, j - W).
+struct Win {
+  const int* arr;
+  int j;
+  int W;
+  __device__ int lo() const {
+    if (!arr) return 0;
+    const int v = arr[j];
+    return v >= 0 ? v : max(0, j - W);
+  }
+};
+
 // part[b * ncols + c] = sum over rows of block b of conj(V[r, c]) * w[r]
 template <int RPT>
 __global__ void __launch_bounds__(KT)
     gemv_t_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ w,
-                  cplx* __restrict__ part, const int* __restrict__ flag) {
+                  cplx* __restrict__ part, const int* __restrict__ flag, Win win = {nullptr, 0, 0}) {
   if (flag && *flag) return;
   __shared__ cplx sh[KT / 64][64];
   const int64_t base = (int64_t)blockIdx.x * KT * RPT;
@@ -52,7 +67,7 @@ __global__ void __launch_bounds__(KT)
     int64_t r = base + (int64_t)q * KT + threadIdx.x;
     wr[q] = (r < n) ? w[r] : cmake(0.0, 0.0);
   }
-  for (int c0 = 0; c0 < ncols; c0 += 64) {
+  for (int c0 = win.lo(); c0 < ncols; c0 += 64) {
     const int cend = min(ncols, c0 + 64);
     for (int c = c0; c < cend; ++c) {
       const cplx* __restrict__ Vc = V + (int64_t)c * ld;
@@ -80,12 +95,22 @@ __global__ void __launch_bounds__(KT)
 }
 
 // h[c] = sum_b part[b * ncols + c]; kcol[c] (+)= h[c].  One block per column.
+// With a window (win.arr set), entries c < nrv * jcols whose basis column
+// c % jcols lies below the window are zero (no block wrote them).
 __global__ void __launch_bounds__(KT)
     reduce_cols_kernel(const cplx* __restrict__ part, int nb, int ncols, cplx* __restrict__ h,
-                       cplx* __restrict__ kcol, int accumulate, const int* __restrict__ flag, int nk = 1 << 30) {
+                       cplx* __restrict__ kcol, int accumulate, const int* __restrict__ flag, int nk = 1 << 30,
+                       Win win = {nullptr, 0, 0}, int jcols = 1, int nrv = 0) {
   if (flag && *flag) return;
   __shared__ cplx sh[KT / 64];
   const int c = blockIdx.x;
+  if (win.arr && c < nrv * jcols && c % jcols < win.lo()) {
+    if (threadIdx.x == 0) {
+      h[c] = cmake(0.0, 0.0);
+      if (kcol && c < nk && !accumulate) kcol[c] = cmake(0.0, 0.0);
+    }
+    return;
+  }
   cplx s = cmake(0.0, 0.0);
   for (int b = threadIdx.x; b < nb; b += KT) s = cadd(s, part[(int64_t)b * ncols + c]);
   s = wave_sum(s);
@@ -103,7 +128,8 @@ __global__ void __launch_bounds__(KT)
 template <int RPT>
 __global__ void __launch_bounds__(KT)
     gemv_n_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ h,
-                  cplx* __restrict__ w, double* __restrict__ norm_part, const int* __restrict__ flag) {
+                  cplx* __restrict__ w, double* __restrict__ norm_part, const int* __restrict__ flag,
+                  Win win = {nullptr, 0, 0}) {
   if (flag && *flag) return;
   __shared__ double sh[KT / 64];
   const int64_t base = (int64_t)blockIdx.x * KT * RPT;
@@ -113,7 +139,7 @@ __global__ void __launch_bounds__(KT)
     int64_t r = base + (int64_t)q * KT + threadIdx.x;
     acc[q] = (r < n) ? w[r] : cmake(0.0, 0.0);
   }
-  for (int c = 0; c < ncols; ++c) {
+  for (int c = win.lo(); c < ncols; ++c) {
     const cplx hc = h[c];
     const cplx* __restrict__ Vc = V + (int64_t)c * ld;
 #pragma unroll
@@ -270,7 +296,8 @@ __global__ void sum_double_kernel(const double* __restrict__ part, int nb, doubl
 template <int RPT, int NR>
 __global__ void __launch_bounds__(KT)
     dc_red_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ u,
-                  const cplx* __restrict__ w, cplx* __restrict__ part, const int* __restrict__ flag) {
+                  const cplx* __restrict__ w, cplx* __restrict__ part, const int* __restrict__ flag,
+                  Win win) {
   if (*flag) return;
   __shared__ cplx sh[KT / 64][64][NR];
   __shared__ cplx sx[KT / 64][2];
@@ -293,7 +320,7 @@ __global__ void __launch_bounds__(KT)
     sx[wv][0] = uu;
     sx[wv][1] = uw;
   }
-  for (int c0 = 0; c0 < ncols; c0 += 64) {
+  for (int c0 = win.lo(); c0 < ncols; c0 += 64) {
     const int cend = min(ncols, c0 + 64);
     int c = c0;
     // two columns at a time: twice the loads in flight per wave and two
@@ -380,13 +407,87 @@ __global__ void __launch_bounds__(KT)
   }
 }
 
+// Partial reorthogonalisation (Simon, "The Lanczos algorithm with partial
+// reorthogonalization", Math. Comp. 42 (1984) 115-142) on top of the delayed
+// scheme, for Hermitian operators: each step projects against the local
+// window of W columns only, and omega_{j,k} ~ q_j^H q_k is estimated from the
+// tridiagonal entries by Simon's recurrence
+//   beta_j w_{j,k} = beta_{k+1} w_{j-1,k+1} + (alpha_k - alpha_{j-1}) w_{j-1,k}
+//                    + beta_k w_{j-1,k-1} - beta_{j-1} w_{j-2,k} +- eps1 |T|
+// (entries inside the window reset to eps1, w_{j,j} = 1).  When max |w_{j,k}|
+// passes the trigger level the next two steps project against the whole
+// basis (the pending vector's second pass and its successor's first and
+// second), which keeps the basis semi-orthogonal: the tridiagonal then
+// agrees with that of a fully orthogonal basis to O(eps |T|).
+struct Pro {
+  int* lo;      // first column per step (Win::arr); nullptr: no estimate
+  double* om;   // three rotating rows of the estimate, stride ldo
+  double* prm;  // [0] |T| estimate, [1] eps1, [2] trigger level, [3] full passes
+  int ldo;
+  int W;
+};
+
+__device__ void pro_step(const Pro& P, const cplx* __restrict__ KR, int ldk, int j, double beta) {
+  __shared__ double smax[KT / 64];
+  __syncthreads();  // column j - 1 is final
+  const int lo_here = Win{P.lo, j, P.W}.lo();
+  double* rj = P.om + (size_t)(j % 3) * P.ldo;
+  const double* r1 = P.om + (size_t)((j + 2) % 3) * P.ldo;  // row j - 1
+  const double* r2 = P.om + (size_t)((j + 1) % 3) * P.ldo;  // row j - 2
+  const double aj1 = KR[(size_t)(j - 1) * ldk + j - 1].x;
+  const double bj1 = j >= 2 ? KR[(size_t)(j - 2) * ldk + j - 1].x : 0.0;
+  const double eps1 = P.prm[1];
+  const double T = eps1 * P.prm[0];
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < j; k += KT) {
+    double v = eps1;
+    if (k < lo_here) {  // lo_here <= j - W, so k + 1 < j - 1
+      const double ak = KR[(size_t)k * ldk + k].x;
+      const double bk1 = KR[(size_t)k * ldk + k + 1].x;
+      double t = bk1 * r1[k + 1] + (ak - aj1) * r1[k] - bj1 * r2[k];
+      if (k > 0) t += KR[(size_t)(k - 1) * ldk + k].x * r1[k - 1];
+      v = (t + copysign(T, t)) / beta;
+      mx = fmax(mx, fabs(v));
+    }
+    rj[k] = v;
+  }
+  if (threadIdx.x == 0) rj[j] = 1.0;
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = smax[0];
+    for (int q = 1; q < KT / 64; ++q) m = fmax(m, smax[q]);
+    P.prm[0] = fmax(P.prm[0], fabs(aj1) + bj1 + beta);
+    if (lo_here == 0 && j > P.W) P.prm[3] += 1.0;
+    if (m > P.prm[2]) {
+      P.lo[j + 1] = 0;
+      P.lo[j + 2] = 0;
+    } else if (P.lo[j + 1] != 0) {
+      P.lo[j + 1] = max(0, j + 1 - P.W);
+    }
+  }
+}
+
+__global__ void pro_init_kernel(int* __restrict__ lo, int nlo, double* __restrict__ om, double* __restrict__ prm,
+                                double eps1, double trigger) {
+  for (int i = threadIdx.x; i < nlo; i += blockDim.x) lo[i] = -1;
+  if (threadIdx.x == 0) {
+    om[0] = 1.0;  // w_{0,0}
+    prm[0] = 0.0;
+    prm[1] = eps1;
+    prm[2] = trigger;
+    prm[3] = 0.0;
+  }
+}
+
 // small algebra of step j (one block).  red: reduced partials (layout above).
 // KR column k at KR + k * ldk, rows 0..k+1.  Outputs coefA (= c), coefB and
 // scal = {1/alpha, Re, Im of f[j]/alpha}.  last != 0: finalisation (no w).
 __global__ void __launch_bounds__(KT)
     dc_small_kernel(const cplx* __restrict__ red, int j, int last, cplx* __restrict__ KR, int ldk,
                     cplx* __restrict__ coefA, cplx* __restrict__ coefB, double* __restrict__ scal,
-                    int* __restrict__ flag, int* __restrict__ bd_step, double eps) {
+                    int* __restrict__ flag, int* __restrict__ bd_step, double eps, Pro pro) {
   if (*flag) return;
   __shared__ double sred[KT / 64];
   __shared__ cplx cred[KT / 64];
@@ -437,6 +538,7 @@ __global__ void __launch_bounds__(KT)
   for (int i = threadIdx.x; i < j; i += KT) col[i] = cadd(col[i], c[i]);
   if (threadIdx.x == 0) col[j] = cmake(alpha, 0.0);
   if (!(alpha >= eps)) return;
+  if (pro.lo) pro_step(pro, KR, ldk, j, alpha);
   const double inv = 1.0 / alpha;
   for (int i = threadIdx.x; i < j; i += KT) coefA[i] = c[i];
   if (threadIdx.x == 0) scal[0] = inv;
@@ -465,13 +567,13 @@ template <int RPT, bool FINAL>
 __global__ void __launch_bounds__(KT)
     dc_update_kernel(int64_t n, cplx* __restrict__ V, int64_t ld, int j, cplx* __restrict__ u,
                      const cplx* __restrict__ w, const cplx* __restrict__ coefA, const cplx* __restrict__ coefB,
-                     const double* __restrict__ scal, const int* __restrict__ flag) {
+                     const double* __restrict__ scal, const int* __restrict__ flag, Win win) {
   if (*flag) return;
   const int64_t base = (int64_t)blockIdx.x * KT * RPT;
   cplx s1[RPT], s2[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) s1[q] = s2[q] = cmake(0.0, 0.0);
-  for (int c = 0; c < j; ++c) {
+  for (int c = win.lo(); c < j; ++c) {
     const cplx a = coefA[c];
     const cplx bcf = FINAL ? cmake(0.0, 0.0) : coefB[c];
     const cplx* __restrict__ Vc = V + (int64_t)c * ld;
@@ -539,7 +641,12 @@ struct KrylovDev {
   cplx* coefB;
   double* scal;
   cplx* red;
+  double* prm;  // partial reorthogonalisation state (Pro)
+  double* om;
+  int* lo;
 };
+
+constexpr int PRO_W = 2;  // local window of the partial scheme
 
 static KrylovDev kr_layout(qd_krylov* kr) {
   KrylovDev d;
@@ -564,6 +671,12 @@ static KrylovDev kr_layout(qd_krylov* kr) {
   p += sizeof(cplx) * (2 * K + 8);
   d.scal = (double*)p;
   p += sizeof(double) * 8;
+  d.prm = (double*)p;
+  p += sizeof(double) * 8;
+  d.om = (double*)p;
+  p += sizeof(double) * 3 * (size_t)(K + 2);
+  d.lo = (int*)p;
+  p += sizeof(int) * (size_t)(K + 4);
   d.npart = (double*)p;
   int64_t nb = kr_nb(kr, 1);
   p += sizeof(double) * (size_t)(nb + 1);
@@ -576,6 +689,7 @@ static size_t kr_scal_bytes(int64_t n, int K) {
   int64_t nb = (n + KT - 1) / KT;  // rpt = 1 is the largest block count
   return 64 + sizeof(cplx) * (size_t)(2 * (K + 2)) + sizeof(cplx) * (size_t)(K + 2) * (K + 1) +
          sizeof(cplx) * (size_t)(2 * (K + 2) + 2 * K + 8) + sizeof(double) * 8 + sizeof(double) * (size_t)(nb + 1) +
+         sizeof(double) * (8 + 3 * (size_t)(K + 2)) + sizeof(int) * (size_t)(K + 4) +
          256 + sizeof(cplx) * (size_t)nb * (2 * K + 4);
 }
 
@@ -620,9 +734,12 @@ extern "C" int qd_krylov_destroy(qd_krylov* kr) {
 }
 
 // Gram-Schmidt variant: 0 = CGS2 (four passes over the basis per step),
-// 1 = DCGS2 (two passes, the default).
+// 1 = DCGS2 (two passes, the default), 2 = DCGS2 over a local window with
+// partial reorthogonalisation (Hermitian only), 3 = local window only.
 extern "C" int qd_krylov_set_method(qd_krylov* kr, int method) {
-  if (!kr || (method != 0 && method != 1)) return QD_E_ARG;
+  if (!kr || method < 0 || method > 3) return QD_E_ARG;
+  if (method >= 2 && !kr->herm)
+    return qd_set_error(kr->ctx, QD_E_ARG, "qd_krylov_set_method: windowed schemes need a Hermitian operator");
   kr->method = method;
   return QD_OK;
 }
@@ -644,7 +761,15 @@ static int kr_init(qd_krylov* kr, const cplx* psi) {
   norm_partial_kernel<<<nb, KT, 0, ctx->stream>>>(kr->n, psi, d.npart);
   sum_double_kernel<<<1, KT, 0, ctx->stream>>>(d.npart, nb, d.nrm2);
   normalise_copy_kernel<<<qd_grid(kr->n, 256), 256, 0, ctx->stream>>>(kr->n, psi, kr->d_V, d.nrm2);
+  // partial scheme: eps1 = sqrt(n) eps / 2 models the rounding of one step;
+  // trigger at a quarter of sqrt(eps / kry_dim) (semi-orthogonality), one
+  // step ahead of the full passes it schedules
+  const double eps = 2.220446049250313e-16;
+  const double trigger = kr->method == 3 ? INFINITY : 0.25 * std::sqrt(eps / kr->kry_dim);
+  pro_init_kernel<<<1, KT, 0, ctx->stream>>>(d.lo, kr->kry_dim + 3, d.om, d.prm, 0.5 * std::sqrt((double)kr->n) * eps,
+                                             trigger);
   QD_CHECK_LAUNCH(ctx);
+  kr->full_steps = 0;
   kr->act_dim = 0;
   kr->breakdown = 0;
   std::fill(kr->KR.begin(), kr->KR.end(), cmake(0.0, 0.0));
@@ -680,27 +805,38 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
   const int ldk = kr->kry_dim + 2;
   cplx* U = kr->d_w;
   cplx* W = kr->d_w + kr->ld;
+  const bool windowed = kr->method >= 2;
+  const Pro pro = windowed ? Pro{d.lo, d.om, d.prm, kr->kry_dim + 2, PRO_W} : Pro{nullptr, nullptr, nullptr, 0, 0};
+  auto win = [&](int j) { return windowed ? Win{d.lo, j, PRO_W} : Win{nullptr, 0, 0}; };
   // bootstrap: u_{s+1} = A q_s - Q_s (Q_s^H A q_s); H[0:s+1, s] = first pass
   int rc = qd_linop_apply(op, kr->d_V + (size_t)s * kr->ld, U, nullptr, 0, nullptr, cmake(0.0, 0.0));
   if (rc) return rc;
-  gemv_t_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, U, d.part, d.flag);
-  reduce_cols_kernel<<<s + 1, KT, 0, st>>>(d.part, nb, s + 1, d.h, d.KR + (size_t)s * ldk, 0, d.flag);
-  gemv_n_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, d.h, U, nullptr, d.flag);
+  const Win w0 = win(s + 1);
+  gemv_t_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, U, d.part, d.flag, w0);
+  reduce_cols_kernel<<<s + 1, KT, 0, st>>>(d.part, nb, s + 1, d.h, d.KR + (size_t)s * ldk, 0, d.flag, 1 << 30, w0,
+                                           s + 1, 1);
+  gemv_n_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, d.h, U, nullptr, d.flag, w0);
   for (int j = s + 1; j < s + add; ++j) {
     rc = qd_linop_apply(op, U, W, nullptr, 0, nullptr, cmake(0.0, 0.0));
     if (rc) return rc;
     const int P = 2 * j + 2;
-    dc_red_kernel<RPT, 2><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.part, d.flag);
-    reduce_cols_kernel<<<P, KT, 0, st>>>(d.part, nb, P, d.red, nullptr, 0, d.flag);
-    dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 0, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS);
-    dc_update_kernel<RPT, false><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.coefA, d.coefB, d.scal, d.flag);
+    const Win wj = win(j);
+    dc_red_kernel<RPT, 2><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.part, d.flag, wj);
+    reduce_cols_kernel<<<P, KT, 0, st>>>(d.part, nb, P, d.red, nullptr, 0, d.flag, 1 << 30, wj, j, 2);
+    dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 0, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS,
+                                      pro);
+    dc_update_kernel<RPT, false><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.coefA, d.coefB, d.scal, d.flag,
+                                                    wj);
   }
   // finalise: reorthogonalise the pending vector into V[:, s + add]
   const int j = s + add;
-  dc_red_kernel<RPT, 1><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.part, d.flag);
-  reduce_cols_kernel<<<j + 2, KT, 0, st>>>(d.part, nb, j + 2, d.red, nullptr, 0, d.flag);
-  dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 1, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS);
-  dc_update_kernel<RPT, true><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.coefA, d.coefB, d.scal, d.flag);
+  const Win wj = win(j);
+  dc_red_kernel<RPT, 1><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.part, d.flag, wj);
+  reduce_cols_kernel<<<j + 2, KT, 0, st>>>(d.part, nb, j + 2, d.red, nullptr, 0, d.flag, 1 << 30, wj, j, 1);
+  dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 1, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS,
+                                    pro);
+  dc_update_kernel<RPT, true><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.coefA, d.coefB, d.scal, d.flag,
+                                                 wj);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }
@@ -720,10 +856,14 @@ static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add) {
   const int K = kr->kry_dim, ldk = K + 2;
   std::vector<cplx> cols((size_t)add * ldk);
   int hflag[4] = {0, 0, 0, 0};  // flag at +0, breakdown step at +8
+  double prm[4] = {0.0, 0.0, 0.0, 0.0};
   QD_HIP(ctx, hipMemcpyAsync(hflag, d.flag, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   QD_HIP(ctx, hipMemcpyAsync(cols.data(), d.KR + (size_t)s * ldk, sizeof(cplx) * cols.size(), hipMemcpyDeviceToHost,
                              ctx->stream));
+  if (kr->method >= 2)
+    QD_HIP(ctx, hipMemcpyAsync(prm, d.prm, sizeof(prm), hipMemcpyDeviceToHost, ctx->stream));
   QD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  kr->full_steps = (int)prm[3];
   int end = s + add;
   if (hflag[0]) {
     end = hflag[2];  // columns 0..end-1 span an invariant subspace
@@ -982,6 +1122,7 @@ extern "C" int qd_krylov_get_info(qd_krylov* kr, qd_krylov_info* info) {
   info->n_spmv = kr->n_spmv;
   info->ritz_resid = kr->ritz_resid;
   info->ritz_gap = kr->ritz_gap;
+  info->full_steps = kr->full_steps;
   return QD_OK;
 }
 

@@ -268,7 +268,8 @@ struct qd_krylov {
   double ritz_gap = 0.0;    // second minus lowest Ritz value
   int breakdown = 0;
   int64_t n_spmv = 0;
-  int method = 1;  // 0 = CGS2, 1 = DCGS2
+  int method = 1;  // 0 = CGS2, 1 = DCGS2, 2 = windowed + partial, 3 = windowed
+  int full_steps = 0;  // steps of the partial scheme that covered the whole basis
 };
 
 // --------------------------------------------------------------- errors

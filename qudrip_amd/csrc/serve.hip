@@ -205,6 +205,9 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result
   if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
   if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
   if ((rc = qd_krylov_create(ctx, n, r.k, 1, &kr))) return done(rc);
+  // local window + partial reorthogonalisation: the tridiagonal matches a
+  // fully reorthogonalised basis to O(eps |H|) at a fraction of the passes
+  if ((rc = qd_krylov_set_method(kr, 2))) return done(rc);
   double E = 0.0, Eprev = 0.0;
   bool have_prev = false;
   qd_krylov_info info{};

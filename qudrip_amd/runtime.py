@@ -229,7 +229,8 @@ class Krylov:
         inf = abi.qd_krylov_info()
         self.ctx.check(self.ctx.lib.qd_krylov_get_info(self.ptr, ctypes.byref(inf)), "info")
         return {"act_dim": inf.act_dim, "breakdown": inf.breakdown, "error": inf.error,
-                "beta_last": inf.beta_last, "n_spmv": inf.n_spmv}
+                "beta_last": inf.beta_last, "n_spmv": inf.n_spmv, "ritz_resid": inf.ritz_resid,
+                "ritz_gap": inf.ritz_gap, "full_steps": inf.full_steps}
 
     def eigenvalues(self, cap=4096):
         buf = (ctypes.c_double * cap)()
