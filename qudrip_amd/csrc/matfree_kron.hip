@@ -25,6 +25,9 @@
 //                     on one XCD so the slab (n_B * W * 16 B) stays in its L2.
 // HBM traffic ~ X twice + Y three times (vs. 32 n algorithmic).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "qd_internal.hpp"
 
@@ -81,6 +84,8 @@ struct qd_kron {
   DevOp* d_dops = nullptr;
   int ndterms = 0;
   KronDesc desc{};
+  int rows_variant = -1;  // chosen by timing at the first product (-1: not yet)
+  int slab_variant = -1;
 };
 
 __device__ inline cplx diag_value(const KronDesc& k, uint64_t s) {
@@ -256,6 +261,128 @@ __global__ void __launch_bounds__(256)
         const cplx v = vt[e[u] & 0xFFFFu];
         sx += v.x * xv[u].x + v.y * xv[u].y;
         sy += v.x * xv[u].y - v.y * xv[u].x;
+      }
+    }
+    if (col_ok) {
+      cplx o = y[b * k.nA + a];
+      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+    }
+  }
+}
+
+// Rows kernel with the loads kept in flight: the RB rows (contiguous in x)
+// are staged NU loads per thread at a time, and the packed T_A entries of an
+// output are fetched 8 at a time before their products (same summation order
+// as kron_rows_packed_kernel: entries in order, pad entries add 0).
+template <int RB, int NU>
+__global__ void __launch_bounds__(1024)
+    kron_rows2_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift) {
+  extern __shared__ cplx lds[];
+  cplx* vt = lds;            // nvtab entries
+  cplx* xr = lds + k.nvtab;  // RB rows of nA
+  const int64_t b0 = (int64_t)blockIdx.x * RB;
+  const int rb = (int)min<int64_t>(RB, k.nB - b0);
+  for (int i = threadIdx.x; i < k.nvtab; i += blockDim.x) vt[i] = k.vtab[i];
+  const int64_t cnt = (int64_t)rb * k.nA;
+  const cplx* __restrict__ xs = x + b0 * k.nA;
+  for (int64_t i0 = threadIdx.x; i0 < cnt; i0 += (int64_t)blockDim.x * NU) {
+    cplx v[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int64_t i = i0 + (int64_t)u * blockDim.x;
+      v[u] = xs[min(i, cnt - 1)];  // clamped: every load in bounds, no stack array
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int64_t i = i0 + (int64_t)u * blockDim.x;
+      if (i < cnt) xr[i] = v[u];
+    }
+  }
+  __syncthreads();
+  uint64_t sB[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) sB[r] = (r < rb) ? (k.statesB[b0 + r] << k.loB) : 0;
+  for (int64_t a = threadIdx.x; a < k.nA; a += blockDim.x) {
+    double sx[RB], sy[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) sx[r] = sy[r] = 0.0;
+    for (int j0 = 0; j0 < k.ell_w; j0 += 8) {
+      uint32_t e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) e[u] = (j0 + u < k.ell_w) ? k.ell[(int64_t)(j0 + u) * k.nA + a] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if ((e[u] & 0xFFFFu) == 0) continue;  // pad entry (value 0): no gather
+        const cplx v = vt[e[u] & 0xFFFFu];
+        const int c = (int)(e[u] >> 16);
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          const cplx xv = xr[r * k.nA + c];
+          sx[r] += v.x * xv.x + v.y * xv.y;  // conj(v) * x
+          sy[r] += v.x * xv.y - v.y * xv.x;
+        }
+      }
+    }
+    const uint64_t sA = k.statesA[a];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (r < rb) {
+        cplx d = cadd(diag_value(k, sB[r] | sA), shift);
+        cplx dx = cmul(d, xr[r * k.nA + a]);
+        y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
+      }
+    }
+  }
+}
+
+// Slab kernel with the block's rows of packed T_B staged in LDS (one
+// coalesced read) and U gathers of X row segments in flight per step; same
+// summation order as kron_slab_packed_kernel.
+template <int W, int U>
+__global__ void __launch_bounds__(256)
+    kron_slab2_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, int chunks,
+                      int rows_per_chunk) {
+  extern __shared__ uint32_t se[];  // rows_per_chunk * ellB_w entries
+  __shared__ cplx vt[256];
+  const int xcd = blockIdx.x & 7;
+  const int kk = blockIdx.x >> 3;
+  const int64_t slab = (int64_t)xcd + 8 * (kk / chunks);
+  const int chunk = kk % chunks;
+  const int64_t a0 = slab * W;
+  const int64_t rb = (int64_t)chunk * rows_per_chunk;
+  const int64_t re = min<int64_t>(k.nB, rb + rows_per_chunk);
+  if (a0 >= k.nA || rb >= re) return;  // uniform over the block
+  const int w = k.ellB_w;
+  for (int i = threadIdx.x; i < k.nvtabB; i += blockDim.x) vt[i] = k.vtabB[i];
+  const int64_t ne = (re - rb) * w;
+  const uint32_t* __restrict__ src = k.ellB + rb * w;
+  for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) se[i] = src[i];
+  __syncthreads();
+  const int lane = threadIdx.x % W;
+  const int sub = threadIdx.x / W;
+  const int nsub = blockDim.x / W;
+  const int64_t a = min<int64_t>(a0 + lane, k.nA - 1);  // clamp: duplicate lanes recompute the last column
+  const bool col_ok = a0 + lane < k.nA;
+  for (int64_t b = rb + sub; b < re; b += nsub) {
+    const uint32_t* er = se + (b - rb) * w;
+    int len = w;  // pad entries (value 0) close the row
+    while (len > 0 && (er[len - 1] & 0xFFFFu) == 0) --len;
+    double sx = 0.0, sy = 0.0;
+    for (int j = 0; j < len; j += U) {
+      uint32_t e[U];
+      cplx xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) e[u] = (j + u < len) ? er[j + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < len) xv[u] = x[(int64_t)(e[u] >> 16) * k.nA + a];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j + u < len) {
+          const cplx v = vt[e[u] & 0xFFFFu];
+          sx += v.x * xv[u].x + v.y * xv[u].y;
+          sy += v.x * xv[u].y - v.y * xv[u].x;
+        }
       }
     }
     if (col_ok) {
@@ -570,40 +697,181 @@ void qd_kron_destroy(qd_kron* k) {
   delete k;
 }
 
-int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
-  const KronDesc& d = k->desc;
-  if (d.nA == 0 || d.nB == 0) return QD_OK;
-  size_t lds = sizeof(cplx) * (size_t)d.nA;
+// launch shapes of the two passes
+namespace {
+constexpr int KRON_SLAB_W = 32;
+
+struct SlabShape {
+  int chunks, rows_per_chunk;
+  unsigned grid;
+};
+
+SlabShape slab_shape(const KronDesc& d, int W) {
+  SlabShape s;
+  const int64_t nslabs = (d.nA + W - 1) / W;
+  s.chunks = (int)std::min<int64_t>(128, std::max<int64_t>(1, d.nB / 8));
+  s.rows_per_chunk = (int)((d.nB + s.chunks - 1) / s.chunks);
+  const int64_t slab_groups = (nslabs + 7) / 8;  // slabs per XCD
+  s.grid = (unsigned)(slab_groups * s.chunks * 8);
+  return s;
+}
+
+int kron_optin(int device) {
   // opt-in LDS limit, and the kernels' attributes raised to it, once per
   // process (thread-safe static initialisation; one device per process)
-  static const int optin = [dev = ctx->device]() {
+  static const int optin = [device]() {
     int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) v = 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, device) != hipSuccess) v = 65536;
     if (v > 65536) {
       hipFuncSetAttribute((const void*)kron_rows_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
       hipFuncSetAttribute((const void*)kron_rows_packed_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+      hipFuncSetAttribute((const void*)kron_rows2_kernel<1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+      hipFuncSetAttribute((const void*)kron_rows2_kernel<2, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+      hipFuncSetAttribute((const void*)kron_rows2_kernel<4, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     }
     return v;
   }();
-  size_t lds2 = sizeof(cplx) * ((size_t)d.nvtab + 2 * (size_t)d.nA);
-  if (d.ell && lds2 <= (size_t)optin) {
-    kron_rows_packed_kernel<2><<<(unsigned)((d.nB + 1) / 2), 1024, lds2, ctx->stream>>>(d, x, y, shift);
-  } else if (lds <= (size_t)optin) {
-    kron_rows_kernel<true><<<(unsigned)d.nB, 1024, lds, ctx->stream>>>(d, x, y, shift);
-  } else {
-    kron_rows_kernel<false><<<(unsigned)d.nB, 1024, 0, ctx->stream>>>(d, x, y, shift);
+  return optin;
+}
+
+// rows pass (writes y): variant 0 = kron_rows_packed<2>, 1 = rows2<2,8>,
+// 2 = rows2<1,8>, 3 = rows2<4,4>; a variant whose LDS does not fit returns
+// false.  Without the packed table: kron_rows_kernel.
+bool launch_rows(const KronDesc& d, int v, int optin, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const size_t lds1 = sizeof(cplx) * (size_t)d.nA;
+  if (!d.ell) {
+    if (v != 0) return false;
+    if (lds1 <= (size_t)optin)
+      kron_rows_kernel<true><<<(unsigned)d.nB, 1024, lds1, st>>>(d, x, y, shift);
+    else
+      kron_rows_kernel<false><<<(unsigned)d.nB, 1024, 0, st>>>(d, x, y, shift);
+    return true;
+  }
+  const int rb = v == 2 ? 1 : v == 3 ? 4 : 2;
+  const size_t lds = sizeof(cplx) * ((size_t)d.nvtab + (size_t)rb * (size_t)d.nA);
+  if (lds > (size_t)optin) {
+    if (v != 0) return false;
+    if (lds1 <= (size_t)optin)
+      kron_rows_kernel<true><<<(unsigned)d.nB, 1024, lds1, st>>>(d, x, y, shift);
+    else
+      kron_rows_kernel<false><<<(unsigned)d.nB, 1024, 0, st>>>(d, x, y, shift);
+    return true;
+  }
+  const unsigned grid = (unsigned)((d.nB + rb - 1) / rb);
+  switch (v) {
+    case 0: kron_rows_packed_kernel<2><<<grid, 1024, lds, st>>>(d, x, y, shift); break;
+    case 1: kron_rows2_kernel<2, 8><<<grid, 1024, lds, st>>>(d, x, y, shift); break;
+    case 2: kron_rows2_kernel<1, 8><<<grid, 1024, lds, st>>>(d, x, y, shift); break;
+    default: kron_rows2_kernel<4, 4><<<grid, 1024, lds, st>>>(d, x, y, shift); break;
+  }
+  return true;
+}
+
+// slab pass (adds into y): variant 0 = kron_slab_packed<32,4>, 1 =
+// slab2<32,8>, 2 = slab2<32,16>.  Without the packed table: kron_slab_kernel.
+bool launch_slab(const KronDesc& d, int v, const cplx* x, cplx* y, hipStream_t st) {
+  const SlabShape sh = slab_shape(d, KRON_SLAB_W);
+  if (!d.ellB) {
+    if (v != 0) return false;
+    kron_slab_kernel<KRON_SLAB_W><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk);
+    return true;
+  }
+  const size_t lds = sizeof(uint32_t) * (size_t)sh.rows_per_chunk * (size_t)d.ellB_w;
+  if (v != 0 && lds > 32768) return false;
+  switch (v) {
+    case 0: kron_slab_packed_kernel<KRON_SLAB_W, 4><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+    case 1: kron_slab2_kernel<KRON_SLAB_W, 8><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+    default: kron_slab2_kernel<KRON_SLAB_W, 16><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+  }
+  return true;
+}
+
+// Variant choice per block shape, shared by every product of the process
+// (the first product of a shape times each variant: 3 launches after one
+// warm-up, best average).
+struct KronKey {
+  int64_t nA, nB;
+  int w, wB, nvtab;
+  bool operator<(const KronKey& o) const {
+    return std::tie(nA, nB, w, wB, nvtab) < std::tie(o.nA, o.nB, o.w, o.wB, o.nvtab);
+  }
+};
+std::mutex g_tune_mu;
+std::map<KronKey, std::pair<int, int>> g_tuned;
+
+template <typename F>
+float time_launches(hipStream_t st, F&& f, int reps) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return 1e30f;
+  }
+  f();
+  hipEventRecord(e0, st);
+  for (int r = 0; r < reps; ++r) f();
+  hipEventRecord(e1, st);
+  float ms = 1e30f;
+  if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 1e30f;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return ms;
+}
+
+int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
+  const KronDesc& d = k->desc;
+  const KronKey key{d.nA, d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, d.nvtab};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      k->rows_variant = it->second.first;
+      k->slab_variant = it->second.second;
+      return QD_OK;
+    }
+  }
+  const int optin = kron_optin(ctx->device);
+  hipStream_t st = ctx->stream;
+  int best_r = 0, best_s = 0;
+  float tr = 1e30f, ts = 1e30f;
+  for (int v = 0; v < 4; ++v) {
+    bool ok = true;
+    float ms = time_launches(st, [&] { ok = ok && launch_rows(d, v, optin, x, y, cmake(0.0, 0.0), st); }, 3);
+    if (ok && ms < tr) {
+      tr = ms;
+      best_r = v;
+    }
+  }
+  for (int v = 0; v < 3; ++v) {
+    bool ok = true;
+    float ms = time_launches(st, [&] { ok = ok && launch_slab(d, v, x, y, st); }, 3);
+    if (ok && ms < ts) {
+      ts = ms;
+      best_s = v;
+    }
   }
   QD_CHECK_LAUNCH(ctx);
-  constexpr int W = 32;
-  int64_t nslabs = (d.nA + W - 1) / W;
-  int chunks = (int)std::min<int64_t>(128, std::max<int64_t>(1, d.nB / 8));
-  int rows_per_chunk = (int)((d.nB + chunks - 1) / chunks);
-  int64_t slab_groups = (nslabs + 7) / 8;  // slabs per XCD
-  int64_t grid = slab_groups * chunks * 8;
-  if (d.ellB)
-    kron_slab_packed_kernel<W, 4><<<(unsigned)grid, 256, 0, ctx->stream>>>(d, x, y, chunks, rows_per_chunk);
-  else
-    kron_slab_kernel<W><<<(unsigned)grid, 256, 0, ctx->stream>>>(d, x, y, chunks, rows_per_chunk);
+  k->rows_variant = best_r;
+  k->slab_variant = best_s;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[key] = {best_r, best_s};
+  return QD_OK;
+}
+}  // namespace
+
+int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
+  const KronDesc& d = k->desc;
+  if (d.nA == 0 || d.nB == 0) return QD_OK;
+  if (k->rows_variant < 0) {
+    int rc = kron_tune(ctx, k, x, y);  // leaves y overwritten: the product below rewrites it
+    if (rc) return rc;
+  }
+  const int optin = kron_optin(ctx->device);
+  if (!launch_rows(d, k->rows_variant, optin, x, y, shift, ctx->stream))
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", k->rows_variant);
+  QD_CHECK_LAUNCH(ctx);
+  if (!launch_slab(d, k->slab_variant, x, y, ctx->stream))
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker slab variant %d", k->slab_variant);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }

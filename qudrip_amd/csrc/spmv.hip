@@ -10,7 +10,7 @@
 #include "qd_internal.hpp"
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
-#define QD_STREAM_VARIANTS 9
+#define QD_STREAM_VARIANTS 13
 
 // ---------------------------------------------------------------- K5
 // G lanes cooperate on one row: each lane strides the row's entries by G,
@@ -151,6 +151,109 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Same partition and summation order as spmv_stream_kernel, with the load
+// phase in straight-line code: a block of U x 64 entries that lies wholly in
+// the slice loads unconditionally (one wait for the 2U streams, one for the U
+// gathers); the tail block clamps its indices to the slice instead of
+// branching per load.  Phase 2 reads four products per LDS wait and adds
+// them in entry order; y is written non-temporally.
+template <int CAPW, int U>
+__global__ void __launch_bounds__(256)
+    spmv_stream2_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                        const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
+                        const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
+                        cplx shift) {
+  __shared__ cplx lds[4][CAPW];
+  const unsigned nb = gridDim.x;
+  const unsigned per = nb >> 3;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)lb * 4 + w) * 64;
+  if (r0 >= nrows) return;  // whole wave leaves together
+  const int64_t row = r0 + lane;
+  const int32_t nnz_end = outer[nrows];
+  int32_t rs = (row < nrows) ? outer[row] : nnz_end;
+  int32_t re = __shfl_down(rs, 1, 64);
+  if (lane == 63) re = (r0 + 64 < nrows) ? outer[r0 + 64] : nnz_end;
+  const int32_t wbeg = __shfl(rs, 0, 64);
+  const int32_t wend = __shfl(re, 63, 64);
+  cplx* __restrict__ buf = lds[w];
+  const dbl2* vp = reinterpret_cast<const dbl2*>(vals);
+  double sx = 0.0, sy = 0.0;
+  for (int32_t c0 = wbeg; c0 < wend; c0 += CAPW) {
+    const int32_t cn = min(wend - c0, (int32_t)CAPW);
+    const dbl2* __restrict__ vc = vp + c0;
+    const int32_t* __restrict__ ic = inner + c0;
+    for (int32_t k0 = 0; k0 < cn; k0 += 64 * U) {
+      dbl2 v[U];
+      int32_t c[U];
+      cplx xv[U];
+      if (k0 + 64 * U <= cn) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(ic + k0 + 64 * u + lane);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(vc + k0 + 64 * u + lane);
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+        // every gather in flight before the first product
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          buf[k0 + 64 * u + lane] =
+              cmake(v[u].x * xv[u].x + v[u].y * xv[u].y, v[u].x * xv[u].y - v[u].y * xv[u].x);
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(ic + min(k0 + 64 * u + lane, cn - 1));
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(vc + min(k0 + 64 * u + lane, cn - 1));
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t kk = k0 + 64 * u + lane;
+          if (kk < cn) buf[kk] = cmake(v[u].x * xv[u].x + v[u].y * xv[u].y, v[u].x * xv[u].y - v[u].y * xv[u].x);
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int32_t a = max(rs, c0) - c0, b = min(re, c0 + cn) - c0;
+    int32_t e = a;
+    for (; e + 4 <= b; e += 4) {
+      const cplx p0 = buf[e], p1 = buf[e + 1], p2 = buf[e + 2], p3 = buf[e + 3];
+      sx += p0.x;
+      sy += p0.y;
+      sx += p1.x;
+      sy += p1.y;
+      sx += p2.x;
+      sy += p2.y;
+      sx += p3.x;
+      sy += p3.y;
+    }
+    for (; e < b; ++e) {
+      const cplx p = buf[e];
+      sx += p.x;
+      sy += p.y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (row < nrows) {
+    double s = xscale ? xscale[xscale_idx] : 1.0;
+    cplx r = cmake(sx * s, sy * s);
+    if (diag || shift.x != 0.0 || shift.y != 0.0) {
+      cplx xr = x[row];
+      cplx d = diag ? cadd(diag[row], shift) : shift;
+      r = cadd(r, crmul(s, cmul(d, xr)));
+    }
+    __builtin_nontemporal_store(r.x, &y[row].x);
+    __builtin_nontemporal_store(r.y, &y[row].y);
+  }
+}
+
 // ---------------------------------------------------------------- K6
 // Matrix-free pull product from the adjoint program: expanding column r of
 // A^dagger = sum conj(c_k) P_k^dagger from state r gives leaves (t, w) with
@@ -232,6 +335,16 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
       QD_STREAM_CASE(7, 128, 4, true)
       QD_STREAM_CASE(8, 1024, 16, true)
 #undef QD_STREAM_CASE
+#define QD_STREAM2_CASE(V, CAPW, U)                                                                  \
+  case V:                                                                                           \
+    spmv_stream2_kernel<CAPW, U><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y, \
+                                                                d_xscale, xscale_idx, d_diag, shift);   \
+    break;
+      QD_STREAM2_CASE(9, 512, 8)
+      QD_STREAM2_CASE(10, 256, 4)
+      QD_STREAM2_CASE(11, 512, 4)
+      QD_STREAM2_CASE(12, 1024, 8)
+#undef QD_STREAM2_CASE
       default:
         return qd_set_error(ctx, QD_E_ARG, "stream variant %d", -op->lanes_per_row);
     }
