@@ -358,6 +358,11 @@ int qd_server_submit(qd_server* s, const qd_request* req);
  * of them, on the model's stored matrix; arrivals join at the next step).
  * 1 (default): each request alone on its worker. */
 int qd_server_set_batch(qd_server* s, int slots);
+/* Orthogonalisation scheme of ground-state requests submitted from now on
+ * (qd_krylov_set_method): 2 (default) = local window with partial
+ * reorthogonalisation, 1 = full delayed CGS2 (the scheme of
+ * qd_krylov_ground_state), 0 = CGS2.  No reference counterpart. */
+int qd_server_set_gs_method(qd_server* s, int method);
 /* Moves up to cap finished results (in completion order) into out. */
 int qd_server_poll(qd_server* s, qd_result* out, int cap, int* count);
 /* Blocks until every submitted request has finished. */

@@ -142,6 +142,7 @@ SIGNATURES = {
                                 ctypes.POINTER(I)]),
     "qd_server_submit": (I, [VP, ctypes.POINTER(qd_request)]),
     "qd_server_set_batch": (I, [VP, I]),
+    "qd_server_set_gs_method": (I, [VP, I]),
     "qd_server_poll": (I, [VP, ctypes.POINTER(qd_result), I, ctypes.POINTER(I)]),
     "qd_server_drain": (I, [VP]),
     "qd_server_clock": (ctypes.c_double, [VP]),

@@ -63,6 +63,68 @@ extern "C" int qd_ctx_destroy(qd_ctx* ctx) {
 
 extern "C" const char* qd_last_error(const qd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int qd_dev_alloc(qd_ctx* ctx, size_t bytes, void** p) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  qd_devpool* P = ctx->pool;
+  if (!P) return hipMalloc(p, bytes) == hipSuccess ? QD_OK : QD_E_OOM;
+  std::vector<void*> release;
+  {
+    std::lock_guard<std::mutex> lk(P->mu);
+    auto it = P->free_blocks.lower_bound(bytes);
+    if (it != P->free_blocks.end() && it->first <= bytes + bytes / 8) {
+      *p = it->second;
+      P->cached -= it->first;
+      P->in_use += it->first;
+      P->live[*p] = it->first;
+      P->free_blocks.erase(it);
+      return QD_OK;
+    }
+    // make room under the cap by dropping cached blocks, largest first
+    while (!P->free_blocks.empty() && P->in_use + P->cached + bytes > P->cap) {
+      auto last = std::prev(P->free_blocks.end());
+      P->cached -= last->first;
+      release.push_back(last->second);
+      P->free_blocks.erase(last);
+    }
+  }
+  for (void* q : release) hipFree(q);
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return QD_E_OOM;
+  }
+  std::lock_guard<std::mutex> lk(P->mu);
+  P->in_use += bytes;
+  P->live[*p] = bytes;
+  return QD_OK;
+}
+
+void qd_dev_free(qd_ctx* ctx, void* p) {
+  if (!p) return;
+  qd_devpool* P = ctx->pool;
+  if (P) {
+    std::lock_guard<std::mutex> lk(P->mu);
+    auto it = P->live.find(p);
+    if (it != P->live.end()) {
+      const size_t sz = it->second;
+      P->live.erase(it);
+      P->in_use -= sz;
+      P->free_blocks.emplace(sz, p);
+      P->cached += sz;
+      return;
+    }
+  }
+  hipFree(p);
+}
+
+void qd_devpool_release_all(qd_devpool* P) {
+  if (!P) return;
+  std::lock_guard<std::mutex> lk(P->mu);
+  for (auto& kv : P->free_blocks) hipFree(kv.second);
+  P->free_blocks.clear();
+  P->cached = 0;
+}
+
 extern "C" int qd_ctx_sync(qd_ctx* ctx) {
   if (!ctx) return QD_E_ARG;
   QD_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -705,14 +705,14 @@ extern "C" int qd_krylov_create(qd_ctx* ctx, int64_t n, int kry_dim, int herm, q
   kr->herm = herm ? 1 : 0;
   kr->ld = (n + 31) / 32 * 32;
   size_t vbytes = sizeof(cplx) * (size_t)kr->ld * (size_t)(kry_dim + 1);
-  if (hipMalloc(&kr->d_V, vbytes) != hipSuccess) {
+  if (qd_dev_alloc(ctx, vbytes, (void**)&kr->d_V) != QD_OK) {
     delete kr;
     return qd_set_error(ctx, QD_E_OOM, "Krylov basis: %zu bytes", vbytes);
   }
-  if (hipMalloc(&kr->d_scal, kr_scal_bytes(n, kry_dim)) != hipSuccess ||
-      hipMalloc(&kr->d_w, sizeof(cplx) * 2 * (size_t)kr->ld) != hipSuccess) {
-    hipFree(kr->d_V);
-    hipFree(kr->d_scal);
+  if (qd_dev_alloc(ctx, kr_scal_bytes(n, kry_dim), (void**)&kr->d_scal) != QD_OK ||
+      qd_dev_alloc(ctx, sizeof(cplx) * 2 * (size_t)kr->ld, (void**)&kr->d_w) != QD_OK) {
+    qd_dev_free(ctx, kr->d_V);
+    qd_dev_free(ctx, kr->d_scal);
     delete kr;
     return qd_set_error(ctx, QD_E_OOM, "Krylov scalars");
   }
@@ -726,9 +726,9 @@ extern "C" int qd_krylov_destroy(qd_krylov* kr) {
   if (!kr) return QD_OK;
   hipSetDevice(kr->ctx->device);
   hipStreamSynchronize(kr->ctx->stream);
-  hipFree(kr->d_V);
-  hipFree(kr->d_scal);
-  hipFree(kr->d_w);
+  qd_dev_free(kr->ctx, kr->d_V);
+  qd_dev_free(kr->ctx, kr->d_scal);
+  qd_dev_free(kr->ctx, kr->d_w);
   delete kr;
   return QD_OK;
 }

@@ -6,7 +6,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/qudrip_hip.h"
@@ -175,11 +178,27 @@ __device__ inline bool eval_leaf(const DevOp* __restrict__ ops, int n_ops, int n
 }
 
 // --------------------------------------------------------------- handles
+// Caching pool for large per-solve device buffers (Krylov bases, request
+// vectors).  Blocks freed into it are kept and handed to later requests of a
+// size within 1/8 above theirs instead of going back to the driver: repeated
+// allocation and release of multi-GB buffers fragments device memory, and
+// kernels streaming through fragmented mappings lose bandwidth to address
+// translation.  live + cached bytes stay within `cap` (cached blocks are
+// released first when a new block would pass it).  Callers return a block
+// only after the work that used it has completed on its stream.
+struct qd_devpool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free_blocks;  // size -> block
+  std::unordered_map<void*, size_t> live;    // block -> size
+  size_t cached = 0, in_use = 0, cap = 0;
+};
+
 struct qd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string err;
+  qd_devpool* pool = nullptr;  // not owned; nullptr: plain hipMalloc / hipFree
   // reusable device scratch for reductions
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -282,6 +301,11 @@ int qd_set_error(qd_ctx* ctx, int code, const char* fmt, ...);
                           hipGetErrorString(e_), __FILE__, __LINE__);                       \
   } while (0)
 #define QD_CHECK_LAUNCH(ctx) QD_HIP(ctx, hipGetLastError())
+
+// large per-solve buffers: through ctx->pool when set, else hipMalloc/hipFree
+int qd_dev_alloc(qd_ctx* ctx, size_t bytes, void** p);
+void qd_dev_free(qd_ctx* ctx, void* p);
+void qd_devpool_release_all(qd_devpool* pool);
 
 // scratch growth (device), returns QD_OK or error
 int qd_ctx_scratch(qd_ctx* ctx, size_t bytes, void** p);

@@ -133,6 +133,7 @@ struct Job {
   double t_submit = 0.0;
   size_t footprint = 0;
   int bypassed = 0;
+  int gs_method = 2;  // Krylov scheme of a ground state, fixed at submission
 };
 
 }  // namespace
@@ -152,6 +153,8 @@ struct qd_server {
   size_t peak_used = 0;
   bool stopping = false;
   int batch_slots = 1;             // evolve requests of one model stepped together (1: no batching)
+  int gs_method = 2;               // qd_krylov_set_method scheme of ground-state requests
+  qd_devpool pool;                 // request buffers (cap: the HBM budget)
   std::vector<char> batch_active;  // per model: a batch owner admits its evolve requests
   std::vector<std::thread> threads;
   std::chrono::steady_clock::time_point t0;
@@ -160,6 +163,35 @@ struct qd_server {
 };
 
 namespace {
+
+// Device contexts (one stream each) of servers live as long as the process
+// and pass from a destroyed server to the next one: a process has a fixed
+// number of hardware queues, and streams created after others were destroyed
+// can land on a shared queue, which serialises workers meant to overlap.
+std::mutex g_ctx_pool_mu;
+std::vector<qd_ctx*> g_ctx_pool;
+
+int ctx_acquire(int device, qd_ctx** out) {
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_pool_mu);
+    for (size_t i = 0; i < g_ctx_pool.size(); ++i) {
+      if (g_ctx_pool[i]->device == device) {
+        *out = g_ctx_pool[i];
+        g_ctx_pool.erase(g_ctx_pool.begin() + (std::ptrdiff_t)i);
+        return QD_OK;
+      }
+    }
+  }
+  return qd_ctx_create(device, out);
+}
+
+void ctx_release(qd_ctx* c) {
+  if (!c) return;
+  hipStreamSynchronize(c->stream);
+  c->pool = nullptr;
+  std::lock_guard<std::mutex> lk(g_ctx_pool_mu);
+  g_ctx_pool.push_back(c);
+}
 
 size_t request_footprint(const Model& M, const qd_request& r) {
   const size_t vec = sizeof(cplx) * (size_t)M.n;
@@ -185,7 +217,7 @@ int diag_expect(qd_ctx* ctx, int64_t n, const cplx* d, const cplx* psi, double* 
   return QD_OK;
 }
 
-int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
+int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int method, qd_result* out) {
   const int64_t n = M.n;
   cplx *diag = nullptr, *psi = nullptr;
   qd_linop* op = nullptr;
@@ -195,19 +227,21 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result
     if (kr) qd_krylov_destroy(kr);
     if (op) qd_linop_destroy(op);
     hipStreamSynchronize(ctx->stream);
-    if (diag) hipFree(diag);
-    if (psi) hipFree(psi);
+    qd_dev_free(ctx, diag);
+    qd_dev_free(ctx, psi);
     return code;
   };
-  if (hipMalloc(&diag, sizeof(cplx) * n) != hipSuccess || hipMalloc(&psi, sizeof(cplx) * n) != hipSuccess)
+  if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&diag) != QD_OK ||
+      qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&psi) != QD_OK)
     return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
   scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
   if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
   if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
   if ((rc = qd_krylov_create(ctx, n, r.k, 1, &kr))) return done(rc);
-  // local window + partial reorthogonalisation: the tridiagonal matches a
-  // fully reorthogonalised basis to O(eps |H|) at a fraction of the passes
-  if ((rc = qd_krylov_set_method(kr, 2))) return done(rc);
+  // 2 (default): local window + partial reorthogonalisation, whose tridiagonal
+  // matches a fully reorthogonalised basis to O(eps |H|) at a fraction of the
+  // passes; 1: full delayed CGS2, the drop-in path's scheme
+  if ((rc = qd_krylov_set_method(kr, method))) return done(rc);
   double E = 0.0, Eprev = 0.0;
   bool have_prev = false;
   qd_krylov_info info{};
@@ -252,12 +286,13 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
     if (op) qd_linop_destroy(op);
     if (prog) qd_termprog_destroy(prog);
     hipStreamSynchronize(ctx->stream);
-    if (diag) hipFree(diag);
-    if (psi) hipFree(psi);
-    if (nxt) hipFree(nxt);
+    qd_dev_free(ctx, diag);
+    qd_dev_free(ctx, psi);
+    qd_dev_free(ctx, nxt);
     return code;
   };
-  if (hipMalloc(&psi, sizeof(cplx) * n) != hipSuccess || hipMalloc(&nxt, sizeof(cplx) * n) != hipSuccess)
+  if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&psi) != QD_OK ||
+      qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&nxt) != QD_OK)
     return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
   if (M.structured) {
     // H = A + c D as one program: the matrix-free product recognises the
@@ -275,7 +310,8 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
       return done(rc);
     if ((rc = qd_linop_matfree(ctx, M.basis, prog, &op))) return done(rc);
   } else {
-    if (hipMalloc(&diag, sizeof(cplx) * n) != hipSuccess) return done(qd_set_error(ctx, QD_E_OOM, "request diagonal"));
+    if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&diag) != QD_OK)
+      return done(qd_set_error(ctx, QD_E_OOM, "request diagonal"));
     scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
     if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
   }
@@ -610,7 +646,8 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
 
 void worker_main(qd_server* S, int wi) {
   qd_ctx* ctx = nullptr;
-  int crc = qd_ctx_create(S->device, &ctx);
+  int crc = ctx_acquire(S->device, &ctx);
+  if (ctx) ctx->pool = &S->pool;
   for (;;) {
     Job job;
     {
@@ -619,7 +656,7 @@ void worker_main(qd_server* S, int wi) {
         if (crc == QD_OK && pick(S, &job)) break;
         if (S->stopping && S->queue.empty()) {
           lk.unlock();
-          if (ctx) qd_ctx_destroy(ctx);
+          ctx_release(ctx);
           return;
         }
         if (crc != QD_OK && !S->queue.empty()) {
@@ -652,7 +689,7 @@ void worker_main(qd_server* S, int wi) {
     res.t_start = S->now();
     res.worker = wi;
     const Model& M = *job.model;
-    res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(ctx, M, job.req, &res)
+    res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(ctx, M, job.req, job.gs_method, &res)
                                                      : run_evolve(ctx, M, job.req, &res);
     res.t_end = S->now();
     {
@@ -674,8 +711,9 @@ extern "C" int qd_server_create(int device, size_t hbm_budget, int workers, qd_s
   qd_server* S = new qd_server();
   S->device = device;
   S->budget = hbm_budget;
+  S->pool.cap = hbm_budget;
   S->t0 = std::chrono::steady_clock::now();
-  int rc = qd_ctx_create(device, &S->setup);
+  int rc = ctx_acquire(device, &S->setup);
   if (rc) {
     delete S;
     return rc;
@@ -776,6 +814,7 @@ extern "C" int qd_server_submit(qd_server* S, const qd_request* r) {
   j.model = S->models[r->model];
   j.t_submit = S->now();
   j.footprint = request_footprint(*j.model, *r);
+  j.gs_method = S->gs_method;
   S->queue.push_back(j);
   ++S->submitted;
   S->cv.notify_all();
@@ -803,6 +842,13 @@ extern "C" int qd_server_set_batch(qd_server* S, int slots) {
   if (!S || slots < 1 || slots > 4096) return QD_E_ARG;
   std::lock_guard<std::mutex> lk(S->mu);
   S->batch_slots = slots;
+  return QD_OK;
+}
+
+extern "C" int qd_server_set_gs_method(qd_server* S, int method) {
+  if (!S || method < 0 || method > 2) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->gs_method = method;
   return QD_OK;
 }
 
@@ -839,7 +885,8 @@ extern "C" int qd_server_destroy(qd_server* S) {
     if (M->d_diagR) hipFree(M->d_diagR);
     delete M;
   }
-  if (S->setup) qd_ctx_destroy(S->setup);
+  ctx_release(S->setup);
+  qd_devpool_release_all(&S->pool);
   delete S;
   return QD_OK;
 }

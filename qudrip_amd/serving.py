@@ -23,7 +23,7 @@ from . import operators as Q
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget=200 << 30, workers=4, batch=1):
+    def __init__(self, device=0, hbm_budget=200 << 30, workers=4, batch=1, gs_method=2):
         self.lib = abi.load()
         p = ctypes.c_void_p()
         rc = self.lib.qd_server_create(device, hbm_budget, workers, ctypes.byref(p))
@@ -31,6 +31,7 @@ class Server:
             raise abi.QDError(rc, "qd_server_create")
         self.ptr = p
         self._check(self.lib.qd_server_set_batch(self.ptr, batch), "set_batch")
+        self._check(self.lib.qd_server_set_gs_method(self.ptr, gs_method), "set_gs_method")
 
     def _check(self, rc, what):
         if rc:
@@ -111,11 +112,11 @@ def hubbard_lattice(Lx=3, Ly=4, nup=6, ndn=6):
 
 
 def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=10, gs_tol=1e-12,
-               rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, log=None):
+               rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, gs_method=2, log=None):
     """Registers the models (timed apart: the structure cache is built once at
     start-up), then submits every request at its arrival time and waits."""
     nsite, bonds, nup, ndn = lattice or hubbard_lattice()
-    srv = Server(hbm_budget=hbm_budget, workers=workers, batch=batch)
+    srv = Server(hbm_budget=hbm_budget, workers=workers, batch=batch, gs_method=gs_method)
     try:
         t0 = time.perf_counter()
         models = {}

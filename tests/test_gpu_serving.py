@@ -32,10 +32,12 @@ def _dense_e0(N, jz):
     return np.linalg.eigvalsh(P.restrict(H, st))[0]
 
 
-def test_served_matches_plain_and_exact(ctx):
+@pytest.mark.parametrize("gs_method", [2, 1])
+def test_served_matches_plain_and_exact(ctx, gs_method):
+    """Ground states with partial (2, default) or full (1) reorthogonalisation."""
     reqs = _reqs()
-    print("serving mix of", len(reqs), flush=True)
-    served = SV.run_served(reqs, lattice=LATTICE, workers=4, hbm_budget=8 << 30, **KW)
+    print("serving mix of", len(reqs), "gs_method", gs_method, flush=True)
+    served = SV.run_served(reqs, lattice=LATTICE, workers=4, hbm_budget=8 << 30, gs_method=gs_method, **KW)
     plain = SV.run_plain(reqs, ctx, lattice=LATTICE, gs_k=KW["gs_k"], rt_k=KW["rt_k"], rt_steps=KW["rt_steps"],
                          dt=KW["dt"])
     for r in reqs:
