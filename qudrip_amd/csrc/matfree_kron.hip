@@ -25,6 +25,8 @@
 //                     on one XCD so the slab (n_B * W * 16 B) stays in its L2.
 // HBM traffic ~ X twice + Y three times (vs. 32 n algorithmic).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -837,6 +839,7 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   for (int v = 0; v < 4; ++v) {
     bool ok = true;
     float ms = time_launches(st, [&] { ok = ok && launch_rows(d, v, optin, x, y, cmake(0.0, 0.0), st); }, 3);
+    if (ok && std::getenv("QD_TUNE_LOG")) std::fprintf(stderr, "[qd]   rows variant %d: %.1f us\n", v, 1000.0 * ms / 3);
     if (ok && ms < tr) {
       tr = ms;
       best_r = v;
@@ -845,12 +848,17 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   for (int v = 0; v < 3; ++v) {
     bool ok = true;
     float ms = time_launches(st, [&] { ok = ok && launch_slab(d, v, x, y, st); }, 3);
+    if (ok && std::getenv("QD_TUNE_LOG")) std::fprintf(stderr, "[qd]   slab variant %d: %.1f us\n", v, 1000.0 * ms / 3);
     if (ok && ms < ts) {
       ts = ms;
       best_s = v;
     }
   }
   QD_CHECK_LAUNCH(ctx);
+  if (std::getenv("QD_TUNE_LOG"))
+    std::fprintf(stderr, "[qd] Kronecker nA=%lld nB=%lld w=%d wB=%d: rows variant %d (%.1f us), slab variant %d (%.1f us)\n",
+                 (long long)d.nA, (long long)d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, best_r,
+                 1000.0 * tr / 3, best_s, 1000.0 * ts / 3);
   k->rows_variant = best_r;
   k->slab_variant = best_s;
   std::lock_guard<std::mutex> lk(g_tune_mu);
