@@ -310,7 +310,10 @@ int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nterms, const 
  * requests may pass a waiting one a bounded number of times; a request that
  * can never fit is answered with QD_E_OOM).  A request's Krylov state stays on
  * the device until it finishes; its result does not depend on what else
- * runs. */
+ * runs.  Request buffers (Krylov bases, vectors) come from a caching pool of
+ * the server, capped at hbm_budget and released by qd_server_destroy; the
+ * workers' device contexts (one stream each) outlive the server and pass to
+ * the next server created in the process. */
 typedef struct qd_server qd_server;
 enum { QD_REQ_GROUND_STATE = 0, QD_REQ_EVOLVE = 1 };
 typedef struct {
