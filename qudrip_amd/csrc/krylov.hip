@@ -8,11 +8,14 @@
 //     h2     = V_{0..j}^H w ;  w -= V h2       (second pass: CGS2)
 //     beta   = |w| ;  v_{j+1} = w / beta
 // and column j of the Hessenberg matrix is h1 + h2 (rows 0..j) and beta
-// (row j+1).  The reference's Lanczos (expandBasis<1>, :395-425) keeps only
-// the last two vectors; full reorthogonalisation changes no exact-arithmetic
-// quantity (same Krylov space, same tridiagonal) and keeps the Ritz values
-// free of ghost copies.  Every reduction has a fixed tree (block partials
-// summed in index order), so a solve gives the same bits whatever else runs.
+// (row j+1).  That is method 0; the default (method 1) is the delayed form
+// below, two passes per step instead of four, and Hermitian operators can
+// use method 2, a local window with partial reorthogonalisation.  The
+// reference's Lanczos (expandBasis<1>, :395-425) keeps only the last two
+// vectors; full reorthogonalisation changes no exact-arithmetic quantity
+// (same Krylov space, same tridiagonal) and keeps the Ritz values free of
+// ghost copies.  Every reduction has a fixed tree (block partials summed in
+// index order), so a solve gives the same bits whatever else runs.
 //
 // Breakdown (beta < 1e-10, :387-390, :421) sets a device flag: the remaining
 // kernels of the expansion see it and return, so the whole expansion is
