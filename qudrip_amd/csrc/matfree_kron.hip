@@ -31,43 +31,10 @@
 #include <mutex>
 #include <tuple>
 
-#include "qd_internal.hpp"
+#include "kron.hpp"
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-struct KronDesc {
-  int64_t nA, nB;
-  int loB;  // first bit of block B
-  // T_A / T_B in pull form: row r = conj of column r of the Hermitian block
-  const int32_t* Aouter;
-  const int32_t* Ainner;
-  const cplx* Avals;
-  const int32_t* Bouter;
-  const int32_t* Binner;
-  const cplx* Bvals;
-  const uint64_t* statesA;
-  const uint64_t* statesB;
-  const DevTerm* dterms;  // diagonal terms other than number products
-  const DevOp* dops;
-  int ndterms;
-  // number-operator products c * prod n_s (hopBitOp(s, s), hopbit.hpp:50-51):
-  // value c when every bit of the mask is set
-  const uint64_t* nmask;
-  const cplx* ncoef;
-  int nnum;
-  // packed T_A (ELL, entry j of row a at ell[j * nA + a]): column in the high
-  // 16 bits, index into vtab in the low 16 bits; vtab[0] = 0 pads short rows.
-  const uint32_t* ell;
-  int ell_w;
-  const cplx* vtab;
-  int nvtab;
-  // packed T_B, row-major (entry j of row b at ellB[b * ellB_w + j]), same
-  // encoding; vtabB[0] = 0 pads short rows
-  const uint32_t* ellB;
-  int ellB_w;
-  const cplx* vtabB;
-  int nvtabB;
-};
 
 struct qd_kron {
   uint32_t* d_ellB = nullptr;
@@ -679,6 +646,8 @@ qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
   }
   return k;
 }
+
+const KronDesc* qd_kron_desc(const qd_kron* k) { return k ? &k->desc : nullptr; }
 
 void qd_kron_destroy(qd_kron* k) {
   if (!k) return;

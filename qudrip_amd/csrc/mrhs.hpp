@@ -642,25 +642,39 @@ struct MrhsBufs {
   cplx *sc, *coef;      // per-slot scalars; y = expm(...) e_1 [q Mtot + slot]
 };
 
+// The multi-RHS product of mrhs_arnoldi: (st, from_x, Wsrc, f, alpha, beta,
+// Vout, Wout) with g_spmm_kernel's meaning.  This one is g_spmm_kernel on the
+// sectors' stored matrices.
+struct SpmmProduct {
+  const GSec* gs;
+  int ns;
+  int64_t blocks;
+  void operator()(hipStream_t st, int from_x, const cplx* Wsrc, const double* f, const cplx* alpha,
+                  const cplx* beta, cplx* Vout, cplx* Wout) const {
+    const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
+    g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+  }
+};
+
 // K Arnoldi vectors (delayed CGS2) for every slot: V_0 = f X (f = per-slot
 // 1/|x|), H_eff = H + alpha D + beta per slot, then y = expm(-i dt K) e_1.
 // X is not touched; mrhs_combine writes V y into it.
-inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t blocks, int64_t ngroups,
-                         int64_t VS, int Mtot, int K, double dt, const double* f, const cplx* alpha, const cplx* beta,
-                         const MrhsBufs& b) {
-  const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
+template <class Prod>
+inline void mrhs_arnoldi_with(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t ngroups, int64_t VS,
+                              int Mtot, int K, double dt, const double* f, const cplx* alpha, const cplx* beta,
+                              const MrhsBufs& b, const Prod& prod) {
   const unsigned tl = (unsigned)tiles;
   cplx* U = b.U;
   cplx* W = b.W;
   // V_0 = f X, U = H_eff V_0, then its first projection: H[0][0] = V_0^H U, U -= V_0 H[0][0]
-  g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 1, nullptr, f, alpha, beta, b.V, U);
+  prod(st, 1, nullptr, f, alpha, beta, b.V, U);
   g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, U, nullptr, Mtot, b.part);
   g_reduce_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, ngroups, Mtot, 1, b.part, b.h, b.hess, 0, K,
                                                           0, nullptr);
   g_pass_kernel<1, 2><<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, U, b.h, Mtot, b.part);
   for (int j = 1; j < K; ++j) {
     const int P = 2 * j + 2;
-    g_spmm_kernel<<<sgrid, TT, 0, st>>>(d_gsec, ns, blocks, 0, U, nullptr, alpha, beta, nullptr, W);
+    prod(st, 0, U, nullptr, alpha, beta, nullptr, W);
     launch_red<2>(j, tl, st, d_gsec, ns, b.V, VS, U, W, b.part);
     g_reduce_p_kernel<<<(unsigned)(ngroups * P), RW * TT, 0, st>>>(d_gsec, ns, ngroups, Mtot, P, b.part, b.red);
     g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, j, K, 0, b.red, b.hess, b.coef2, b.sc, 1e-10);
@@ -672,6 +686,12 @@ inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t til
                                                                          b.red);
   g_dc_small_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, K, K, 1, b.red, b.hess, b.coef2, b.sc, 1e-10);
   small_expm_kernel<<<(unsigned)Mtot, 64, 0, st>>>(Mtot, K, K, b.hess, dt, b.coef);
+}
+
+inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t blocks, int64_t ngroups,
+                         int64_t VS, int Mtot, int K, double dt, const double* f, const cplx* alpha, const cplx* beta,
+                         const MrhsBufs& b) {
+  mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, f, alpha, beta, b, SpmmProduct{d_gsec, ns, blocks});
 }
 
 inline void mrhs_combine(hipStream_t st, const GSec* d_gsec, int ns, int64_t VS, int Mtot, int K, const MrhsBufs& b) {

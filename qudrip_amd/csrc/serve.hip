@@ -25,7 +25,7 @@
 #include <thread>
 #include <vector>
 
-#include "mrhs.hpp"
+#include "mrhs_kron.hpp"
 
 namespace {
 
@@ -120,6 +120,7 @@ struct Model {
   qd_termprog* progA = nullptr;
   qd_csr* csrA = nullptr;
   qd_linop* opA = nullptr;
+  qd_kron* kronA = nullptr;  // structured models: A in two-block form (lockstep batches)
   cplx* d_diagD = nullptr;
   double* d_diagR = nullptr;  // its real part (the diagonal of a Hermitian D)
   size_t bytes = 0;
@@ -438,10 +439,12 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
   const int Kmax = std::max(1, std::min(first.req.k - 1, MAXK));
   const double dt = first.req.dt;
   hipStream_t st = ctx->stream;
+  const KronDesc* kd = qd_kron_desc(M.kronA);
+  const int cmax = kd ? 64 : S->batch_slots;  // the Kronecker product takes up to 64 slots
   int C = 1;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    while (C < S->batch_slots && S->used + first.footprint <= S->budget) {
+    while (C < S->batch_slots && C < cmax && S->used + first.footprint <= S->budget) {
       S->used += first.footprint;
       ++C;
     }
@@ -602,7 +605,11 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
     inv_sqrt_kernel<<<g1(m, 256), 256, 0, st>>>(m, d_n2, d_scale);
     double maxest = 0.0;
     for (;;) {
-      mrhs_arnoldi(st, d_gsec, 1, tiles, blocks, ngroups, VS, m, Kcur, dt, d_scale, d_alpha, d_beta, mb);
+      if (kd)
+        mrhs_arnoldi_with(st, d_gsec, 1, tiles, ngroups, VS, m, Kcur, dt, d_scale, d_alpha, d_beta, mb,
+                          KronProduct{*kd, d_gsec, m});
+      else
+        mrhs_arnoldi(st, d_gsec, 1, tiles, blocks, ngroups, VS, m, Kcur, dt, d_scale, d_alpha, d_beta, mb);
       g_step_estimate_kernel<<<g1(m, 256), 256, 0, st>>>(m, Kcur, mb.hess, mb.coef, d_est);
       hipMemcpyAsync(h_est.data(), d_est, sizeof(double) * m, hipMemcpyDeviceToHost, st);
       if (hipStreamSynchronize(st) != hipSuccess) {
@@ -741,6 +748,7 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
   }
   auto fail = [&](int code) {
     S->err = qd_last_error(ctx);
+    if (M->kronA) qd_kron_destroy(M->kronA);
     if (M->opA) qd_linop_destroy(M->opA);
     if (M->csrA) qd_csr_destroy(M->csrA);
     if (M->progA) qd_termprog_destroy(M->progA);
@@ -757,6 +765,15 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
   if ((rc = qd_termprog_create(ctx, nbits, tA, ntA, oA, noA, &M->progA))) return fail(rc);
   if ((rc = qd_csr_build(ctx, M->basis, M->progA, &M->csrA))) return fail(rc);
   if ((rc = qd_linop_from_csr(ctx, M->csrA, &M->opA))) return fail(rc);
+  if (structured) {
+    // the batched evolve product needs both packed block tables
+    M->kronA = qd_kron_try_build(ctx, M->basis, M->progA);
+    const KronDesc* kd = qd_kron_desc(M->kronA);
+    if (kd && (!kd->ell || !kd->ellB)) {
+      qd_kron_destroy(M->kronA);
+      M->kronA = nullptr;
+    }
+  }
   if (hipMalloc(&M->d_diagD, sizeof(cplx) * M->n) != hipSuccess) return fail(qd_set_error(ctx, QD_E_OOM, "model diagonal"));
   if (ntD) {
     qd_termprog* pD = nullptr;
@@ -877,6 +894,7 @@ extern "C" int qd_server_destroy(qd_server* S) {
   S->cv.notify_all();
   for (auto& t : S->threads) t.join();
   for (Model* M : S->models) {
+    if (M->kronA) qd_kron_destroy(M->kronA);
     if (M->opA) qd_linop_destroy(M->opA);
     if (M->csrA) qd_csr_destroy(M->csrA);
     if (M->progA) qd_termprog_destroy(M->progA);
