@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--traj-cpu", type=int, default=8, help="trajectories in the oracle's CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
+    ap.add_argument("--hbm-probe-mb", type=int, default=0,
+                    help="measure the streaming read / copy bandwidth over this many MiB and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -60,6 +62,19 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.hbm_probe_mb:
+        # streaming ceiling: non-temporal read and copy kernels of known size
+        import ctypes
+
+        from qudrip_amd.runtime import Context
+        ctx = Context(0)
+        rd, cp = ctypes.c_double(), ctypes.c_double()
+        ctx.check(ctx.lib.qd_stream_bandwidth(ctx.ptr, args.hbm_probe_mb << 20, 20, ctypes.byref(rd),
+                                              ctypes.byref(cp)), "stream_bandwidth")
+        print(json.dumps({"probe_bytes": args.hbm_probe_mb << 20, "read_GBs": round(rd.value, 1),
+                          "copy_GBs": round(cp.value, 1), "peak_GBs": 8000.0}), flush=True)
+        return
 
     if args.calib_copy_mb:
         # plain copy kernel of known size: reads and writes calib_copy_mb MiB each

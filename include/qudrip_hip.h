@@ -204,6 +204,11 @@ int qd_norm2(qd_ctx* ctx, int64_t n, const double* d_psi, double* h_out);
  * i from splitmix64((seed << 32) ^ (2i + c)) mapped to U(-1, 1), then
  * normalised. */
 int qd_seeded_state(qd_ctx* ctx, int64_t n, uint64_t seed, double* d_psi);
+/* Achievable streaming bandwidth (no reference counterpart; the ceiling that
+ * HBM-bound kernels are read against): a non-temporal read of `bytes`, and a
+ * copy of `bytes` into a second buffer, each averaged over `reps` launches
+ * after a warm-up.  Allocates both buffers for the call. */
+int qd_stream_bandwidth(qd_ctx* ctx, size_t bytes, int reps, double* read_gbs, double* copy_gbs);
 
 /* ------------------------------------------------------------ Krylov solver */
 /* Arnoldi::getSolver (herm = 0) / getLanczosSolver (herm = 1)

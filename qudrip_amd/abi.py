@@ -143,6 +143,7 @@ SIGNATURES = {
     "qd_server_submit": (I, [VP, ctypes.POINTER(qd_request)]),
     "qd_server_set_batch": (I, [VP, I]),
     "qd_server_set_gs_method": (I, [VP, I]),
+    "qd_stream_bandwidth": (I, [VP, ctypes.c_size_t, I, DP, DP]),
     "qd_server_poll": (I, [VP, ctypes.POINTER(qd_result), I, ctypes.POINTER(I)]),
     "qd_server_drain": (I, [VP]),
     "qd_server_clock": (ctypes.c_double, [VP]),

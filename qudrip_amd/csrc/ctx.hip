@@ -309,3 +309,91 @@ int qd_scan_counts(qd_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_o
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }
+
+// ------------------------------------------------------------ bandwidth
+// Streaming kernels of known traffic: the ceiling that HBM-bound kernels are
+// read against (non-temporal loads, 4 in flight per lane, grid-stride).
+namespace {
+typedef double bw_d2 __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) bw_read_kernel(const bw_d2* __restrict__ a, int64_t n, double* __restrict__ sink) {
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * 256 * 4;
+  for (int64_t base = (int64_t)blockIdx.x * 256 * 4 + threadIdx.x; base < n; base += stride) {
+    bw_d2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t i = base + (int64_t)u * 256;
+      v[u] = i < n ? __builtin_nontemporal_load(a + i) : bw_d2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += v[u].x + v[u].y;
+  }
+  if (s == 1.25e300) sink[blockIdx.x] = s;  // keeps the loads; never true for the zeroed input
+}
+
+__global__ void __launch_bounds__(256) bw_copy_kernel(const bw_d2* __restrict__ a, bw_d2* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * 256 * 4;
+  for (int64_t base = (int64_t)blockIdx.x * 256 * 4 + threadIdx.x; base < n; base += stride) {
+    bw_d2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t i = base + (int64_t)u * 256;
+      if (i < n) v[u] = __builtin_nontemporal_load(a + i);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t i = base + (int64_t)u * 256;
+      if (i < n) __builtin_nontemporal_store(v[u], b + i);
+    }
+  }
+}
+}  // namespace
+
+extern "C" int qd_stream_bandwidth(qd_ctx* ctx, size_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+  if (!ctx || bytes < 4096 || reps < 1 || !read_gbs || !copy_gbs)
+    return qd_set_error(ctx, QD_E_ARG, "qd_stream_bandwidth: bad arguments");
+  hipSetDevice(ctx->device);
+  const int64_t n = (int64_t)(bytes / 16);
+  bw_d2 *a = nullptr, *b = nullptr;
+  double* sink = nullptr;
+  const unsigned grid = 8192;
+  if (hipMalloc(&a, 16 * (size_t)n) != hipSuccess) return qd_set_error(ctx, QD_E_OOM, "bandwidth buffers");
+  if (hipMalloc(&b, 16 * (size_t)n) != hipSuccess || hipMalloc(&sink, sizeof(double) * grid) != hipSuccess) {
+    hipFree(a);
+    hipFree(b);
+    return qd_set_error(ctx, QD_E_OOM, "bandwidth buffers");
+  }
+  hipStream_t st = ctx->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = QD_OK;
+  float ms_read = 0.f, ms_copy = 0.f;
+  if (hipMemsetAsync(a, 0, 16 * (size_t)n, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = QD_E_HIP;
+  } else {
+    bw_read_kernel<<<grid, 256, 0, st>>>(a, n, sink);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r) bw_read_kernel<<<grid, 256, 0, st>>>(a, n, sink);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms_read, e0, e1) != hipSuccess) rc = QD_E_HIP;
+    bw_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r) bw_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
+    hipEventRecord(e1, st);
+    if (rc == QD_OK &&
+        (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms_copy, e0, e1) != hipSuccess))
+      rc = QD_E_HIP;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(st);
+  hipFree(a);
+  hipFree(b);
+  hipFree(sink);
+  if (rc) return qd_set_error(ctx, rc, "qd_stream_bandwidth: HIP call failed");
+  const double moved = 16.0 * (double)n;
+  *read_gbs = moved * reps / (ms_read * 1e-3) / 1e9;
+  *copy_gbs = 2.0 * moved * reps / (ms_copy * 1e-3) / 1e9;
+  return QD_OK;
+}
