@@ -270,7 +270,11 @@ def test_lanczos_matches_oracle_and_dense(ctx):
         _close(kr, op, c, p, b)
 
 
-def test_lanczos_incremental_and_breakdown(ctx):
+@pytest.mark.parametrize("method", [1, 2])
+def test_lanczos_incremental_and_breakdown(ctx, method):
+    """getGroundState with add_dim in two calls, then an eigenvector start
+    (breakdown at the first step); method 2 carries its orthogonality
+    estimate across the calls."""
     ham, sec = xxz(10, 1.0, 1.0)[1:]
     ref_basis = oracle.basis_scan(sec)
     H = oracle.csc_to_dense(oracle.build_csc(ref_basis, ham))
@@ -278,6 +282,7 @@ def test_lanczos_incremental_and_breakdown(ctx):
     b, p, c = _build(ctx, ham, sec)
     op = LinOp.from_csr(ctx, c)
     kr = Krylov(ctx, b.size, 60, herm=True)
+    kr.set_method(method)
     try:
         psi = to_dev(ctx, oracle.seeded_state(b.size, 3))
         e_a = kr.ground_state(op, psi, 20)  # getGroundState(H, psi, 0, 20)
