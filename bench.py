@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--traj-cpu", type=int, default=8, help="trajectories in the oracle's CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
+    ap.add_argument("--serve-profile", action="store_true",
+                    help="serve the request stream once (default scheme) and exit: a run to profile the served path")
     ap.add_argument("--hbm-probe-mb", type=int, default=0,
                     help="measure the streaming read / copy bandwidth over this many MiB and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -74,6 +76,15 @@ def main():
                                               ctypes.byref(cp)), "stream_bandwidth")
         print(json.dumps({"probe_bytes": args.hbm_probe_mb << 20, "read_GBs": round(rd.value, 1),
                           "copy_GBs": round(cp.value, 1), "peak_GBs": 8000.0}), flush=True)
+        return
+
+    if args.serve_profile:
+        from qudrip_amd import serving as SV
+        reqs = SV.request_stream(args.serve_requests, rate=200.0, seed=4)
+        out = SV.run_served(reqs, workers=args.serve_workers)
+        print(json.dumps({"served_requests": len(reqs), "workers": args.serve_workers,
+                          "solves_per_s": round(out["solves_per_s"], 3), "span_s": round(out["span_s"], 3),
+                          "failed": sum(1 for x in out["results"].values() if x["status"] != 0)}), flush=True)
         return
 
     if args.calib_copy_mb:

@@ -38,7 +38,12 @@ struct KronDesc {
   int ellB_w;
   const cplx* vtabB;
   int nvtabB;
+  // per product: an extra diagonal (n values, state b nA + a), or nullptr
+  const cplx* ext_diag;
 };
 
 // the description of a built Kronecker form (nullptr for nullptr)
 const KronDesc* qd_kron_desc(const qd_kron* k);
+// choose the launch variants now (x, y: n-vector scratch), so that later
+// products from several threads only read them
+int qd_kron_prepare(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y);

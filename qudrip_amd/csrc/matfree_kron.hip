@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(1024)
     }
     cplx xa = LDSX ? xrow[a] : xb[a];
     cplx d = cadd(diag_value(k, sB | k.statesA[a]), shift);
+    if (k.ext_diag) d = cadd(d, k.ext_diag[b * k.nA + a]);
     cplx dx = cmul(d, xa);
     y[b * k.nA + a] = cmake(sx + dx.x, sy + dx.y);
   }
@@ -146,6 +147,7 @@ __global__ void __launch_bounds__(1024)
     for (int r = 0; r < RB; ++r) {
       if (r < rb) {
         cplx d = cadd(diag_value(k, sB[r] | sA), shift);
+        if (k.ext_diag) d = cadd(d, k.ext_diag[(b0 + r) * k.nA + a]);
         cplx dx = cmul(d, xr[r * k.nA + a]);
         y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
       }
@@ -297,6 +299,7 @@ __global__ void __launch_bounds__(1024)
     for (int r = 0; r < RB; ++r) {
       if (r < rb) {
         cplx d = cadd(diag_value(k, sB[r] | sA), shift);
+        if (k.ext_diag) d = cadd(d, k.ext_diag[(b0 + r) * k.nA + a]);
         cplx dx = cmul(d, xr[r * k.nA + a]);
         y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
       }
@@ -649,6 +652,7 @@ qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
 
 const KronDesc* qd_kron_desc(const qd_kron* k) { return k ? &k->desc : nullptr; }
 
+
 void qd_kron_destroy(qd_kron* k) {
   if (!k) return;
   hipFree(k->d_ell);
@@ -836,13 +840,14 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
 }
 }  // namespace
 
-int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
-  const KronDesc& d = k->desc;
-  if (d.nA == 0 || d.nB == 0) return QD_OK;
+int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, const cplx* ext_diag) {
+  if (k->desc.nA == 0 || k->desc.nB == 0) return QD_OK;
   if (k->rows_variant < 0) {
     int rc = kron_tune(ctx, k, x, y);  // leaves y overwritten: the product below rewrites it
     if (rc) return rc;
   }
+  KronDesc d = k->desc;
+  d.ext_diag = ext_diag;
   const int optin = kron_optin(ctx->device);
   if (!launch_rows(d, k->rows_variant, optin, x, y, shift, ctx->stream))
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", k->rows_variant);
@@ -851,4 +856,9 @@ int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift) {
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker slab variant %d", k->slab_variant);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
+}
+
+int qd_kron_prepare(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
+  if (!k || k->rows_variant >= 0 || k->desc.nA == 0 || k->desc.nB == 0) return QD_OK;
+  return kron_tune(ctx, k, x, y);
 }

@@ -259,13 +259,13 @@ struct qd_linop {
   qd_kron* kron = nullptr;  // two-block Kronecker form, when the program has it
   int use_kron = 1;
   // extra diagonal added by every product (qd_linop_with_diagonal); a view
-  // linop borrows the arrays of its base and owns none
+  // linop borrows the arrays (and Kronecker form) of its base and owns none
   const cplx* diag0 = nullptr;
   bool view = false;
 };
 qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog);
 void qd_kron_destroy(qd_kron* k);
-int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift);
+int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, const cplx* ext_diag = nullptr);
 
 struct qd_krylov {
   qd_ctx* ctx = nullptr;

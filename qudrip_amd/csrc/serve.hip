@@ -120,7 +120,8 @@ struct Model {
   qd_termprog* progA = nullptr;
   qd_csr* csrA = nullptr;
   qd_linop* opA = nullptr;
-  qd_kron* kronA = nullptr;  // structured models: A in two-block form (lockstep batches)
+  qd_linop* opK = nullptr;   // structured models: matrix-free A (two-block form when it has one)
+  qd_kron* kronA = nullptr;  // opK's two-block form with both packed tables (borrowed; lockstep batches)
   cplx* d_diagD = nullptr;
   double* d_diagR = nullptr;  // its real part (the diagonal of a Hermitian D)
   size_t bytes = 0;
@@ -279,13 +280,11 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
   const int64_t n = M.n;
   cplx *diag = nullptr, *psi = nullptr, *nxt = nullptr;
   qd_linop* op = nullptr;
-  qd_termprog* prog = nullptr;
   qd_krylov* kr = nullptr;
   int rc = QD_OK;
   auto done = [&](int code) {
     if (kr) qd_krylov_destroy(kr);
     if (op) qd_linop_destroy(op);
-    if (prog) qd_termprog_destroy(prog);
     hipStreamSynchronize(ctx->stream);
     qd_dev_free(ctx, diag);
     qd_dev_free(ctx, psi);
@@ -295,27 +294,12 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
   if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&psi) != QD_OK ||
       qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&nxt) != QD_OK)
     return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
-  if (M.structured) {
-    // H = A + c D as one program: the matrix-free product recognises the
-    // two-block (Kronecker) form of spinful lattice models
-    std::vector<qd_term> terms = M.tA;
-    std::vector<qd_op> ops = M.oA;
-    for (qd_term t : M.tD) {
-      t.coeff[0] *= r.coeff;
-      t.coeff[1] *= r.coeff;
-      t.op_begin += (int32_t)M.oA.size();
-      terms.push_back(t);
-    }
-    ops.insert(ops.end(), M.oD.begin(), M.oD.end());
-    if ((rc = qd_termprog_create(ctx, M.nbits, terms.data(), (int)terms.size(), ops.data(), (int)ops.size(), &prog)))
-      return done(rc);
-    if ((rc = qd_linop_matfree(ctx, M.basis, prog, &op))) return done(rc);
-  } else {
-    if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&diag) != QD_OK)
-      return done(qd_set_error(ctx, QD_E_OOM, "request diagonal"));
-    scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
-    if ((rc = qd_linop_with_diagonal(ctx, M.opA, (const double*)diag, &op))) return done(rc);
-  }
+  // H = A + c D: a diagonal view of the model's A (for structured models its
+  // matrix-free two-block form, else its stored matrix)
+  if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&diag) != QD_OK)
+    return done(qd_set_error(ctx, QD_E_OOM, "request diagonal"));
+  scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
+  if ((rc = qd_linop_with_diagonal(ctx, M.structured ? M.opK : M.opA, (const double*)diag, &op))) return done(rc);
   if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
   if ((rc = qd_krylov_create(ctx, n, r.k, 0, &kr))) return done(rc);
   double h0[2] = {0.0, 0.0}, n0 = 0.0;
@@ -748,7 +732,7 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
   }
   auto fail = [&](int code) {
     S->err = qd_last_error(ctx);
-    if (M->kronA) qd_kron_destroy(M->kronA);
+    if (M->opK) qd_linop_destroy(M->opK);
     if (M->opA) qd_linop_destroy(M->opA);
     if (M->csrA) qd_csr_destroy(M->csrA);
     if (M->progA) qd_termprog_destroy(M->progA);
@@ -766,12 +750,24 @@ extern "C" int qd_server_add_model(qd_server* S, int nbits, const qd_constraint*
   if ((rc = qd_csr_build(ctx, M->basis, M->progA, &M->csrA))) return fail(rc);
   if ((rc = qd_linop_from_csr(ctx, M->csrA, &M->opA))) return fail(rc);
   if (structured) {
-    // the batched evolve product needs both packed block tables
-    M->kronA = qd_kron_try_build(ctx, M->basis, M->progA);
-    const KronDesc* kd = qd_kron_desc(M->kronA);
-    if (kd && (!kd->ell || !kd->ellB)) {
-      qd_kron_destroy(M->kronA);
-      M->kronA = nullptr;
+    // A's two-block form, built once: real-time requests add c D as a
+    // diagonal view of it; its launch variants are chosen here, before
+    // several workers share it
+    if ((rc = qd_linop_matfree(ctx, M->basis, M->progA, &M->opK))) return fail(rc);
+    if (M->opK->kron) {
+      cplx *sx = nullptr, *sy = nullptr;
+      if (hipMalloc(&sx, sizeof(cplx) * M->n) != hipSuccess || hipMalloc(&sy, sizeof(cplx) * M->n) != hipSuccess) {
+        hipFree(sx);
+        return fail(qd_set_error(ctx, QD_E_OOM, "model scratch"));
+      }
+      rc = qd_seeded_state(ctx, M->n, 1, (double*)sx);
+      if (rc == QD_OK) rc = qd_kron_prepare(ctx, M->opK->kron, sx, sy);
+      hipStreamSynchronize(ctx->stream);
+      hipFree(sx);
+      hipFree(sy);
+      if (rc) return fail(rc);
+      const KronDesc* kd = qd_kron_desc(M->opK->kron);
+      if (kd->ell && kd->ellB) M->kronA = M->opK->kron;  // the batched product needs both packed tables
     }
   }
   if (hipMalloc(&M->d_diagD, sizeof(cplx) * M->n) != hipSuccess) return fail(qd_set_error(ctx, QD_E_OOM, "model diagonal"));
@@ -894,7 +890,7 @@ extern "C" int qd_server_destroy(qd_server* S) {
   S->cv.notify_all();
   for (auto& t : S->threads) t.join();
   for (Model* M : S->models) {
-    if (M->kronA) qd_kron_destroy(M->kronA);
+    if (M->opK) qd_linop_destroy(M->opK);
     if (M->opA) qd_linop_destroy(M->opA);
     if (M->csrA) qd_csr_destroy(M->csrA);
     if (M->progA) qd_termprog_destroy(M->progA);

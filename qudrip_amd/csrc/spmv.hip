@@ -365,8 +365,8 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
       default:
         return qd_set_error(ctx, QD_E_ARG, "lanes_per_row %d", op->lanes_per_row);
     }
-  } else if (op->kron && op->use_kron && !d_xscale && !d_diag) {
-    return qd_kron_apply(ctx, op->kron, x, y, shift);
+  } else if (op->kron && op->use_kron && !d_xscale) {
+    return qd_kron_apply(ctx, op->kron, x, y, shift, d_diag);
   } else {
     qd_termprog* p = op->prog;
     spmv_matfree_kernel<<<grid8((n + 255) / 256), 256, 0, ctx->stream>>>(
@@ -440,7 +440,7 @@ extern "C" int qd_linop_set_structured(qd_linop* op, int on, int* has_structured
 }
 
 extern "C" int qd_linop_destroy(qd_linop* op) {
-  if (op && op->kron) qd_kron_destroy(op->kron);
+  if (op && op->kron && !op->view) qd_kron_destroy(op->kron);
   delete op;
   return QD_OK;
 }
@@ -452,8 +452,6 @@ extern "C" int qd_linop_with_diagonal(qd_ctx* ctx, qd_linop* base, const double*
   if (base->diag0) return qd_set_error(ctx, QD_E_UNSUPPORTED, "the base operator already carries a diagonal");
   qd_linop* op = new qd_linop(*base);
   op->ctx = ctx;
-  op->kron = nullptr;  // the Kronecker kernels take no diagonal; the generic kernels do
-  op->use_kron = 0;
   op->diag0 = reinterpret_cast<const cplx*>(d_diag);
   op->view = true;
   *out = op;
