@@ -12,6 +12,11 @@ roofline.achieved = B / average kernel duration from HIP events on the
 stream the kernel runs on.  The line also carries the full Lanczos ground
 state (k = 150, full reorthogonalisation) against the Bethe-ansatz energy,
 and the oracle's single-thread CPU product timed on a bounded sample.
+Further sections (each can be skipped): the Hubbard 4x4 quench (ground state
+then real-time steps with the matrix-free Kronecker product), 4096 batched
+quantum-jump trajectories of the dissipative Hubbard chain L=8 against the
+one-at-a-time loop and the oracle, and the served request stream (partial and
+full reorthogonalisation) against the plain one-at-a-time path.
 """
 import argparse
 import json
