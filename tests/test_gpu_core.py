@@ -408,6 +408,17 @@ def test_matfree_structured_matches_oracle(ctx, case):
         scale = np.linalg.norm(yr)
         assert np.linalg.norm(ys - yr) / scale < 1e-14
         assert np.linalg.norm(yg - yr) / scale < 1e-14
+        # a diagonal view keeps the two-block form: (H + diag(d)) x, d complex
+        rng = np.random.default_rng(5)
+        d = rng.standard_normal(len(basis)) + 1j * rng.standard_normal(len(basis))
+        yd = yr + d * x
+        dd = to_dev(ctx, d)
+        for structured in (True, False):
+            mf.set_structured(structured)
+            view = LinOp.with_diagonal(ctx, mf, dd)
+            yv = view.apply(xd).cpu().numpy()
+            view.close()
+            assert np.linalg.norm(yv - yd) / np.linalg.norm(yd) < 1e-14, structured
     finally:
         _close(mf, p, b)
 
