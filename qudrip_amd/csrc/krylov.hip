@@ -751,6 +751,7 @@ extern "C" int qd_krylov_reset(qd_krylov* kr) {
   if (!kr) return QD_E_ARG;
   kr->act_dim = 0;
   kr->breakdown = 0;
+  kr->pending = false;
   kr->beta_last = 0.0;
   std::fill(kr->KR.begin(), kr->KR.end(), cmake(0.0, 0.0));
   return QD_OK;
@@ -775,6 +776,7 @@ static int kr_init(qd_krylov* kr, const cplx* psi) {
   kr->full_steps = 0;
   kr->act_dim = 0;
   kr->breakdown = 0;
+  kr->pending = false;
   std::fill(kr->KR.begin(), kr->KR.end(), cmake(0.0, 0.0));
   return QD_OK;
 }
@@ -799,8 +801,13 @@ static void launch_gs_step(qd_krylov* kr, const KrylovDev& d, int j, cplx* w) {
   scale_kernel<<<qd_grid(n, 256), 256, 0, st>>>(n, w, d.inv, d.flag);
 }
 
+// cont: U holds the pending vector u_{s+1} left by the previous expansion
+// (no bootstrap); fin: the last step second-projects the pending vector into
+// V[:, s + add] (no product), else it runs as a full step and leaves
+// u_{s+add+1} pending.  Either way V[:, 0..s+add] and Hessenberg columns
+// 0..s+add-1 are final afterwards.
 template <int RPT>
-static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int s, int add) {
+static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int s, int add, bool cont, bool fin) {
   qd_ctx* ctx = kr->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = kr->n;
@@ -811,15 +818,19 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
   const bool windowed = kr->method >= 2;
   const Pro pro = windowed ? Pro{d.lo, d.om, d.prm, kr->kry_dim + 2, PRO_W} : Pro{nullptr, nullptr, nullptr, 0, 0};
   auto win = [&](int j) { return windowed ? Win{d.lo, j, PRO_W} : Win{nullptr, 0, 0}; };
-  // bootstrap: u_{s+1} = A q_s - Q_s (Q_s^H A q_s); H[0:s+1, s] = first pass
-  int rc = qd_linop_apply(op, kr->d_V + (size_t)s * kr->ld, U, nullptr, 0, nullptr, cmake(0.0, 0.0));
-  if (rc) return rc;
-  const Win w0 = win(s + 1);
-  gemv_t_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, U, d.part, d.flag, w0);
-  reduce_cols_kernel<<<s + 1, KT, 0, st>>>(d.part, nb, s + 1, d.h, d.KR + (size_t)s * ldk, 0, d.flag, 1 << 30, w0,
-                                           s + 1, 1);
-  gemv_n_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, d.h, U, nullptr, d.flag, w0);
-  for (int j = s + 1; j < s + add; ++j) {
+  int rc;
+  if (!cont) {
+    // bootstrap: u_{s+1} = A q_s - Q_s (Q_s^H A q_s); H[0:s+1, s] = first pass
+    rc = qd_linop_apply(op, kr->d_V + (size_t)s * kr->ld, U, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    if (rc) return rc;
+    const Win w0 = win(s + 1);
+    gemv_t_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, U, d.part, d.flag, w0);
+    reduce_cols_kernel<<<s + 1, KT, 0, st>>>(d.part, nb, s + 1, d.h, d.KR + (size_t)s * ldk, 0, d.flag, 1 << 30, w0,
+                                             s + 1, 1);
+    gemv_n_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, s + 1, d.h, U, nullptr, d.flag, w0);
+  }
+  const int jend = fin ? s + add : s + add + 1;
+  for (int j = s + 1; j < jend; ++j) {
     rc = qd_linop_apply(op, U, W, nullptr, 0, nullptr, cmake(0.0, 0.0));
     if (rc) return rc;
     const int P = 2 * j + 2;
@@ -830,6 +841,10 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
                                       pro);
     dc_update_kernel<RPT, false><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.coefA, d.coefB, d.scal, d.flag,
                                                     wj);
+  }
+  if (!fin) {
+    QD_CHECK_LAUNCH(ctx);
+    return QD_OK;
   }
   // finalise: reorthogonalise the pending vector into V[:, s + add]
   const int j = s + add;
@@ -846,15 +861,23 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
 
 // DCGS2 expansion by add columns from act_dim (two passes over the basis per
 // step).  Breakdown at step j leaves act_dim = j.
-static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add) {
+// may_continue: a later call will probably extend this basis (chunked
+// ground states), so the last step leaves the pending vector for it instead
+// of finalising (saves the finalisation and the next bootstrap: two passes
+// over the basis).
+static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add, bool may_continue) {
   qd_ctx* ctx = kr->ctx;
   if (kr->breakdown || add <= 0) return QD_OK;
   KrylovDev d = kr_layout(kr);
   QD_HIP(ctx, hipMemsetAsync(d.flag, 0, 16, ctx->stream));
   const int s = kr->act_dim;
+  const bool cont = kr->pending;
+  const bool fin = !may_continue || s + add >= kr->kry_dim;
+  kr->pending = false;
   int rpt = pick_rpt(kr->n, ctx->num_cus);
-  int rc = (rpt == 8) ? dc_expand_launch<8>(kr, op, d, s, add)
-                      : (rpt == 4) ? dc_expand_launch<4>(kr, op, d, s, add) : dc_expand_launch<1>(kr, op, d, s, add);
+  int rc = (rpt == 8)   ? dc_expand_launch<8>(kr, op, d, s, add, cont, fin)
+           : (rpt == 4) ? dc_expand_launch<4>(kr, op, d, s, add, cont, fin)
+                        : dc_expand_launch<1>(kr, op, d, s, add, cont, fin);
   if (rc) return rc;
   const int K = kr->kry_dim, ldk = K + 2;
   std::vector<cplx> cols((size_t)add * ldk);
@@ -872,7 +895,8 @@ static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add) {
     end = hflag[2];  // columns 0..end-1 span an invariant subspace
     kr->breakdown = 1;
   }
-  kr->n_spmv += add;
+  kr->n_spmv += add + (cont ? 0 : 1) - (fin ? 1 : 0);
+  kr->pending = !fin && !hflag[0];
   for (int j = s; j < end; ++j) {
     const cplx* c = cols.data() + (size_t)(j - s) * ldk;
     for (int i = 0; i <= j && i < K; ++i) kr->KR[(size_t)j * K + i] = c[i];
@@ -886,8 +910,12 @@ static int kr_expand_dc(qd_krylov* kr, qd_linop* op, int add) {
 
 // extend the basis from act_dim by add columns; reads KR back.
 static int kr_expand_cgs2(qd_krylov* kr, qd_linop* op, int add);
-static int kr_expand(qd_krylov* kr, qd_linop* op, int add) {
-  return kr->method == 0 ? kr_expand_cgs2(kr, op, add) : kr_expand_dc(kr, op, add);
+static int kr_expand(qd_krylov* kr, qd_linop* op, int add, bool may_continue = false) {
+  if (kr->method == 0) {
+    kr->pending = false;
+    return kr_expand_cgs2(kr, op, add);
+  }
+  return kr_expand_dc(kr, op, add, may_continue);
 }
 
 static int kr_expand_cgs2(qd_krylov* kr, qd_linop* op, int add) {
@@ -958,7 +986,7 @@ int qd_krylov_gs_step(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, d
   }
   int add = add_dim;
   if (add == -1 || kr->kry_dim < kr->act_dim + add) add = kr->kry_dim - kr->act_dim;  // :523-524
-  rc = kr_expand(kr, op, add);
+  rc = kr_expand(kr, op, add, true);  // chunked callers extend the same basis again
   if (rc) return rc;
   int m = kr->act_dim;
   if (m < 1) return qd_set_error(ctx, QD_E_BREAKDOWN, "Krylov space is empty");

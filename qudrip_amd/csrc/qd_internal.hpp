@@ -289,6 +289,7 @@ struct qd_krylov {
   int64_t n_spmv = 0;
   int method = 1;  // 0 = CGS2, 1 = DCGS2, 2 = windowed + partial, 3 = windowed
   int full_steps = 0;  // steps of the partial scheme that covered the whole basis
+  bool pending = false;  // d_w holds the next pending vector of a resumable expansion
 };
 
 // --------------------------------------------------------------- errors
