@@ -265,7 +265,7 @@ def serving(ctx, args):
     16..26, Jz ~ U[0,2]), 30 % 3x4 Hubbard real-time solves (U ~ U[2,8], 100
     steps of 0.05).  Served: the solve server (structure cache, concurrent
     workers, ground states stopped when the Ritz-residual bound r^2/gap or the
-    move of E0 over 10 Krylov vectors is <= 1e-12 |E0|, matrix-free real-time
+    move of E0 over 4 Krylov vectors is <= 1e-12 |E0|, matrix-free real-time
     operator with the Krylov dimension set per step by Saad's estimate).  The
     ground states use partial reorthogonalisation by default; the stream is
     served a second time with full reorthogonalisation (the drop-in path's

@@ -111,7 +111,7 @@ def hubbard_lattice(Lx=3, Ly=4, nup=6, ndn=6):
     return nsite, Q.hubbard_bonds_square(Lx, Ly), nup, ndn
 
 
-def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=10, gs_tol=1e-12,
+def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=4, gs_tol=1e-12,
                rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, gs_method=2, log=None):
     """Registers the models (timed apart: the structure cache is built once at
     start-up), then submits every request at its arrival time and waits."""
