@@ -32,13 +32,28 @@ namespace {
 
 constexpr int KT = 256;  // threads per block for the tall-skinny kernels
 
-__device__ inline cplx wave_sum(cplx s) {
-  for (int o = 32; o > 0; o >>= 1) {
-    s.x += __shfl_xor(s.x, o, 64);
-    s.y += __shfl_xor(s.y, o, 64);
-  }
-  return s;
+// Wavefront sums with DPP lane moves (vector ALU, no LDS round trips):
+// quad xor 1 and 2, half-row and row mirrors (every lane of a 16-lane row
+// holds the row's sum), then row broadcasts 15 and 31 accumulate the rows
+// into lane 63, which is broadcast to every lane.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, true);
+  return __hiloint2double(hi, lo);
 }
+__device__ inline double wave_sum_d(double v) {
+  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xF>(v);  // row_mirror
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline cplx wave_sum(cplx s) { return cmake(wave_sum_d(s.x), wave_sum_d(s.y)); }
 
 // Columns [lo, ncols) of the basis that a projection covers.  arr == nullptr:
 // the whole basis.  Otherwise arr[j] is the first column for step j: >= 0 as
@@ -165,7 +180,7 @@ __global__ void __launch_bounds__(KT)
     }
   }
   if (norm_part) {
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    s = wave_sum_d(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -184,7 +199,7 @@ __global__ void __launch_bounds__(KT)
   __shared__ double sh[KT / 64];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += KT) s += norm_part[b];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = wave_sum_d(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -255,7 +270,7 @@ __global__ void norm_partial_kernel(int64_t n, const cplx* __restrict__ a, doubl
     cplx v = a[i];
     s += v.x * v.x + v.y * v.y;
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = wave_sum_d(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -268,7 +283,7 @@ __global__ void sum_double_kernel(const double* __restrict__ part, int nb, doubl
   __shared__ double sh[KT / 64];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += KT) s += part[b];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = wave_sum_d(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -345,17 +360,11 @@ __global__ void __launch_bounds__(KT)
           }
         }
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        a0.x += __shfl_xor(a0.x, o, 64);
-        a0.y += __shfl_xor(a0.y, o, 64);
-        a1.x += __shfl_xor(a1.x, o, 64);
-        a1.y += __shfl_xor(a1.y, o, 64);
-        if (NR == 2) {
-          b0.x += __shfl_xor(b0.x, o, 64);
-          b0.y += __shfl_xor(b0.y, o, 64);
-          b1.x += __shfl_xor(b1.x, o, 64);
-          b1.y += __shfl_xor(b1.y, o, 64);
-        }
+      a0 = wave_sum(a0);
+      a1 = wave_sum(a1);
+      if (NR == 2) {
+        b0 = wave_sum(b0);
+        b1 = wave_sum(b1);
       }
       if (lane == 0) {
         sh[wv][c - c0][0] = a0;
@@ -509,7 +518,7 @@ __global__ void __launch_bounds__(KT)
     cc += c[i].x * c[i].x + c[i].y * c[i].y;
     if (!last) chd = cadd(chd, cdotmul(c[i], d[i]));
   }
-  for (int o = 32; o > 0; o >>= 1) cc += __shfl_xor(cc, o, 64);
+  cc = wave_sum_d(cc);
   chd = wave_sum(chd);
   if ((threadIdx.x & 63) == 0) {
     sred[threadIdx.x >> 6] = cc;
