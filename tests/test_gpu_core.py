@@ -496,3 +496,12 @@ def test_partial_reorth_matches_full(ctx, case):
     ov = abs(np.vdot(g1, g2)) / (np.linalg.norm(g1) * np.linalg.norm(g2))
     assert ov > 1 - 1e-9
     assert 0 < inf2["full_steps"] < 75, inf2
+
+
+def test_stream_bandwidth_probe(ctx):
+    """qd_stream_bandwidth: positive rates below the HBM peak; bad sizes rejected."""
+    import ctypes
+    rd, cp = ctypes.c_double(), ctypes.c_double()
+    assert ctx.lib.qd_stream_bandwidth(ctx.ptr, 256 << 20, 3, ctypes.byref(rd), ctypes.byref(cp)) == 0
+    assert 500.0 < rd.value < 8000.0 and 500.0 < cp.value < 8000.0, (rd.value, cp.value)
+    assert ctx.lib.qd_stream_bandwidth(ctx.ptr, 16, 3, ctypes.byref(rd), ctypes.byref(cp)) != 0
