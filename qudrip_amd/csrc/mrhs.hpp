@@ -33,6 +33,8 @@ struct GSec {
   int64_t tile0;  // first reduction tile (tiles = nch * groups)
   int64_t blk0;   // first product block (blocks = ceil(n / SPMM_RB) * groups)
   int64_t grp0;   // first 64-slot group (sum of groups over the earlier sectors)
+  int mp;         // lanes per row in a reduction tile: 64 (one slot per lane), or for
+                  // a single-group sector a power of two >= m (64 / mp rows at a time)
   cplx* X;
   const int32_t* outer;
   const int32_t* inner;
@@ -57,8 +59,9 @@ __device__ inline int find_sec(int ns, int64_t v, F key) {
 
 
 // Reduction tile -> (sector, chunk, slot of this lane); one wave per tile.
+// With mp < 64 the wave covers rstep = 64 / mp rows at a time: lane = sub mp + t.
 struct Tile {
-  int sec, chunk, t;
+  int sec, chunk, t, sub, rstep;
 };
 __device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
   Tile r;
@@ -67,9 +70,24 @@ __device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
   const GSec& S = gs[r.sec];
   const int64_t local = b - S.tile0;
   r.chunk = (int)(local / S.groups);
-  r.t = (int)(local % S.groups) * TT + threadIdx.x;
+  if (S.mp < TT) {
+    r.t = (int)threadIdx.x % S.mp;
+    r.sub = (int)threadIdx.x / S.mp;
+    r.rstep = TT / S.mp;
+  } else {
+    r.t = (int)(local % S.groups) * TT + threadIdx.x;
+    r.sub = 0;
+    r.rstep = 1;
+  }
   return r;
 }
+
+// sum over the rstep lanes of a tile that hold the same slot (lanes t, t + mp, ...)
+__device__ inline double sub_sum(double v, int mp) {
+  for (int o = mp; o < TT; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline cplx sub_sum(cplx v, int mp) { return cmake(sub_sum(v.x, mp), sub_sum(v.y, mp)); }
 
 // One pass over a row chunk of NQ basis vectors and W for every sector
 // (trajectory-minor blocks in the V / W pools, lds = m).
@@ -87,21 +105,22 @@ __global__ void __launch_bounds__(TT)
   const GSec& S = gs[tl.sec];
   const int t = tl.t;
   const int m = S.m;
-  if (t >= m) return;
+  const bool act = t < m;
+  if (tl.rstep == 1 && !act) return;  // no cross-lane sums in one-slot-per-lane tiles
   const int64_t r0 = (int64_t)tl.chunk * S.rpc;
-  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const int64_t r1 = act ? min<int64_t>(S.n, r0 + S.rpc) : r0;
   const cplx* __restrict__ Vb = V + S.vo;
   cplx* __restrict__ Wb = W + S.vo;
   cplx hc[NQ];
   if (MODE != 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) hc[q] = h[(int64_t)q * Mtot + S.so + t];
+    for (int q = 0; q < NQ; ++q) hc[q] = act ? h[(int64_t)q * Mtot + S.so + t] : cmake(0.0, 0.0);
   }
   cplx acc[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) acc[q] = cmake(0.0, 0.0);
 #pragma unroll 2
-  for (int64_t s = r0; s < r1; ++s) {
+  for (int64_t s = r0 + tl.sub; s < r1; s += tl.rstep) {
     const int64_t o = s * m + t;
     cplx v[NQ];
 #pragma unroll
@@ -122,6 +141,11 @@ __global__ void __launch_bounds__(TT)
       for (int q = 0; q < NQ; ++q) acc[q] = cadd(acc[q], cdotmul(v[q], w));
     }
   }
+  if (tl.rstep > 1) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = sub_sum(acc[q], S.mp);
+  }
+  if (!act || tl.sub != 0) return;
   cplx* pb = part + NP * S.co;
 #pragma unroll
   for (int q = 0; q < NP; ++q) pb[((int64_t)tl.chunk * NP + q) * m + t] = acc[q];
@@ -278,9 +302,10 @@ __global__ void __launch_bounds__(TT)
   const GSec& S = gs[tl.sec];
   const int t = tl.t;
   const int m = S.m;
-  if (t >= m) return;
+  const bool act = t < m;
+  if (tl.rstep == 1 && !act) return;  // no cross-lane sums in one-slot-per-lane tiles
   const int64_t r0 = (int64_t)tl.chunk * S.rpc;
-  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const int64_t r1 = act ? min<int64_t>(S.n, r0 + S.rpc) : r0;
   const cplx* __restrict__ Vb = V + S.vo;
   const cplx* __restrict__ Ub = U + S.vo;
   const cplx* __restrict__ Wb = W + S.vo;
@@ -291,7 +316,7 @@ __global__ void __launch_bounds__(TT)
 #pragma unroll
   for (int q = 0; q < (NR == 2 ? NQ : 1); ++q) ad[q] = cmake(0.0, 0.0);
 #pragma unroll 2
-  for (int64_t s = r0; s < r1; ++s) {
+  for (int64_t s = r0 + tl.sub; s < r1; s += tl.rstep) {
     const int64_t o = s * m + t;
     const cplx u = Ub[o];
     const cplx w = (NR == 2) ? Wb[o] : cmake(0.0, 0.0);
@@ -304,6 +329,17 @@ __global__ void __launch_bounds__(TT)
     uu = cadd(uu, cdotmul(u, u));
     if (NR == 2) uw = cadd(uw, cdotmul(u, w));
   }
+  if (tl.rstep > 1) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) ac[q] = sub_sum(ac[q], S.mp);
+    if (NR == 2) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) ad[q] = sub_sum(ad[q], S.mp);
+    }
+    uu = sub_sum(uu, S.mp);
+    uw = sub_sum(uw, S.mp);
+  }
+  if (!act || tl.sub != 0) return;
   cplx* pb = part + P * S.co + (int64_t)tl.chunk * P * m + t;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) pb[(int64_t)q * m] = ac[q];
@@ -419,7 +455,7 @@ __global__ void __launch_bounds__(TT)
   cplx* __restrict__ Ub = U + S.vo;
   const cplx* __restrict__ Wb = W + S.vo;
 #pragma unroll 2
-  for (int64_t s = r0; s < r1; ++s) {
+  for (int64_t s = r0 + tl.sub; s < r1; s += tl.rstep) {
     const int64_t o = s * m + t;
     cplx s1 = cmake(0.0, 0.0), s2 = cmake(0.0, 0.0);
 #pragma unroll
@@ -511,13 +547,14 @@ __global__ void __launch_bounds__(TT)
   const GSec& S = gs[tl.sec];
   const int t = tl.t;
   const int m = S.m;
-  if (t >= m) return;
+  const bool act = t < m;
+  if (tl.rstep == 1 && !act) return;
   const int64_t r0 = (int64_t)tl.chunk * S.rpc;
-  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);
+  const int64_t r1 = act ? min<int64_t>(S.n, r0 + S.rpc) : r0;
   double acc[MAXM + 1];
 #pragma unroll
   for (int q = 0; q <= MAXM; ++q) acc[q] = 0.0;
-  for (int64_t s = r0; s < r1; ++s) {
+  for (int64_t s = r0 + tl.sub; s < r1; s += tl.rstep) {
     const cplx v = S.X[s * S.ldx + t];
     const double p = v.x * v.x + v.y * v.y;
     const uint64_t x = S.states[s];
@@ -526,6 +563,13 @@ __global__ void __launch_bounds__(TT)
       if ((x >> q) & 1ull) acc[q] += p;
     acc[MAXM] += p;
   }
+  if (tl.rstep > 1) {
+#pragma unroll
+    for (int q = 0; q < MAXM; ++q)
+      if (q < M) acc[q] = sub_sum(acc[q], S.mp);
+    acc[MAXM] = sub_sum(acc[MAXM], S.mp);
+  }
+  if (!act || tl.sub != 0) return;
   double* pb = part + (int64_t)(M + 1) * S.co;
 #pragma unroll
   for (int q = 0; q < MAXM; ++q)

@@ -491,6 +491,7 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
   };
   std::vector<Job> pending{first};
   int Kcur = std::min(Kmax, first.req.chunk > 0 ? first.req.chunk : 12);
+  int calm = 0;  // consecutive steps whose estimate had a wide margin
   int rc = ok ? QD_OK : QD_E_OOM;
   for (;;) {
     // admit: this model's queued evolve requests with the same k and dt
@@ -563,6 +564,11 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
     g.m = m;
     g.ldx = C;
     g.groups = (m + TT - 1) / TT;
+    g.mp = TT;
+    if (m <= TT / 2) {  // a few slots: 64 / mp rows per wave, mp >= m lanes per row
+      g.mp = 1;
+      while (g.mp < m) g.mp <<= 1;
+    }
     g.rpc = rows_per_chunk(n, g.groups, 2048);
     g.nch = (int)((n + g.rpc - 1) / g.rpc);
     g.X = X;
@@ -614,7 +620,13 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
       ++sl.steps_done;
       sl.msum += Kcur;
     }
-    if (tol < 1e300 && maxest < 0.01 * tol && Kcur > 2) --Kcur;
+    // shrink only after a sustained margin: a step redone at a larger
+    // dimension costs a whole Arnoldi step
+    calm = (tol < 1e300 && maxest < 1e-3 * tol) ? calm + 1 : 0;
+    if (calm >= 10 && Kcur > 2) {
+      --Kcur;
+      calm = 0;
+    }
   }
   if (rc != QD_OK) {
     for (auto& sl : slots) fail_slot(sl, rc);

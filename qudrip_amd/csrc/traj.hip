@@ -353,6 +353,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       g.m = S.m;
       g.ldx = S.ld;
       g.groups = (S.m + TT - 1) / TT;
+      g.mp = TT;  // one slot per lane (sector results do not depend on the mix of sizes)
       g.rpc = rows_per_chunk(S.n, g.groups, occupancy_target);
       g.nch = (int)((S.n + g.rpc - 1) / g.rpc);
       g.so = (int)h_slot2traj.size();
