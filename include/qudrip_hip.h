@@ -166,9 +166,10 @@ int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);
 int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, qd_linop** out);
 int qd_linop_destroy(qd_linop* op);
 /* A view of `base` whose products add diag(d): (base + diag(d)) x, with
- * d_diag n complex values on the device.  Borrows base's arrays and d_diag
- * (both must outlive the view).  Lets many operators H = A + c D share one
- * stored A (no reference counterpart; used by the solve server below). */
+ * d_diag n complex values on the device.  Borrows base's arrays (stored
+ * matrix, or matrix-free program and its two-block form) and d_diag (both
+ * must outlive the view).  Lets many operators H = A + c D share one A
+ * (no reference counterpart; used by the solve server below). */
 int qd_linop_with_diagonal(qd_ctx* ctx, qd_linop* base, const double* d_diag, qd_linop** out);
 int64_t qd_linop_dim(const qd_linop* op);
 /* Matrix-free operators over a two-block sector (up bits low, down bits
