@@ -55,6 +55,47 @@ __device__ inline double wave_sum_d(double v) {
 }
 __device__ inline cplx wave_sum(cplx s) { return cmake(wave_sum_d(s.x), wave_sum_d(s.y)); }
 
+// gfx950 cross-half swaps: permlane32 trades lanes 32-63 of a with lanes
+// 0-31 of b; permlane16 trades the odd 16-lane rows of a with the even rows
+// of b.
+__device__ inline void swap_lanes32(double& a, double& b) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+  a = __hiloint2double((int)hi[0], (int)lo[0]);
+  b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ inline void swap_lanes16(double& a, double& b) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+  a = __hiloint2double((int)hi[0], (int)lo[0]);
+  b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// Eight wavefront sums in one butterfly (each exchange step halves the
+// values a lane carries): afterwards lanes 8v .. 8v+7 hold the sum of d[v]
+// over the wave.  About 34 instructions against 160 for eight wave_sum_d.
+__device__ inline double wave_sum8(const double (&d)[8]) {
+  double e[4], f[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {  // lane bit 5 picks d[i] or d[i + 4]
+    double a = d[i], b = d[i + 4];
+    swap_lanes32(a, b);
+    e[i] = a + b;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {  // lane bit 4 picks e[i] or e[i + 2]
+    double a = e[i], b = e[i + 2];
+    swap_lanes16(a, b);
+    f[i] = a + b;
+  }
+  const bool hi8 = (threadIdx.x & 8) != 0;  // lane bit 3 picks f[0] or f[1]
+  const double own = hi8 ? f[1] : f[0], send = hi8 ? f[0] : f[1];
+  double g = own + dpp_move<0x128, 0xF>(send);  // row_ror:8 = lane xor 8
+  g += dpp_move<0xB1, 0xF>(g);
+  g += dpp_move<0x4E, 0xF>(g);
+  g += dpp_move<0x141, 0xF>(g);
+  return g;
+}
+
 // Columns [lo, ncols) of the basis that a projection covers.  arr == nullptr:
 // the whole basis.  Otherwise arr[j] is the first column for step j: >= 0 as
 // set by the step before (0 when partial reorthogonalisation asks for a full
@@ -309,6 +350,99 @@ __global__ void sum_double_kernel(const double* __restrict__ part, int nb, doubl
 // and a second pass over Q_{j-1} writes q_j and u_{j+1}: 2 passes per step
 // instead of CGS2's 4.
 
+// The reduction pass of a DCGS2 step in a form whose blocks all stay
+// resident: gridDim.x blocks (at most what the device holds at once) take
+// contiguous ranges of rows_pb rows (a multiple of 64) and walk them in tiles
+// of KT * RPT rows; each wave adds the butterfly sums of a column pair into
+// its own LDS accumulators.  Columns go in groups of cmax (u and w are read
+// again for each further group).  Output as dc_red_kernel<RPT, 2>:
+// part[b * P + c] = Q_c^H u, part[b * P + ncols + c] = Q_c^H w for the
+// window's columns, part[b * P + P - 2] = u^H u, P - 1 -> u^H w.
+template <int RPT>
+__global__ void __launch_bounds__(KT)
+    dc_red_rows_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ u,
+                       const cplx* __restrict__ w, cplx* __restrict__ part, const int* __restrict__ flag, Win win,
+                       int64_t rows_pb, int cmax) {
+  if (*flag) return;
+  extern __shared__ double acc[];  // [KT / 64][cmax][4]: Re, Im of Q^H u, then of Q^H w
+  __shared__ cplx sx[KT / 64][2];
+  const int P = 2 * ncols + 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_pb;
+  const int64_t r1 = min(n, r0 + rows_pb);
+  double* __restrict__ wacc = acc + (size_t)wv * cmax * 4;
+  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+  const int lo = win.lo();
+  // the first group always runs (it also accumulates u^H u and u^H w)
+  for (int cg = lo; cg == lo || cg < ncols; cg += cmax) {
+    const int cend = min(ncols, cg + cmax);
+    for (int i = lane; i < (cend - cg) * 4; i += 64) wacc[i] = 0.0;
+    __syncthreads();
+    for (int64_t t0 = r0; t0 < r1; t0 += (int64_t)KT * RPT) {
+      cplx ur[RPT], wr[RPT];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int64_t r = t0 + (int64_t)q * KT + threadIdx.x;
+        ur[q] = (r < r1) ? u[r] : cmake(0.0, 0.0);
+        wr[q] = (r < r1) ? w[r] : cmake(0.0, 0.0);
+        if (cg == lo) {
+          uu = cadd(uu, cdotmul(ur[q], ur[q]));
+          uw = cadd(uw, cdotmul(ur[q], wr[q]));
+        }
+      }
+      for (int c = cg; c < cend; c += 2) {
+        const bool two = c + 1 < cend;  // a lone last column repeats in the second slot, unused
+        const cplx* __restrict__ Vc = V + (int64_t)c * ld;
+        const cplx* __restrict__ Vd = two ? Vc + ld : Vc;
+        cplx a0 = cmake(0.0, 0.0), b0 = cmake(0.0, 0.0), a1 = cmake(0.0, 0.0), b1 = cmake(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int64_t r = t0 + (int64_t)q * KT + threadIdx.x;
+          if (r < r1) {
+            const cplx v0 = Vc[r], v1 = Vd[r];
+            a0 = cadd(a0, cdotmul(v0, ur[q]));
+            a1 = cadd(a1, cdotmul(v1, ur[q]));
+            b0 = cadd(b0, cdotmul(v0, wr[q]));
+            b1 = cadd(b1, cdotmul(v1, wr[q]));
+          }
+        }
+        const double d8[8] = {a0.x, a0.y, a1.x, a1.y, b0.x, b0.y, b1.x, b1.y};
+        const double g = wave_sum8(d8);
+        if ((lane & 7) == 0) {  // lane 8v holds d8[v]
+          const int v = lane >> 3, second = (v >> 1) & 1;
+          if (two || !second) wacc[(c - cg + second) * 4 + (v >> 2) * 2 + (v & 1)] += g;
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < cend - cg; t += KT) {
+      double sm[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < KT / 64; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sm[k] += acc[((size_t)q * cmax + t) * 4 + k];
+      part[(int64_t)blockIdx.x * P + cg + t] = cmake(sm[0], sm[1]);
+      part[(int64_t)blockIdx.x * P + ncols + cg + t] = cmake(sm[2], sm[3]);
+    }
+    __syncthreads();
+  }
+  uu = wave_sum(uu);
+  uw = wave_sum(uw);
+  if (lane == 0) {
+    sx[wv][0] = uu;
+    sx[wv][1] = uw;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cplx t = sx[0][0], t2 = sx[0][1];
+    for (int q = 1; q < KT / 64; ++q) {
+      t = cadd(t, sx[q][0]);
+      t2 = cadd(t2, sx[q][1]);
+    }
+    part[(int64_t)blockIdx.x * P + P - 2] = t;
+    part[(int64_t)blockIdx.x * P + P - 1] = t2;
+  }
+}
+
 // part[b * P + k]: k < ncols -> Q_k^H u; NR == 2: ncols + k -> Q_k^H w;
 // P - 2 -> u^H u, P - 1 -> u^H w (0 when NR == 1), P = NR * ncols + 2
 template <int RPT, int NR>
@@ -360,18 +494,19 @@ __global__ void __launch_bounds__(KT)
           }
         }
       }
-      a0 = wave_sum(a0);
-      a1 = wave_sum(a1);
       if (NR == 2) {
-        b0 = wave_sum(b0);
-        b1 = wave_sum(b1);
-      }
-      if (lane == 0) {
-        sh[wv][c - c0][0] = a0;
-        sh[wv][c + 1 - c0][0] = a1;
-        if (NR == 2) {
-          sh[wv][c - c0][NR - 1] = b0;
-          sh[wv][c + 1 - c0][NR - 1] = b1;
+        const double d8[8] = {a0.x, a0.y, a1.x, a1.y, b0.x, b0.y, b1.x, b1.y};
+        const double g = wave_sum8(d8);
+        if ((lane & 7) == 0) {  // lane 8v holds d8[v]
+          const int v = lane >> 3;
+          reinterpret_cast<double*>(&sh[wv][c - c0 + ((v >> 1) & 1)][v >> 2])[v & 1] = g;
+        }
+      } else {
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        if (lane == 0) {
+          sh[wv][c - c0][0] = a0;
+          sh[wv][c + 1 - c0][0] = a1;
         }
       }
     }
@@ -815,6 +950,34 @@ static void launch_gs_step(qd_krylov* kr, const KrylovDev& d, int j, cplx* w) {
 // V[:, s + add] (no product), else it runs as a full step and leaves
 // u_{s+add+1} pending.  Either way V[:, 0..s+add] and Hessenberg columns
 // 0..s+add-1 are final afterwards.
+// Launch shape of dc_red_rows_kernel<RT>: every block takes the same number
+// k of whole tiles, k the fewest that lets all blocks be resident at once
+// (occupancy at the kernel's LDS size, measured once).  Whole tiles because
+// each tile costs a full pass of the column loop (latency-bound when its
+// lanes are mostly idle).  The count never exceeds kr_nb(kr, 1), the size of
+// the partials buffer.
+struct RowsGrid {
+  static constexpr int kCmax = 160;  // columns per LDS group: 20 KB of accumulators
+  int blocks;
+  int64_t rows_pb;
+  size_t lds;
+};
+template <int RT>
+static RowsGrid rows_grid(int64_t n, int ncu) {
+  static int per_cu = 0;
+  const size_t lds = (size_t)(KT / 64) * RowsGrid::kCmax * 4 * sizeof(double);
+  if (per_cu <= 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dc_red_rows_kernel<RT>, KT, lds) != hipSuccess || b < 1) b = 1;
+    per_cu = b;
+  }
+  const int64_t tile = (int64_t)KT * RT;
+  const int64_t tiles = std::max<int64_t>(1, (n + tile - 1) / tile);
+  const int64_t slots = std::max<int64_t>(1, (int64_t)per_cu * ncu);
+  const int64_t k = (tiles + slots - 1) / slots;
+  return RowsGrid{(int)((tiles + k - 1) / k), k * tile, lds};
+}
+
 template <int RPT>
 static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int s, int add, bool cont, bool fin) {
   qd_ctx* ctx = kr->ctx;
@@ -827,6 +990,8 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
   const bool windowed = kr->method >= 2;
   const Pro pro = windowed ? Pro{d.lo, d.om, d.prm, kr->kry_dim + 2, PRO_W} : Pro{nullptr, nullptr, nullptr, 0, 0};
   auto win = [&](int j) { return windowed ? Win{d.lo, j, PRO_W} : Win{nullptr, 0, 0}; };
+  constexpr int RT = RPT >= 4 ? 4 : RPT;  // rows per thread of a reduction tile
+  const RowsGrid rg = rows_grid<RT>(n, ctx->num_cus);
   int rc;
   if (!cont) {
     // bootstrap: u_{s+1} = A q_s - Q_s (Q_s^H A q_s); H[0:s+1, s] = first pass
@@ -844,8 +1009,9 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
     if (rc) return rc;
     const int P = 2 * j + 2;
     const Win wj = win(j);
-    dc_red_kernel<RPT, 2><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.part, d.flag, wj);
-    reduce_cols_kernel<<<P, KT, 0, st>>>(d.part, nb, P, d.red, nullptr, 0, d.flag, 1 << 30, wj, j, 2);
+    dc_red_rows_kernel<RT><<<rg.blocks, KT, rg.lds, st>>>(n, kr->d_V, kr->ld, j, U, W, d.part, d.flag, wj,
+                                                          rg.rows_pb, RowsGrid::kCmax);
+    reduce_cols_kernel<<<P, KT, 0, st>>>(d.part, rg.blocks, P, d.red, nullptr, 0, d.flag, 1 << 30, wj, j, 2);
     dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 0, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS,
                                       pro);
     dc_update_kernel<RPT, false><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, W, d.coefA, d.coefB, d.scal, d.flag,
