@@ -272,6 +272,43 @@ int qd_krylov_reset(qd_krylov* kr);
  * Takes effect when a basis is started. */
 int qd_krylov_set_method(qd_krylov* kr, int method);
 
+/* ------------------------------------------- Hermitian two-vector Lanczos */
+/* Hermitian::LanczosSolver / Hermitian::getSolver(index, kry_dim)
+ * (sparse_algorithm.hpp:236-313): the three-term recurrence on two vectors,
+ * without reorthogonalisation, in two passes per solve.  Pass 1
+ * (getKrylovReprAd, :67-121) builds the tridiagonal: q_0 = psi / |psi|,
+ * a_i = q_i^H H q_i (complex), r = H q_i - a_i q_i - b_{i-1} q_{i-1},
+ * b_i = |r|; it stops when |r|^2 <= 1e-10 or once kry_dim + 1 coefficients
+ * a are held.  The tridiagonal (Re a, b) is solved in FP64 on the host.
+ * Pass 2 (getKrylovGS, :166-233) reruns the recurrence with Re a and
+ * accumulates sum_i c_i q_i.  Holds three device vectors of n entries. */
+typedef struct qd_hlanczos qd_hlanczos;
+int qd_hlanczos_create(qd_ctx* ctx, int64_t n, int kry_dim, qd_hlanczos** out);
+int qd_hlanczos_destroy(qd_hlanczos* lz);
+/* LanczosSolver::getGroundState(H, psi, tstp) (:285-300): d_psi is
+ * overwritten with sum_i U[i][0] q_i, U[:,0] the eigenvector of the lowest
+ * eigenvalue (not renormalised); E0 (host, may be NULL) receives that
+ * eigenvalue. */
+int qd_hlanczos_ground_state(qd_hlanczos* lz, qd_linop* op, double* d_psi, double* h_E0);
+/* LanczosSolver::evolve(H, psi, tstp, h) (:257-283): d_psi_out =
+ * sum_i c_i q_i with c = U diag(exp(-i lam h)) U^T e_0, i.e. the Krylov
+ * approximation of exp(-i h H) psi_in / |psi_in|.  d_psi_out may equal
+ * d_psi_in. */
+int qd_hlanczos_evolve(qd_hlanczos* lz, qd_linop* op, const double* d_psi_in, double* d_psi_out, double h);
+typedef struct {
+  int dim;         /* coefficients a of the last pass 1 (the Krylov matrix's order) */
+  double error;    /* LanczosSolver::error(): |r| after the last step of pass 1 */
+  int64_t n_spmv;  /* operator applications of the recurrences so far */
+} qd_hlanczos_info;
+int qd_hlanczos_get_info(qd_hlanczos* lz, qd_hlanczos_info* info);
+/* eigenvalues of the last Krylov matrix, ascending (LanczosSolver::storeE,
+ * :302-308); writes min(cap, dim) values */
+int qd_hlanczos_eigenvalues(qd_hlanczos* lz, double* h_evals, int cap, int* count);
+/* the last pass 1's a_i (interleaved complex, 2 doubles each) and b_i
+ * (b_i couples q_i and q_{i+1}; the last one is error()); min(cap, dim) of
+ * each */
+int qd_hlanczos_coefficients(qd_hlanczos* lz, double* h_alpha, double* h_beta, int cap, int* count);
+
 /* ------------------------------------------------ batched trajectories */
 /* Quantum-jump trajectories of a spinful fermion model (up spins on bits
  * 0..nsite-1, down spins on nsite..2nsite-1), the loop of

@@ -58,6 +58,9 @@ def lib():
     L.orc_tridiag_eigvals.argtypes = [ctypes.c_int, _VP, _VP, _VP]
     L.orc_tridiag_eigvec.restype = ctypes.c_int
     L.orc_tridiag_eigvec.argtypes = [ctypes.c_int, _VP, _VP, ctypes.c_double, _VP]
+    L.orc_hlanczos.restype = ctypes.c_int
+    L.orc_hlanczos.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _VP,
+                               _VP, _VP, _VP, _VP, _VP]
     L.orc_lanczos_gs.restype = ctypes.c_int
     L.orc_lanczos_gs.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP, _VP]
     L.orc_expm.restype = ctypes.c_int
@@ -196,6 +199,23 @@ def lanczos_gs(csc, k, psi0):
     if act < 0:
         raise RuntimeError("oracle lanczos failed")
     return {"E0": e0[0], "psi": out, "evals": ev[:act], "alpha": al[:act], "beta": be[:act], "act_dim": act}
+
+
+def hlanczos(csc, kry_dim, psi0, mode=0, h=0.0):
+    """Hermitian::LanczosSolver (sparse_algorithm.hpp:236-313): mode 0 getGroundState, mode 1 evolve by h."""
+    colptr, rowidx, vals = csc
+    n = len(colptr) - 1
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    out = np.empty(n, dtype=np.complex128)
+    al = np.zeros(kry_dim + 2, dtype=np.complex128)
+    be = np.zeros(kry_dim + 2)
+    ev = np.zeros(kry_dim + 2)
+    err = np.zeros(1)
+    dim = lib().orc_hlanczos(n, _p(colptr), _p(rowidx), _p(vals), kry_dim, mode, h, _p(psi0), _p(out), _p(al),
+                             _p(be), _p(ev), _p(err))
+    if dim < 0:
+        raise RuntimeError("oracle Hermitian Lanczos failed")
+    return {"psi": out, "alpha": al[:dim], "beta": be[:dim], "evals": ev[:dim], "err": err[0], "dim": dim}
 
 
 def expm(A):

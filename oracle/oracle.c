@@ -734,3 +734,160 @@ int orc_arnoldi_evolve(int64_t n, const int32_t* colptr, const int32_t* rowidx,
   free(E);
   return rc == 0 ? act : -1;
 }
+
+/* ======================================================================
+ * Hermitian::LanczosSolver (sparse_algorithm.hpp:236-313), as the reference
+ * runs it, two vectors v, w taking turns:
+ *   pass 1, getKrylovReprAd (:67-121): v = psi / sqrt(norm(psi));
+ *     a_0 = v^H H v; w = H v - a_0 v; n = |w|^2; w /= sqrt(n); then while
+ *     n > eps (1e-10, qudrip.hpp:18; n is the SQUARED norm): with x the
+ *     newer vector and y the other, a = x^H H x (complex), b = sqrt(n),
+ *     y = H x - a x - sqrt(n) y, n = |y|^2, y /= sqrt(n); the loop breaks
+ *     once the count of a exceeds kry_dim (idx > kry_dim, :104-106).
+ *     err = sqrt(n) (:119).
+ *   Krylov matrix: a on the diagonal, b beside it; the eigen-solver reads
+ *     the real diagonal.
+ *   mode 0, getGroundState (:285-300): coefficients = eigenvector of the
+ *     lowest eigenvalue (inverse iteration, orc_tridiag_eigvec);
+ *   mode 1, evolve (:257-283): coefficients = first column of
+ *     exp(-i h T), from orc_expm on the dense tridiagonal (the reference
+ *     goes through the eigen-decomposition; same matrix function).
+ *   pass 2, getKrylovGS (:166-233): rerun the recurrence from psi with
+ *     alpha = Re(x^H H x) and accumulate out = sum_i c_i q_i (the vector the
+ *     reference computes after the last term is not used and not formed).
+ * Outputs: psi_out; alpha (complex, dim); beta[i] = b_i (dim - 1 of them)
+ * then beta[dim - 1] = err; evals (dim, ascending); *err.  Returns dim (the
+ * count of a), or -1.
+ * ====================================================================== */
+int orc_hlanczos(int64_t n, const int32_t* colptr, const int32_t* rowidx, const double* vals,
+                 int kry_dim, int mode, double h, const double* psi_in, double* psi_out,
+                 double* alpha_out, double* beta_out, double* evals, double* err_out) {
+  if (kry_dim < 1 || n < 1) return -1;
+  oc* v = (oc*)calloc((size_t)n, sizeof(oc));
+  oc* w = (oc*)calloc((size_t)n, sizeof(oc));
+  oc* t = (oc*)malloc(sizeof(oc) * (size_t)n);
+  oc* a = (oc*)malloc(sizeof(oc) * (size_t)(kry_dim + 2));
+  double* b = (double*)malloc(sizeof(double) * (size_t)(kry_dim + 2));
+  int rc = -1;
+  double nrm = sqrt(v_norm2(n, (const oc*)psi_in));
+  /* ---- pass 1 */
+  for (int64_t r = 0; r < n; ++r) {
+    v[r].re = psi_in[2 * r] / nrm;
+    v[r].im = psi_in[2 * r + 1] / nrm;
+  }
+  int na = 0, nb = 0;
+  orc_spmv_csc(n, colptr, rowidx, vals, (const double*)v, (double*)t);
+  a[na++] = v_dot(n, v, t);
+  for (int64_t r = 0; r < n; ++r) w[r] = c_sub(t[r], c_mul(a[0], v[r]));
+  double nn = v_norm2(n, w);
+  for (int64_t r = 0; r < n; ++r) {
+    w[r].re /= sqrt(nn);
+    w[r].im /= sqrt(nn);
+  }
+  int use_v = 0, idx = 1;
+  while (nn > ORC_EPS) {
+    oc* x = use_v ? v : w; /* the newer vector */
+    oc* y = use_v ? w : v;
+    orc_spmv_csc(n, colptr, rowidx, vals, (const double*)x, (double*)t);
+    oc ai = v_dot(n, x, t);
+    a[na++] = ai;
+    b[nb++] = sqrt(nn);
+    for (int64_t r = 0; r < n; ++r) y[r] = c_sub(c_sub(t[r], c_mul(ai, x[r])), c_rmul(sqrt(nn), y[r]));
+    nn = v_norm2(n, y);
+    for (int64_t r = 0; r < n; ++r) {
+      y[r].re /= sqrt(nn);
+      y[r].im /= sqrt(nn);
+    }
+    use_v = !use_v;
+    ++idx;
+    if (idx > kry_dim) break;
+  }
+  const int dim = na;
+  const double err = sqrt(nn);
+  double* d = (double*)malloc(sizeof(double) * dim);
+  double* e = (double*)malloc(sizeof(double) * (dim > 1 ? dim : 1));
+  double* ev = (double*)malloc(sizeof(double) * dim);
+  oc* c = (oc*)calloc((size_t)dim, sizeof(oc));
+  for (int i = 0; i < dim; ++i) d[i] = a[i].re;
+  for (int i = 0; i + 1 < dim; ++i) e[i] = b[i];
+  if (orc_tridiag_eigvals(dim, d, e, ev) != 0) goto done;
+  if (mode == 0) {
+    double* z = (double*)malloc(sizeof(double) * dim);
+    int r2 = orc_tridiag_eigvec(dim, d, e, ev[0], z);
+    for (int i = 0; i < dim; ++i) c[i] = c_make(z[i], 0.0);
+    free(z);
+    if (r2 != 0) goto done;
+  } else {
+    oc* T = (oc*)calloc((size_t)dim * dim, sizeof(oc));
+    oc* E = (oc*)malloc(sizeof(oc) * (size_t)dim * dim);
+    for (int i = 0; i < dim; ++i) {
+      T[i + i * dim] = c_make(0.0, -h * d[i]); /* -i h T */
+      if (i + 1 < dim) {
+        T[(i + 1) + i * dim] = c_make(0.0, -h * e[i]);
+        T[i + (i + 1) * dim] = c_make(0.0, -h * e[i]);
+      }
+    }
+    int r2 = orc_expm(dim, (const double*)T, (double*)E);
+    for (int i = 0; i < dim; ++i) c[i] = E[i]; /* column 0 */
+    free(T);
+    free(E);
+    if (r2 != 0) goto done;
+  }
+  /* ---- pass 2 */
+  {
+    oc* out = (oc*)psi_out;
+    for (int64_t r = 0; r < n; ++r) {
+      v[r].re = psi_in[2 * r] / nrm;
+      v[r].im = psi_in[2 * r + 1] / nrm;
+    }
+    for (int64_t r = 0; r < n; ++r) out[r] = c_mul(c[0], v[r]);
+    if (dim > 1) {
+      orc_spmv_csc(n, colptr, rowidx, vals, (const double*)v, (double*)t);
+      double al = v_dot(n, v, t).re;
+      for (int64_t r = 0; r < n; ++r) w[r] = c_sub(t[r], c_rmul(al, v[r]));
+      nn = v_norm2(n, w);
+      for (int64_t r = 0; r < n; ++r) {
+        w[r].re /= sqrt(nn);
+        w[r].im /= sqrt(nn);
+      }
+    }
+    for (int i = 1; i < dim; ++i) {
+      oc* x = (i % 2) ? w : v; /* odd: w is the newer vector */
+      oc* y = (i % 2) ? v : w;
+      for (int64_t r = 0; r < n; ++r) out[r] = c_add(out[r], c_mul(c[i], x[r]));
+      if (i + 1 == dim) break;
+      orc_spmv_csc(n, colptr, rowidx, vals, (const double*)x, (double*)t);
+      double al = v_dot(n, x, t).re;
+      for (int64_t r = 0; r < n; ++r) y[r] = c_sub(c_sub(t[r], c_rmul(al, x[r])), c_rmul(sqrt(nn), y[r]));
+      nn = v_norm2(n, y);
+      for (int64_t r = 0; r < n; ++r) {
+        y[r].re /= sqrt(nn);
+        y[r].im /= sqrt(nn);
+      }
+    }
+  }
+  if (alpha_out)
+    for (int i = 0; i < dim; ++i) {
+      alpha_out[2 * i] = a[i].re;
+      alpha_out[2 * i + 1] = a[i].im;
+    }
+  if (beta_out) {
+    for (int i = 0; i + 1 < dim; ++i) beta_out[i] = b[i];
+    beta_out[dim - 1] = err;
+  }
+  if (evals)
+    for (int i = 0; i < dim; ++i) evals[i] = ev[i];
+  if (err_out) *err_out = err;
+  rc = dim;
+done:
+  free(v);
+  free(w);
+  free(t);
+  free(a);
+  free(b);
+  free(d);
+  free(e);
+  free(ev);
+  free(c);
+  return rc;
+}

@@ -76,6 +76,10 @@ class qd_krylov_info(ctypes.Structure):
                 ("ritz_gap", ctypes.c_double), ("full_steps", ctypes.c_int)]
 
 
+class qd_hlanczos_info(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("error", ctypes.c_double), ("n_spmv", ctypes.c_int64)]
+
+
 VP = ctypes.c_void_p
 I = ctypes.c_int
 I64 = ctypes.c_int64
@@ -133,6 +137,13 @@ SIGNATURES = {
     "qd_krylov_converge_ev": (I, [VP, I, ctypes.c_double, ctypes.c_double]),
     "qd_krylov_reset": (I, [VP]),
     "qd_krylov_set_method": (I, [VP, I]),
+    "qd_hlanczos_create": (I, [VP, I64, I, PP]),
+    "qd_hlanczos_destroy": (I, [VP]),
+    "qd_hlanczos_ground_state": (I, [VP, VP, VP, DP]),
+    "qd_hlanczos_evolve": (I, [VP, VP, VP, VP, ctypes.c_double]),
+    "qd_hlanczos_get_info": (I, [VP, ctypes.POINTER(qd_hlanczos_info)]),
+    "qd_hlanczos_eigenvalues": (I, [VP, DP, I, ctypes.POINTER(I)]),
+    "qd_hlanczos_coefficients": (I, [VP, DP, DP, I, ctypes.POINTER(I)]),
     "qd_traj_run": (I, [VP, I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, I, I, VP,
                         ctypes.POINTER(qd_jump), I, I, I, ctypes.c_double, I, ctypes.c_uint64, VP, VP, VP,
                         ctypes.POINTER(qd_traj_stats)]),

@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "qd_internal.hpp"
+#include "wave.hpp"
 
 int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale, int xscale_idx,
                    const cplx* d_diag, cplx shift);
@@ -32,28 +33,6 @@ namespace {
 
 constexpr int KT = 256;  // threads per block for the tall-skinny kernels
 
-// Wavefront sums with DPP lane moves (vector ALU, no LDS round trips):
-// quad xor 1 and 2, half-row and row mirrors (every lane of a 16-lane row
-// holds the row's sum), then row broadcasts 15 and 31 accumulate the rows
-// into lane 63, which is broadcast to every lane.
-template <int CTRL, int ROW_MASK>
-__device__ inline double dpp_move(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
-__device__ inline double wave_sum_d(double v) {
-  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
-  v += dpp_move<0x140, 0xF>(v);  // row_mirror
-  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-  return __hiloint2double(hi, lo);
-}
-__device__ inline cplx wave_sum(cplx s) { return cmake(wave_sum_d(s.x), wave_sum_d(s.y)); }
 
 // gfx950 cross-half swaps: permlane32 trades lanes 32-63 of a with lanes
 // 0-31 of b; permlane16 trades the odd 16-lane rows of a with the even rows
@@ -355,7 +334,7 @@ __global__ void sum_double_kernel(const double* __restrict__ part, int nb, doubl
 // contiguous ranges of rows_pb rows (a multiple of 64) and walk them in tiles
 // of KT * RPT rows; each wave adds the butterfly sums of a column pair into
 // its own LDS accumulators.  Columns go in groups of cmax (u and w are read
-// again for each further group).  Output as dc_red_kernel<RPT, 2>:
+// again for each further group).  Output:
 // part[b * P + c] = Q_c^H u, part[b * P + ncols + c] = Q_c^H w for the
 // window's columns, part[b * P + P - 2] = u^H u, P - 1 -> u^H w.
 template <int RPT>
@@ -443,114 +422,79 @@ __global__ void __launch_bounds__(KT)
   }
 }
 
-// part[b * P + k]: k < ncols -> Q_k^H u; NR == 2: ncols + k -> Q_k^H w;
-// P - 2 -> u^H u, P - 1 -> u^H w (0 when NR == 1), P = NR * ncols + 2
-template <int RPT, int NR>
+// The reduction pass of the step that finalises a basis (one vector u):
+// part[b * P + k] = Q_k^H u for the window's columns k < ncols,
+// part[b * P + P - 2] = u^H u, P - 1 -> 0; P = ncols + 2; one tile of KT * RPT
+// rows per block.
+template <int RPT>
 __global__ void __launch_bounds__(KT)
     dc_red_kernel(int64_t n, const cplx* __restrict__ V, int64_t ld, int ncols, const cplx* __restrict__ u,
-                  const cplx* __restrict__ w, cplx* __restrict__ part, const int* __restrict__ flag,
-                  Win win) {
+                  cplx* __restrict__ part, const int* __restrict__ flag, Win win) {
   if (*flag) return;
-  __shared__ cplx sh[KT / 64][64][NR];
-  __shared__ cplx sx[KT / 64][2];
-  const int P = NR * ncols + 2;
+  __shared__ cplx sh[KT / 64][64];
+  __shared__ cplx sx[KT / 64];
+  const int P = ncols + 2;
   const int64_t base = (int64_t)blockIdx.x * KT * RPT;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  cplx ur[RPT], wr[RPT];
-  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+  cplx ur[RPT];
+  cplx uu = cmake(0.0, 0.0);
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     int64_t r = base + (int64_t)q * KT + threadIdx.x;
     ur[q] = (r < n) ? u[r] : cmake(0.0, 0.0);
-    wr[q] = (NR == 2 && r < n) ? w[r] : cmake(0.0, 0.0);
     uu = cadd(uu, cdotmul(ur[q], ur[q]));
-    if (NR == 2) uw = cadd(uw, cdotmul(ur[q], wr[q]));
   }
   uu = wave_sum(uu);
-  uw = wave_sum(uw);
-  if (lane == 0) {
-    sx[wv][0] = uu;
-    sx[wv][1] = uw;
-  }
+  if (lane == 0) sx[wv] = uu;
   for (int c0 = win.lo(); c0 < ncols; c0 += 64) {
     const int cend = min(ncols, c0 + 64);
     int c = c0;
     // two columns at a time: twice the loads in flight per wave and two
-    // independent reduction chains (each wave's columns run in sequence)
+    // independent reduction chains
     for (; c + 1 < cend; c += 2) {
       const cplx* __restrict__ Vc = V + (int64_t)c * ld;
       const cplx* __restrict__ Vd = Vc + ld;
-      cplx a0 = cmake(0.0, 0.0), b0 = cmake(0.0, 0.0), a1 = cmake(0.0, 0.0), b1 = cmake(0.0, 0.0);
+      cplx a0 = cmake(0.0, 0.0), a1 = cmake(0.0, 0.0);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         int64_t r = base + (int64_t)q * KT + threadIdx.x;
         if (r < n) {
-          const cplx v0 = Vc[r], v1 = Vd[r];
-          a0 = cadd(a0, cdotmul(v0, ur[q]));
-          a1 = cadd(a1, cdotmul(v1, ur[q]));
-          if (NR == 2) {
-            b0 = cadd(b0, cdotmul(v0, wr[q]));
-            b1 = cadd(b1, cdotmul(v1, wr[q]));
-          }
+          a0 = cadd(a0, cdotmul(Vc[r], ur[q]));
+          a1 = cadd(a1, cdotmul(Vd[r], ur[q]));
         }
       }
-      if (NR == 2) {
-        const double d8[8] = {a0.x, a0.y, a1.x, a1.y, b0.x, b0.y, b1.x, b1.y};
-        const double g = wave_sum8(d8);
-        if ((lane & 7) == 0) {  // lane 8v holds d8[v]
-          const int v = lane >> 3;
-          reinterpret_cast<double*>(&sh[wv][c - c0 + ((v >> 1) & 1)][v >> 2])[v & 1] = g;
-        }
-      } else {
-        a0 = wave_sum(a0);
-        a1 = wave_sum(a1);
-        if (lane == 0) {
-          sh[wv][c - c0][0] = a0;
-          sh[wv][c + 1 - c0][0] = a1;
-        }
+      a0 = wave_sum(a0);
+      a1 = wave_sum(a1);
+      if (lane == 0) {
+        sh[wv][c - c0] = a0;
+        sh[wv][c + 1 - c0] = a1;
       }
     }
     for (; c < cend; ++c) {
       const cplx* __restrict__ Vc = V + (int64_t)c * ld;
-      cplx a = cmake(0.0, 0.0), bb = cmake(0.0, 0.0);
+      cplx a = cmake(0.0, 0.0);
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         int64_t r = base + (int64_t)q * KT + threadIdx.x;
-        if (r < n) {
-          cplx v = Vc[r];
-          a = cadd(a, cdotmul(v, ur[q]));
-          if (NR == 2) bb = cadd(bb, cdotmul(v, wr[q]));
-        }
+        if (r < n) a = cadd(a, cdotmul(Vc[r], ur[q]));
       }
       a = wave_sum(a);
-      if (NR == 2) bb = wave_sum(bb);
-      if (lane == 0) {
-        sh[wv][c - c0][0] = a;
-        if (NR == 2) sh[wv][c - c0][NR - 1] = bb;
-      }
+      if (lane == 0) sh[wv][c - c0] = a;
     }
     __syncthreads();
     if ((int)threadIdx.x < cend - c0) {
       const int c = threadIdx.x;
-      cplx t = sh[0][c][0];
-      for (int q = 1; q < KT / 64; ++q) t = cadd(t, sh[q][c][0]);
+      cplx t = sh[0][c];
+      for (int q = 1; q < KT / 64; ++q) t = cadd(t, sh[q][c]);
       part[(int64_t)blockIdx.x * P + c0 + c] = t;
-      if (NR == 2) {
-        cplx t2 = sh[0][c][NR - 1];
-        for (int q = 1; q < KT / 64; ++q) t2 = cadd(t2, sh[q][c][NR - 1]);
-        part[(int64_t)blockIdx.x * P + ncols + c0 + c] = t2;
-      }
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    cplx t = sx[0][0], t2 = sx[0][1];
-    for (int q = 1; q < KT / 64; ++q) {
-      t = cadd(t, sx[q][0]);
-      t2 = cadd(t2, sx[q][1]);
-    }
+    cplx t = sx[0];
+    for (int q = 1; q < KT / 64; ++q) t = cadd(t, sx[q]);
     part[(int64_t)blockIdx.x * P + P - 2] = t;
-    part[(int64_t)blockIdx.x * P + P - 1] = t2;
+    part[(int64_t)blockIdx.x * P + P - 1] = cmake(0.0, 0.0);
   }
 }
 
@@ -1024,7 +968,7 @@ static int dc_expand_launch(qd_krylov* kr, qd_linop* op, const KrylovDev& d, int
   // finalise: reorthogonalise the pending vector into V[:, s + add]
   const int j = s + add;
   const Win wj = win(j);
-  dc_red_kernel<RPT, 1><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, nullptr, d.part, d.flag, wj);
+  dc_red_kernel<RPT><<<nb, KT, 0, st>>>(n, kr->d_V, kr->ld, j, U, d.part, d.flag, wj);
   reduce_cols_kernel<<<j + 2, KT, 0, st>>>(d.part, nb, j + 2, d.red, nullptr, 0, d.flag, 1 << 30, wj, j, 1);
   dc_small_kernel<<<1, KT, 0, st>>>(d.red, j, 1, d.KR, ldk, d.coefA, d.coefB, d.scal, d.flag, d.bd_step, QD_LEAF_EPS,
                                     pro);

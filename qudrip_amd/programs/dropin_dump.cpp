@@ -2,9 +2,14 @@
 // its term program and `>>` matrix as raw binary files, so the tests can
 // compare them with the Python mirror and the CPU oracle.
 //   usage: dropin_dump <xy|xxz|hubchain|lazy> <N> <out-prefix> [Jxy Jz | t U]
+//          [herm <kry_dim> <h>]
 // Files: <prefix>.terms (qd_term[]), <prefix>.ops (qd_op[]), <prefix>.outer,
 // <prefix>.inner (int32), <prefix>.vals (complex128), <prefix>.y (H x for the
-// seeded x of seed 7; for "lazy" computed as psi2 = ham * psi1).
+// seeded x of seed 7; for "lazy" computed as psi2 = ham * psi1).  With
+// "herm k h" the two-vector solver (Hermitian::getSolver(sub, k)) then
+// evolves x by h into <prefix>.ev and replaces x by its ground state,
+// <prefix>.gs, with the Ritz values in <prefix>.E (storeE) and
+// "error = ..." on stdout.
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -78,6 +83,17 @@ int main(int argc, char** argv) {
   }
   dump(out + ".x", psi.host(0, false), sub.range());
   dump(out + ".y", psi.host(1, false), sub.range());
+  if (argc > 8 && std::string(argv[6]) == "herm" && model != "lazy") {
+    auto H = ham >> sub;
+    auto solver = Hermitian::getSolver(sub, std::stoi(argv[7]));
+    solver.evolve(H, psi, 0, std::stod(argv[8]));
+    dump(out + ".ev", psi.host(1, false), sub.range());
+    solver.getGroundState(H, psi, 0);
+    dump(out + ".gs", psi.host(0, false), sub.range());
+    solver.storeE(out + ".E");
+    std::cout.precision(17);
+    std::cout << "error = " << solver.error() << std::endl;
+  }
   std::cout << "dumped " << terms.size() << " terms, n = " << sub.range() << std::endl;
   return 0;
 }

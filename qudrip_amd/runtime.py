@@ -250,6 +250,59 @@ class Krylov:
             self.ptr = None
 
 
+class HermitianLanczos:
+    """Hermitian::getSolver / Hermitian::LanczosSolver (sparse_algorithm.hpp:236-313):
+    the two-vector Lanczos without reorthogonalisation, two passes per solve."""
+
+    def __init__(self, ctx, n, kry_dim):
+        self.ctx = ctx
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.qd_hlanczos_create(ctx.ptr, n, kry_dim, ctypes.byref(p)), "hlanczos")
+        self.ptr = p
+        self.n = n
+        self.kry_dim = kry_dim
+
+    def ground_state(self, op, psi):
+        e0 = ctypes.c_double()
+        self.ctx.check(self.ctx.lib.qd_hlanczos_ground_state(self.ptr, op.ptr, ctypes.c_void_p(psi.data_ptr()),
+                                                             ctypes.byref(e0)), "hlanczos ground_state")
+        return e0.value
+
+    def evolve(self, op, psi_in, psi_out, h):
+        self.ctx.check(self.ctx.lib.qd_hlanczos_evolve(self.ptr, op.ptr, ctypes.c_void_p(psi_in.data_ptr()),
+                                                       ctypes.c_void_p(psi_out.data_ptr()), float(h)),
+                       "hlanczos evolve")
+
+    def info(self):
+        inf = abi.qd_hlanczos_info()
+        self.ctx.check(self.ctx.lib.qd_hlanczos_get_info(self.ptr, ctypes.byref(inf)), "hlanczos info")
+        return {"dim": inf.dim, "error": inf.error, "n_spmv": inf.n_spmv}
+
+    def eigenvalues(self):
+        cap = self.kry_dim + 2
+        buf = (ctypes.c_double * cap)()
+        cnt = ctypes.c_int()
+        self.ctx.check(self.ctx.lib.qd_hlanczos_eigenvalues(self.ptr, buf, cap, ctypes.byref(cnt)),
+                       "hlanczos eigenvalues")
+        return np.array(buf[:cnt.value])
+
+    def coefficients(self):
+        """(alpha complex[dim], beta[dim]); beta[dim-1] is error()."""
+        cap = self.kry_dim + 2
+        al = np.zeros(cap, dtype=np.complex128)
+        be = np.zeros(cap)
+        cnt = ctypes.c_int()
+        self.ctx.check(self.ctx.lib.qd_hlanczos_coefficients(
+            self.ptr, al.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            be.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap, ctypes.byref(cnt)), "hlanczos coefficients")
+        return al[:cnt.value], be[:cnt.value]
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.qd_hlanczos_destroy(self.ptr)
+            self.ptr = None
+
+
 def jump_channels(nsite, dephase_rate, loss_rate):
     """Dephasing sqrt(g) n_m and loss sqrt(g) c_m on every mode m (0 rates omitted)."""
     ch = []

@@ -266,3 +266,89 @@ def test_arnoldi_evolve_matches_dense_expm():
     out = oracle.arnoldi_evolve(csc, 8, 0.05, psi)
     ref = scipy.linalg.expm(-1j * 0.05 * H) @ psi
     assert np.allclose(out, ref, atol=1e-12)
+
+
+# -------------------------------------------- Hermitian::LanczosSolver (two vectors)
+def _two_site_hubbard(U):
+    idx, ham = Q.hubbard(2, Q.hubbard_bonds_chain(2, False), 1.0, U)
+    b = oracle.basis_scan(Q.hubbard_sector(idx, 2, 1, 1))
+    return oracle.build_csc(b, ham)
+
+
+def test_hlanczos_two_site_hubbard_closed_form():
+    """n = 4, four distinct levels: the recurrence ends on |r|^2 <= 1e-10
+    after exactly four coefficients and its Ritz values are the spectrum
+    (U -+ sqrt(U^2 + 16 t^2)) / 2, 0, U."""
+    U = 4.0
+    csc = _two_site_hubbard(U)
+    res = oracle.hlanczos(csc, 50, oracle.seeded_state(4, 9))
+    ref = sorted([(U - np.sqrt(U * U + 16)) / 2, 0.0, U, (U + np.sqrt(U * U + 16)) / 2])
+    assert res["dim"] == 4 and res["err"] ** 2 <= 1e-10
+    assert np.allclose(res["evals"], ref, atol=1e-12)
+    assert np.max(np.abs(res["alpha"].imag)) < 1e-14
+    g = res["psi"]
+    H = oracle.csc_to_dense(csc)
+    assert abs(np.linalg.norm(g) - 1) < 1e-12
+    assert np.linalg.norm(H @ g - ref[0] * g) < 1e-11
+
+
+def test_hlanczos_two_level_closed_form():
+    """XY chain N = 2, one up spin: H = sigma_x.  From e_0: a = (0, 0), b_0 = 1,
+    then |r| = 0 ends it; evolve gives cos h e_0 - i sin h e_1 (the sign of
+    the exponent), the ground state (e_0 - e_1) / sqrt(2) up to phase."""
+    idx, ham = Q.xy_chain(2, 1.0, pbc=False)
+    b = oracle.basis_scan(Q.Constrain(idx, Q.getNset(idx) == 1))
+    csc = oracle.build_csc(b, ham)
+    e0 = np.array([1.0, 0.0], dtype=np.complex128)
+    h = 0.3
+    ev = oracle.hlanczos(csc, 10, e0, mode=1, h=h)
+    assert ev["dim"] == 2 and ev["err"] == 0.0
+    assert np.allclose(ev["alpha"], [0, 0], atol=1e-15) and abs(ev["beta"][0] - 1) < 1e-15
+    assert np.allclose(ev["psi"], [np.cos(h), -1j * np.sin(h)], atol=1e-14)
+    gs = oracle.hlanczos(csc, 10, 3.0 * e0)
+    assert np.allclose(gs["evals"], [-1, 1], atol=1e-14)
+    ph = gs["psi"][0] / abs(gs["psi"][0])
+    assert np.allclose(gs["psi"] / ph, np.array([1, -1]) / np.sqrt(2), atol=1e-14)
+
+
+def test_hlanczos_stops_on_squared_norm_and_kry_dim():
+    """A start 1e-6 off an eigenvector leaves |r| ~ 1e-6: |r|^2 <= 1e-10
+    ends the recurrence after one coefficient (a test on |r| would not);
+    otherwise it holds kry_dim + 1 coefficients."""
+    idx, ham = Q.xxz_chain(8, 1.0, 0.7, pbc=True)
+    b = oracle.basis_scan(Q.Constrain(idx, Q.getNset(idx) == 4))
+    csc = oracle.build_csc(b, ham)
+    w, z = np.linalg.eigh(oracle.csc_to_dense(csc))
+    psi = (z[:, 0] + 1e-6 * z[:, -1]).astype(np.complex128)
+    r1 = oracle.hlanczos(csc, 30, psi)
+    assert r1["dim"] == 1 and 1e-7 < r1["err"] < 1e-5
+    assert abs(r1["evals"][0] - w[0]) < 1e-9
+    r2 = oracle.hlanczos(csc, 7, oracle.seeded_state(len(b), 3))
+    assert r2["dim"] == 8 and r2["err"] > 1e-3
+
+
+def test_hlanczos_ground_state_xy16_closed_form():
+    idx, ham = Q.xy_chain(16, pbc=False)
+    b = oracle.basis_scan(Q.Constrain(idx, Q.getNset(idx) == 8))
+    csc = oracle.build_csc(b, ham)
+    res = oracle.hlanczos(csc, 100, oracle.seeded_state(len(b), 1))
+    e_ref = P.xy_free_fermion_e0(16, 8)
+    assert res["dim"] == 101
+    assert abs(res["evals"][0] - e_ref) / abs(e_ref) < 1e-10
+    g = res["psi"]
+    ev = oracle.eval_expect(csc, g)
+    assert abs(ev.real / np.vdot(g, g).real - e_ref) / abs(e_ref) < 1e-9
+
+
+def test_hlanczos_evolve_matches_dense_expm():
+    """Short step, kry_dim 8 (nine coefficients): exp(-i h H) psi / |psi|
+    (the input norm is not carried)."""
+    L = 3
+    idx, ham = Q.hubbard(L, Q.hubbard_bonds_chain(L, False), 1.0, 2.0)
+    b = oracle.basis_scan(Q.hubbard_sector(idx, L, 2, 1))
+    csc = oracle.build_csc(b, ham)
+    H = oracle.csc_to_dense(csc)
+    psi = 2.5 * oracle.seeded_state(len(b), 4)
+    out = oracle.hlanczos(csc, 8, psi, mode=1, h=0.05)["psi"]
+    ref = scipy.linalg.expm(-1j * 0.05 * H) @ (psi / np.linalg.norm(psi))
+    assert np.allclose(out, ref, atol=1e-12)
