@@ -183,6 +183,29 @@ def main():
         }
         print(f"[bench] lanczos k={args.krylov}: E0={e0:.14f} solve {solve_s:.3f} s", file=sys.stderr, flush=True)
         kr.close()
+        # Hermitian::getSolver's two-vector Lanczos (no reorthogonalisation,
+        # two passes of the recurrence) on the same operator and start
+        from qudrip_amd.runtime import HermitianLanczos
+        hl = HermitianLanczos(ctx, n, args.krylov)
+        psi = ctx.seeded_state(n, 2)
+        hl.ground_state(op, psi)  # warm-up (first launches of its kernels)
+        psi = ctx.seeded_state(n, 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0h = hl.ground_state(op, psi)
+        torch.cuda.synchronize()
+        hs = time.perf_counter() - t0
+        inf = hl.info()
+        steps = 2 * inf["dim"] - 1  # recurrence steps over both passes (each one product)
+        result["hermitian_lanczos"] = {
+            "k": args.krylov, "E0": e0h, "rel_err": (abs(e0h - ref) / abs(ref)) if ref else None,
+            "coefficients": inf["dim"], "solve_s": round(hs, 4), "ms_per_step": round(hs * 1e3 / steps, 4),
+            "bytes_per_step": B + 6 * 16 * n,
+            "step_GBs": round((B + 6 * 16 * n) * steps / hs / 1e9, 1),
+            "reorth": "none (Hermitian::LanczosSolver, two passes)"}
+        print(f"[bench] hermitian lanczos k={args.krylov}: E0={e0h:.14f} solve {hs:.3f} s "
+              f"({hs * 1e3 / steps:.3f} ms/step)", file=sys.stderr, flush=True)
+        hl.close()
 
     if not args.no_quench:
         from qudrip_amd import workloads as WL
