@@ -3,6 +3,7 @@
 // Same call surface as the reference:
 //   getQbits(N)                         index.hpp:397 (QbitsIndex :177-191)
 //   getNset / getNup / getNdo, `cq = v`  conservation.hpp:89-115, 233-246
+//   getNd / getNh                         conservation.hpp:117-151, 248-256
 //   Constrain(idx, cq = v, ...)          conservation.hpp:212-231 -> SubIndex
 //   cursor protocol idx[i], ++idx, (idxv_type)idx, idx.range()
 // The sector table is produced on the GPU (qd_basis_create); the host keeps
@@ -69,11 +70,39 @@ class Nset {
   }
   uint64_t mask() const { return mask_; }
   long long val() const { return val_; }
+  qd_quantity quantity() const { return qd_quantity{mask_, 0, 0, 0, (int64_t)val_}; }
+};
+
+// Doubly occupied (hole = 0) or empty (hole = 1) sites of a spinful index
+// (Nd, conservation.hpp:117-151): popcount(((x >> Nsite) ^ m) & (x ^ m)),
+// m = bits 0..Nsite-1 when hole, else 0.
+template <class T1 = idx_size_t>
+class Nd {
+  int nsite_;
+  uint64_t flip_ = 0;
+  long long val_ = 0;
+
+ public:
+  using quant_type = T1;
+  struct is_cq {};
+  Nd(const QbitsIndex<>& index, int Nsite, bool hole = 0) : nsite_(Nsite) {
+    (void)index;
+    for (int i = 0; i < Nsite && i < 64; ++i)
+      if (hole) flip_ |= (1ull << i);
+  }
+  Nd& operator=(quant_type v) {
+    val_ = (long long)v;
+    return *this;
+  }
+  long long val() const { return val_; }
+  qd_quantity quantity() const { return qd_quantity{~0ull, flip_, nsite_, 0, (int64_t)val_}; }
 };
 
 inline Nset<> getNset(const QbitsIndex<>& index) { return Nset<>(index, -1, -1); }
 inline Nset<> getNup(const QbitsIndex<>& index, int Nsite) { return Nset<>(index, 0, Nsite - 1); }
 inline Nset<> getNdo(const QbitsIndex<>& index, int Nsite) { return Nset<>(index, Nsite, 2 * Nsite - 1); }
+inline Nd<> getNd(const QbitsIndex<>& index, int Nsite) { return Nd<>(index, Nsite, 0); }  // :248-251
+inline Nd<> getNh(const QbitsIndex<>& index, int Nsite) { return Nd<>(index, Nsite, 1); }  // :253-256
 
 namespace detail {
 // device sector table shared by every object built on it
@@ -92,10 +121,10 @@ struct BasisHolder {
     return host;
   }
 };
-inline std::shared_ptr<BasisHolder> make_basis(int nbits, const std::vector<qd_constraint>& cons) {
+inline std::shared_ptr<BasisHolder> make_basis(int nbits, const std::vector<qd_quantity>& q) {
   auto h = std::make_shared<BasisHolder>();
   h->nbits = nbits;
-  check(qd_basis_create(ctx(), nbits, cons.data(), (int)cons.size(), &h->b), "qd_basis_create");
+  check(qd_basis_create_q(ctx(), nbits, q.data(), (int)q.size(), &h->b), "qd_basis_create_q");
   return h;
 }
 }  // namespace detail
@@ -143,9 +172,9 @@ class SubIndex {
 template <class... CQs>
 SubIndex<> Constrain(QbitsIndex<>& index, CQs&&... cqs) {
   std::cout << "Constrain: original range " << index.range() << std::endl;  // :214
-  std::vector<qd_constraint> cons;
-  (cons.push_back(qd_constraint{cqs.mask(), (int64_t)cqs.val()}), ...);
-  SubIndex<> sub(index, detail::make_basis(index.nbits(), cons));
+  std::vector<qd_quantity> q;
+  (q.push_back(cqs.quantity()), ...);
+  SubIndex<> sub(index, detail::make_basis(index.nbits(), q));
   sub[0];  // :223
   return sub;
 }

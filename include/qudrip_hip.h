@@ -122,6 +122,26 @@ int qd_memcpy_d2h(qd_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
  * (disjoint contiguous masks) or by a predicate scan (any other masks). */
 int qd_basis_create(qd_ctx* ctx, int nbits, const qd_constraint* cons, int ncons,
                     qd_basis** out);
+/* A conserved quantity in the form shared by the reference's Nset and Nd
+ * classes: value(x) = popcount(((x >> shift) ^ flip) & (x ^ flip) & sel).
+ *   Nset (conservation.hpp:89-115): shift 0, flip 0, sel = its mask;
+ *   Nd   (conservation.hpp:117-151; getNd :248-251): shift Nsite, flip 0,
+ *        sel all bits: the doubly occupied sites;
+ *   Nh   (getNh :253-256): shift Nsite, flip bits 0..Nsite-1, sel all bits:
+ *        the empty sites. */
+typedef struct {
+  uint64_t sel;
+  uint64_t flip;
+  int32_t shift; /* 0..63 */
+  int32_t reserved;
+  int64_t count;
+} qd_quantity;
+/* Constrain(index, cq = v, ...) (conservation.hpp:212-231) over such
+ * quantities (at most 16).  Plain Nset lists take the qd_basis_create path;
+ * with Nd / Nh the sector of the Nset part is enumerated (or, if that part is
+ * not a disjoint set of bit ranges, all 2^nbits states are scanned, nbits <=
+ * 40) and filtered in order; such a basis ranks by binary search. */
+int qd_basis_create_q(qd_ctx* ctx, int nbits, const qd_quantity* q, int nq, qd_basis** out);
 int qd_basis_destroy(qd_basis* basis);
 /* SubIndex::range() (index.hpp:357). */
 int64_t qd_basis_size(const qd_basis* basis);

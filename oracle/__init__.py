@@ -34,6 +34,8 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     L.orc_basis_scan.restype = ctypes.c_int64
     L.orc_basis_scan.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int64]
+    L.orc_basis_scan_q.restype = ctypes.c_int64
+    L.orc_basis_scan_q.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int64]
     L.orc_rank.restype = ctypes.c_int64
     L.orc_rank.argtypes = [_VP, ctypes.c_int64, ctypes.c_uint64]
     L.orc_binom.restype = ctypes.c_uint64
@@ -84,6 +86,12 @@ def _p(a):
 def basis_scan(sector):
     """Constrain / LooseConstrain (conservation.hpp:212-231): ascending scan."""
     from qudrip_amd.abi import qd_constraint
+    if not all(c.plain for c in sector.constraints):
+        arr, nq = sector.ctypes_quantities()
+        n = lib().orc_basis_scan_q(sector.nbits, arr, nq, None, 0)
+        out = np.empty(max(n, 1), dtype=np.uint64)
+        lib().orc_basis_scan_q(sector.nbits, arr, nq, _p(out), n)
+        return out[:n]
     cons = (qd_constraint * max(1, len(sector.constraints)))()
     for i, c in enumerate(sector.constraints):
         cons[i].mask = c.mask

@@ -81,6 +81,33 @@ int64_t orc_basis_scan(int nbits, const qd_constraint* cons, int ncons, uint64_t
   return count;
 }
 
+/* Constrain over Nset and Nd quantities (conservation.hpp:103,129-132,
+ * 212-231): Nd::eval is popcount(((x >> Nsite) ^ mask) & (x ^ mask)) with
+ * mask = bits 0..Nsite-1 for getNh, 0 for getNd; an Nset is the same form
+ * with shift 0, flip 0 and sel = its mask.  q[k] = {sel, flip, shift, count}
+ * as qd_quantity. */
+typedef struct {
+  uint64_t sel, flip;
+  int32_t shift, reserved;
+  int64_t count;
+} orc_quantity;
+int64_t orc_basis_scan_q(int nbits, const orc_quantity* q, int nq, uint64_t* out, int64_t cap) {
+  uint64_t range = (nbits >= 64) ? 0 : (1ull << nbits);
+  int64_t count = 0;
+  for (uint64_t x = 0; x < range; ++x) {
+    int ok = 1;
+    for (int c = 0; c < nq && ok; ++c) {
+      const uint64_t f = q[c].flip;
+      if ((int64_t)popc(((x >> q[c].shift) ^ f) & (x ^ f) & q[c].sel) != q[c].count) ok = 0;
+    }
+    if (ok) {
+      if (count < cap && out) out[count] = x;
+      ++count;
+    }
+  }
+  return count;
+}
+
 /* index.hpp:331-337: inv_mapping lookup; a miss returns range() == n.  The
  * mapping is ascending, so a binary search finds the same entry. */
 int64_t orc_rank(const uint64_t* basis, int64_t n, uint64_t x) {

@@ -76,6 +76,11 @@ class qd_krylov_info(ctypes.Structure):
                 ("ritz_gap", ctypes.c_double), ("full_steps", ctypes.c_int)]
 
 
+class qd_quantity(ctypes.Structure):
+    _fields_ = [("sel", ctypes.c_uint64), ("flip", ctypes.c_uint64), ("shift", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("count", ctypes.c_int64)]
+
+
 class qd_hlanczos_info(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("error", ctypes.c_double), ("n_spmv", ctypes.c_int64)]
 
@@ -100,6 +105,7 @@ SIGNATURES = {
     "qd_memcpy_h2d": (I, [VP, VP, VP, ctypes.c_size_t]),
     "qd_memcpy_d2h": (I, [VP, VP, VP, ctypes.c_size_t]),
     "qd_basis_create": (I, [VP, I, ctypes.POINTER(qd_constraint), I, PP]),
+    "qd_basis_create_q": (I, [VP, I, ctypes.POINTER(qd_quantity), I, PP]),
     "qd_basis_destroy": (I, [VP]),
     "qd_basis_size": (I64, [VP]),
     "qd_basis_download": (I, [VP, VP]),

@@ -9,8 +9,12 @@
 // the position equals sum_j C(p_j, j) over the set bits p_1 < ... < p_k of
 // the field (colex order); disjoint blocks combine mixed-radix with the
 // highest block most significant, because ascending x compares high bits
-// first.
+// first.  Sectors that also fix the doubly occupied or empty sites (Nd / Nh,
+// conservation.hpp:117-151,248-256) enumerate the sector of their Nset part
+// and keep, in order, the states whose quantities match; they rank by binary
+// search in that table.
 #include <algorithm>
+#include <vector>
 
 #include "qd_internal.hpp"
 
@@ -119,23 +123,36 @@ __global__ void __launch_bounds__(256) unrank_kernel(SectorDesc sec, uint64_t* _
 }
 
 // ------------------------------------------------------------ general path
-struct ConsList {
+// Quantities popcount(((x >> shift) ^ flip) & (x ^ flip) & sel) == cnt; a
+// plain Nset has shift 0 and flip 0.
+struct QList {
   int n;
-  uint64_t mask[16];
+  uint64_t sel[16];
+  uint64_t flip[16];
+  int shift[16];
   int64_t cnt[16];
 };
-__device__ inline bool cons_ok(const ConsList& c, uint64_t x) {
-  for (int k = 0; k < c.n; ++k)
-    if ((int64_t)__popcll(x & c.mask[k]) != c.cnt[k]) return false;
+__device__ inline bool q_ok(const QList& c, uint64_t x) {
+  for (int k = 0; k < c.n; ++k) {
+    const uint64_t f = c.flip[k];
+    if ((int64_t)__popcll(((x >> c.shift[k]) ^ f) & (x ^ f) & c.sel[k]) != c.cnt[k]) return false;
+  }
   return true;
 }
+// candidate i of the scan: the value i itself, or entry i of a table
+template <bool TABLE>
+__device__ inline uint64_t candidate(const uint64_t* __restrict__ src, uint64_t i) {
+  return TABLE ? src[i] : i;
+}
 #define SCANCHUNK 4096
-__global__ void __launch_bounds__(256) pred_count(ConsList c, uint64_t range, int32_t* __restrict__ counts) {
+template <bool TABLE>
+__global__ void __launch_bounds__(256) pred_count(QList c, const uint64_t* __restrict__ src, uint64_t range,
+                                                  int32_t* __restrict__ counts) {
   uint64_t base = (uint64_t)blockIdx.x * SCANCHUNK;
   int s = 0;
   for (int k = threadIdx.x; k < SCANCHUNK; k += 256) {
-    uint64_t x = base + k;
-    if (x < range && cons_ok(c, x)) ++s;
+    uint64_t i = base + k;
+    if (i < range && q_ok(c, candidate<TABLE>(src, i))) ++s;
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   __shared__ int red[4];
@@ -143,15 +160,17 @@ __global__ void __launch_bounds__(256) pred_count(ConsList c, uint64_t range, in
   __syncthreads();
   if (threadIdx.x == 0) counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
-// ordered compaction of one chunk: each wave handles 64 consecutive values at
-// a time, ballot gives in-order positions.
-__global__ void __launch_bounds__(64) pred_fill(ConsList c, uint64_t range, const int64_t* __restrict__ offs,
-                                                uint64_t* __restrict__ out) {
+// ordered compaction of one chunk: each wave handles 64 consecutive
+// candidates at a time, ballot gives in-order positions.
+template <bool TABLE>
+__global__ void __launch_bounds__(64) pred_fill(QList c, const uint64_t* __restrict__ src, uint64_t range,
+                                                const int64_t* __restrict__ offs, uint64_t* __restrict__ out) {
   uint64_t base = (uint64_t)blockIdx.x * SCANCHUNK;
   int64_t pos = offs[blockIdx.x];
   for (int k = 0; k < SCANCHUNK; k += 64) {
-    uint64_t x = base + k + threadIdx.x;
-    bool ok = x < range && cons_ok(c, x);
+    uint64_t i = base + k + threadIdx.x;
+    uint64_t x = i < range ? candidate<TABLE>(src, i) : 0;
+    bool ok = i < range && q_ok(c, x);
     uint64_t bal = __ballot(ok);
     if (ok) {
       int before = __popcll(bal & ((1ull << threadIdx.x) - 1ull));
@@ -159,6 +178,53 @@ __global__ void __launch_bounds__(64) pred_fill(ConsList c, uint64_t range, cons
     }
     pos += __popcll(bal);
   }
+}
+
+// In-order selection of the candidates that satisfy c: the values
+// [0, range) (src == nullptr) or the entries of the ascending table src.
+// Leaves b->d_states (ascending) and b->sec as a table-ranked sector.
+static int select_states(qd_ctx* ctx, qd_basis* b, int nbits, const QList& c, const uint64_t* src,
+                         uint64_t range) {
+  int64_t nchunks = (int64_t)((range + SCANCHUNK - 1) / SCANCHUNK);
+  if (nchunks < 1) nchunks = 1;
+  int32_t* counts = nullptr;
+  int64_t* offs = nullptr;
+  if (hipMalloc(&counts, sizeof(int32_t) * nchunks) != hipSuccess ||
+      hipMalloc(&offs, sizeof(int64_t) * (nchunks + 1)) != hipSuccess) {
+    hipFree(counts);
+    return qd_set_error(ctx, QD_E_OOM, "scan buffers");
+  }
+  if (src)
+    pred_count<true><<<(unsigned)nchunks, 256, 0, ctx->stream>>>(c, src, range, counts);
+  else
+    pred_count<false><<<(unsigned)nchunks, 256, 0, ctx->stream>>>(c, src, range, counts);
+  int rc = qd_scan_counts(ctx, counts, nchunks, offs);
+  int64_t total = 0;
+  if (rc == QD_OK) {
+    hipMemcpyAsync(&total, offs + nchunks, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+    if (hipMalloc(&b->d_states, sizeof(uint64_t) * (total ? total : 1)) != hipSuccess) rc = QD_E_OOM;
+  }
+  if (rc == QD_OK && total > 0) {
+    if (src)
+      pred_fill<true><<<(unsigned)nchunks, 64, 0, ctx->stream>>>(c, src, range, offs, b->d_states);
+    else
+      pred_fill<false><<<(unsigned)nchunks, 64, 0, ctx->stream>>>(c, src, range, offs, b->d_states);
+  }
+  hipError_t le = hipGetLastError();
+  hipStreamSynchronize(ctx->stream);
+  hipFree(counts);
+  hipFree(offs);
+  if (rc == QD_OK && le != hipSuccess) rc = qd_set_error(ctx, QD_E_HIP, "basis scan: %s", hipGetErrorString(le));
+  if (rc != QD_OK) {
+    hipFree(b->d_states);
+    b->d_states = nullptr;
+    return rc == QD_E_OOM ? qd_set_error(ctx, QD_E_OOM, "basis table") : rc;
+  }
+  b->sec.cns = 0;
+  b->sec.nbits = nbits;
+  b->sec.total = (uint64_t)total;
+  return QD_OK;
 }
 
 // ------------------------------------------------------------ K2: rank
@@ -212,42 +278,81 @@ extern "C" int qd_basis_create(qd_ctx* ctx, int nbits, const qd_constraint* cons
     return qd_set_error(ctx, QD_E_UNSUPPORTED,
                         "sector with overlapping/non-contiguous masks needs a scan of 2^%d states", nbits);
   }
-  ConsList cl{};
+  QList cl{};
   cl.n = ncons;
   for (int c = 0; c < ncons; ++c) {
-    cl.mask[c] = cons[c].mask;
+    cl.sel[c] = cons[c].mask;
     cl.cnt[c] = cons[c].count;
   }
-  uint64_t range = 1ull << nbits;
-  int64_t nchunks = (int64_t)((range + SCANCHUNK - 1) / SCANCHUNK);
-  int32_t* counts = nullptr;
-  int64_t* offs = nullptr;
-  if (hipMalloc(&counts, sizeof(int32_t) * nchunks) != hipSuccess ||
-      hipMalloc(&offs, sizeof(int64_t) * (nchunks + 1)) != hipSuccess) {
-    hipFree(counts);
-    delete b;
-    return qd_set_error(ctx, QD_E_OOM, "scan buffers");
-  }
-  pred_count<<<(unsigned)nchunks, 256, 0, ctx->stream>>>(cl, range, counts);
-  int rc = qd_scan_counts(ctx, counts, nchunks, offs);
-  int64_t total = 0;
-  if (rc == QD_OK) {
-    hipMemcpyAsync(&total, offs + nchunks, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
-    hipStreamSynchronize(ctx->stream);
-    if (hipMalloc(&b->d_states, sizeof(uint64_t) * (total ? total : 1)) != hipSuccess) rc = QD_E_OOM;
-  }
-  if (rc == QD_OK && total > 0) pred_fill<<<(unsigned)nchunks, 64, 0, ctx->stream>>>(cl, range, offs, b->d_states);
-  hipStreamSynchronize(ctx->stream);
-  hipFree(counts);
-  hipFree(offs);
+  int rc = select_states(ctx, b, nbits, cl, nullptr, 1ull << nbits);
   if (rc != QD_OK) {
-    hipFree(b->d_states);
     delete b;
-    return rc == QD_E_OOM ? qd_set_error(ctx, QD_E_OOM, "basis table") : rc;
+    return rc;
   }
-  b->sec.cns = 0;
-  b->sec.nbits = nbits;
-  b->sec.total = (uint64_t)total;
+  *out = b;
+  return QD_OK;
+}
+
+extern "C" int qd_basis_create_q(qd_ctx* ctx, int nbits, const qd_quantity* q, int nq, qd_basis** out) {
+  if (!ctx || !out || nbits < 0 || nbits > QD_MAX_BITS || nq < 0 || nq > 16 || (nq > 0 && !q))
+    return qd_set_error(ctx, QD_E_ARG, "qd_basis_create_q: bad arguments");
+  std::vector<qd_constraint> plain;
+  QList gen{};
+  for (int k = 0; k < nq; ++k) {
+    if (q[k].shift < 0 || q[k].shift > 63) return qd_set_error(ctx, QD_E_ARG, "qd_basis_create_q: shift");
+    if (q[k].shift == 0 && q[k].flip == 0) {
+      plain.push_back(qd_constraint{q[k].sel, q[k].count});
+    } else {
+      gen.sel[gen.n] = q[k].sel;
+      gen.flip[gen.n] = q[k].flip;
+      gen.shift[gen.n] = q[k].shift;
+      gen.cnt[gen.n] = q[k].count;
+      ++gen.n;
+    }
+  }
+  if (gen.n == 0) return qd_basis_create(ctx, nbits, plain.data(), (int)plain.size(), out);
+  *out = nullptr;
+  hipSetDevice(ctx->device);
+  // the sector of the Nset part, enumerated when it has the CNS form
+  qd_basis* base = nullptr;
+  SectorDesc probe{};
+  if (analyse_sector(nbits, plain.data(), (int)plain.size(), &probe) == 1) {
+    int rc = qd_basis_create(ctx, nbits, plain.data(), (int)plain.size(), &base);
+    if (rc != QD_OK) return rc;
+  }
+  qd_basis* b = new qd_basis();
+  b->ctx = ctx;
+  int rc;
+  if (base) {
+    if (base->sec.total == 0) {  // empty Nset sector: an empty table
+      b->sec.nbits = nbits;
+      b->sec.total = 0;
+      b->sec.cns = 0;
+      rc = QD_OK;
+    } else {
+      rc = select_states(ctx, b, nbits, gen, base->d_states, base->sec.total);
+    }
+    qd_basis_destroy(base);
+  } else {
+    if (nbits > 40) {
+      delete b;
+      return qd_set_error(ctx, QD_E_UNSUPPORTED, "constraint set needs a scan of 2^%d states", nbits);
+    }
+    QList all = gen;
+    for (auto& c : plain) {
+      if (all.n >= 16) break;
+      all.sel[all.n] = c.mask;
+      all.flip[all.n] = 0;
+      all.shift[all.n] = 0;
+      all.cnt[all.n] = c.count;
+      ++all.n;
+    }
+    rc = select_states(ctx, b, nbits, all, nullptr, 1ull << nbits);
+  }
+  if (rc != QD_OK) {
+    delete b;
+    return rc;
+  }
   *out = b;
   return QD_OK;
 }

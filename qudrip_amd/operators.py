@@ -41,9 +41,18 @@ def getQbits(N):
 
 
 class Constraint:
-    def __init__(self, mask, count):
+    """cq = count for popcount(((x >> shift) ^ flip) & (x ^ flip) & mask): a
+    plain Nset has shift 0 and flip 0 (the qd_quantity form)."""
+
+    def __init__(self, mask, count, shift=0, flip=0):
         self.mask = int(mask)
         self.count = int(count)
+        self.shift = int(shift)
+        self.flip = int(flip)
+
+    @property
+    def plain(self):
+        return self.shift == 0 and self.flip == 0
 
 
 class Nset:
@@ -64,8 +73,32 @@ class Nset:
         return Constraint(self.mask, value)
 
 
+class Nd:
+    """Doubly occupied (hole=False) or empty (hole=True) sites of a spinful
+    index, up spins on bits 0..Nsite-1 (conservation.hpp:117-151):
+    popcount(((x >> Nsite) ^ m) & (x ^ m)), m = bits 0..Nsite-1 if hole."""
+
+    ALL = (1 << 64) - 1
+
+    def __init__(self, index, Nsite, hole=False):
+        self.index = index
+        self.Nsite = int(Nsite)
+        self.flip = ((1 << self.Nsite) - 1) if hole else 0
+
+    def __eq__(self, value):
+        return Constraint(self.ALL, value, shift=self.Nsite, flip=self.flip)
+
+
 def getNset(index):
     return Nset(index, -1, -1)
+
+
+def getNd(index, Nsite):
+    return Nd(index, Nsite, False)
+
+
+def getNh(index, Nsite):
+    return Nd(index, Nsite, True)
 
 
 def getNup(index, Nsite):
@@ -84,10 +117,24 @@ class Sector:
         self.nbits = index.nbits
         self.constraints = list(constraints)
 
+    @property
+    def plain(self):
+        return all(c.plain for c in self.constraints)
+
     def ctypes_constraints(self):
         arr = (abi.qd_constraint * max(1, len(self.constraints)))()
         for i, c in enumerate(self.constraints):
+            assert c.plain, "Nd/Nh constraints: use ctypes_quantities"
             arr[i].mask = c.mask
+            arr[i].count = c.count
+        return arr, len(self.constraints)
+
+    def ctypes_quantities(self):
+        arr = (abi.qd_quantity * max(1, len(self.constraints)))()
+        for i, c in enumerate(self.constraints):
+            arr[i].sel = c.mask
+            arr[i].flip = c.flip
+            arr[i].shift = c.shift
             arr[i].count = c.count
         return arr, len(self.constraints)
 

@@ -1,11 +1,13 @@
 // dropin_dump.cpp -- builds a model through the drop-in C++ API and writes
 // its term program and `>>` matrix as raw binary files, so the tests can
 // compare them with the Python mirror and the CPU oracle.
-//   usage: dropin_dump <xy|xxz|hubchain|lazy> <N> <out-prefix> [Jxy Jz | t U]
+//   usage: dropin_dump <xy|xxz|hubchain|hubnd|lazy> <N> <out-prefix> [Jxy Jz | t U]
 //          [herm <kry_dim> <h>]
 // Files: <prefix>.terms (qd_term[]), <prefix>.ops (qd_op[]), <prefix>.outer,
 // <prefix>.inner (int32), <prefix>.vals (complex128), <prefix>.y (H x for the
-// seeded x of seed 7; for "lazy" computed as psi2 = ham * psi1).  With
+// seeded x of seed 7; for "lazy" computed as psi2 = ham * psi1).  "hubnd" is
+// the Hubbard chain in the sector that also fixes N/4 doubly occupied sites
+// (getNd).  With
 // "herm k h" the two-vector solver (Hermitian::getSolver(sub, k)) then
 // evolves x by h into <prefix>.ev and replaces x by its ground state,
 // <prefix>.gs, with the Ritz values in <prefix>.E (storeE) and
@@ -35,7 +37,7 @@ int main(int argc, char** argv) {
   std::string out = argv[3];
   double a = argc > 4 ? std::stod(argv[4]) : 1.0, b = argc > 5 ? std::stod(argv[5]) : 1.0;
 
-  bool hub = (model == "hubchain" || model == "lazy");
+  bool hub = (model == "hubchain" || model == "hubnd" || model == "lazy");
   auto q = getQbits(hub ? 2 * N : N);
   OperatorSum ham = getEmptyOperator();
   if (!hub) {
@@ -67,7 +69,10 @@ int main(int argc, char** argv) {
   auto nset = getNset(q);
   auto nup = getNup(q, N);
   auto ndo = getNdo(q, N);
-  auto sub = hub ? Constrain(q, nup = N / 2, ndo = N / 2) : Constrain(q, nset = N / 2);
+  auto nd = getNd(q, N);
+  auto sub = model == "hubnd" ? Constrain(q, nup = N / 2, ndo = N / 2, nd = N / 4)
+             : hub            ? Constrain(q, nup = N / 2, ndo = N / 2)
+                              : Constrain(q, nset = N / 2);
   auto psi = getState(sub, 2);
   shuffle_seed() = 7;
   shuffle_state(psi(0));

@@ -60,9 +60,13 @@ class Context:
 class Basis:
     def __init__(self, ctx, sector):
         self.ctx = ctx
-        arr, n = sector.ctypes_constraints()
         p = ctypes.c_void_p()
-        ctx.check(ctx.lib.qd_basis_create(ctx.ptr, sector.nbits, arr, n, ctypes.byref(p)), "basis_create")
+        if sector.plain:
+            arr, n = sector.ctypes_constraints()
+            ctx.check(ctx.lib.qd_basis_create(ctx.ptr, sector.nbits, arr, n, ctypes.byref(p)), "basis_create")
+        else:
+            arr, n = sector.ctypes_quantities()
+            ctx.check(ctx.lib.qd_basis_create_q(ctx.ptr, sector.nbits, arr, n, ctypes.byref(p)), "basis_create_q")
         self.ptr = p
         self.nbits = sector.nbits
         self.size = ctx.lib.qd_basis_size(p)

@@ -23,6 +23,17 @@ def test_headers_compile_standalone(tmp_path):
                    check=True)
 
 
+def test_headers_compile_with_nd_nh(tmp_path):
+    """getNd / getNh (conservation.hpp:248-256) next to getNup / getNdo in Constrain."""
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "qudrip/qudrip.hpp"\nusing namespace qudrip;\n'
+                   'void f(){ auto q = getQbits(8); auto nup = getNup(q, 4); auto ndo = getNdo(q, 4);\n'
+                   ' auto nd = getNd(q, 4); auto nh = getNh(q, 4);\n'
+                   ' auto s = Constrain(q, nup = 2, ndo = 2, nd = 1, nh = 1); (void)s; }\nint main(){}\n')
+    subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"), str(src)],
+                   check=True)
+
+
 def test_driver_programs_built():
     for p in ("spin", "dropin_dump"):
         assert os.access(os.path.join(BIN, p), os.X_OK), p
@@ -51,7 +62,8 @@ def _load(prefix, ext, dtype):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model,N,a,b", [("xy", 12, 1.0, 1.0), ("xxz", 12, 0.9317, 0.3711), ("hubchain", 6, 0.77, 5.123)])
+@pytest.mark.parametrize("model,N,a,b", [("xy", 12, 1.0, 1.0), ("xxz", 12, 0.9317, 0.3711), ("hubchain", 6, 0.77, 5.123),
+                                         ("hubnd", 8, 0.77, 5.123)])
 def test_dropin_matches_python_mirror_and_oracle(tmp_path, model, N, a, b):
     prefix = str(tmp_path / model)
     _run([os.path.join(BIN, "dropin_dump"), model, str(N), prefix, repr(a), repr(b)])
@@ -64,6 +76,8 @@ def test_dropin_matches_python_mirror_and_oracle(tmp_path, model, N, a, b):
     else:
         idx, ham = Q.hubbard(N, Q.hubbard_bonds_chain(N, False), t=a, U=b)
         sec = Q.hubbard_sector(idx, N, N // 2, N // 2)
+        if model == "hubnd":
+            sec = Q.Constrain(idx, *sec.constraints, Q.getNd(idx, N) == N // 4)
     terms, nt, ops, no = ham.program()
     # identical term programs (C++ algebra == Python mirror, term order and coefficients)
     assert _load(prefix, ".terms", np.uint8).tobytes() == bytes(terms)[: nt * 24]

@@ -73,6 +73,33 @@ def test_basis_rank_miss_and_general_path(ctx):
         b.close()
 
 
+@pytest.mark.parametrize("case", ["hub8_44_nd2", "hub6_33_nh1", "hub6_nd1_scan", "hub7_43_nd1_nh2", "hub6_empty"])
+def test_basis_nd_nh_bit_exact(ctx, case):
+    """getNd / getNh sectors: the Nset sector enumerated then filtered in
+    order (or a scan when there is no Nset part); rank by binary search."""
+    L = {"hub8_44_nd2": 8, "hub7_43_nd1_nh2": 7}.get(case, 6)
+    idx = Q.getQbits(2 * L)
+    nup, ndo, nd, nh = Q.getNup(idx, L), Q.getNdo(idx, L), Q.getNd(idx, L), Q.getNh(idx, L)
+    cons = {"hub8_44_nd2": [nup == 4, ndo == 4, nd == 2],
+            "hub6_33_nh1": [nup == 3, ndo == 3, nh == 1],
+            "hub6_nd1_scan": [nd == 1],
+            "hub7_43_nd1_nh2": [nup == 4, ndo == 3, nd == 1, nh == 2],
+            "hub6_empty": [nup == 2, ndo == 2, nd == 3]}[case]
+    sec = Q.Constrain(idx, *cons)
+    ref = oracle.basis_scan(sec)
+    b = Basis(ctx, sec)
+    try:
+        assert b.size == len(ref)
+        assert np.array_equal(b.states(), ref)
+        if len(ref):
+            assert np.array_equal(b.rank(ref), np.arange(len(ref)))
+            miss = np.array([int(ref[0]) ^ 1, (1 << (2 * L)) | int(ref[-1])], dtype=np.uint64)
+            r = b.rank(miss)
+            assert r[1] == len(ref) and (r[0] == len(ref) or ref[r[0]] == miss[0])
+    finally:
+        b.close()
+
+
 def test_basis_empty_sector(ctx):
     idx = Q.getQbits(6)
     b = Basis(ctx, Q.Constrain(idx, Q.getNset(idx) == 9))
@@ -101,6 +128,11 @@ CSR_CASES = {
     "hub8_obc_44": lambda: hub(8, Q.hubbard_bonds_chain(8, False), 4, 4)[1:],
     "hub8_obc_random_U": lambda: hub(8, Q.hubbard_bonds_chain(8, False), 3, 2, t=0.77, U=5.123)[1:],
     "hub3x4_33": lambda: hub(12, Q.hubbard_bonds_square(3, 4), 3, 3)[1:],
+    # doubly occupied sites fixed (getNd): table-ranked sector, hops that
+    # change the count leave it and are dropped, as `>>` drops any miss
+    "hub8_obc_44_nd2": lambda: (lambda i, h: (h, Q.Constrain(i, Q.getNup(i, 8) == 4, Q.getNdo(i, 8) == 4,
+                                                             Q.getNd(i, 8) == 2)))(
+        *Q.hubbard(8, Q.hubbard_bonds_chain(8, False), 1.0, 4.0)),
 }
 
 

@@ -352,3 +352,37 @@ def test_hlanczos_evolve_matches_dense_expm():
     out = oracle.hlanczos(csc, 8, psi, mode=1, h=0.05)["psi"]
     ref = scipy.linalg.expm(-1j * 0.05 * H) @ (psi / np.linalg.norm(psi))
     assert np.allclose(out, ref, atol=1e-12)
+
+
+# ---------------------------------------------- Nd / Nh (double occupancy, holes)
+def _site_counts(x, L):
+    up = [(x >> i) & 1 for i in range(L)]
+    dn = [(x >> (i + L)) & 1 for i in range(L)]
+    return (sum(up), sum(dn), sum(u & d for u, d in zip(up, dn)), sum((1 - u) & (1 - d) for u, d in zip(up, dn)))
+
+
+@pytest.mark.parametrize("L,nup,ndn,nd,nh", [(4, 2, 2, 1, None), (4, 2, 2, None, 1), (4, None, None, 1, None),
+                                             (5, 3, 2, 1, 1), (3, 3, 3, 2, None)])
+def test_basis_scan_nd_nh_matches_site_definition(L, nup, ndn, nd, nh):
+    """getNd / getNh count doubly occupied / empty sites (brute force over
+    the 2L-bit states, site by site); combined with getNup / getNdo."""
+    idx = Q.getQbits(2 * L)
+    cons = []
+    if nup is not None:
+        cons += [Q.getNup(idx, L) == nup, Q.getNdo(idx, L) == ndn]
+    if nd is not None:
+        cons.append(Q.getNd(idx, L) == nd)
+    if nh is not None:
+        cons.append(Q.getNh(idx, L) == nh)
+    got = oracle.basis_scan(Q.Constrain(idx, *cons))
+    ref = []
+    for x in range(1 << (2 * L)):
+        u, d, dd, hh = _site_counts(x, L)
+        if nup is not None and (u != nup or d != ndn):
+            continue
+        if nd is not None and dd != nd:
+            continue
+        if nh is not None and hh != nh:
+            continue
+        ref.append(x)
+    assert np.array_equal(got, np.array(ref, dtype=np.uint64))
