@@ -4,6 +4,7 @@
 //   getQbits(N)                         index.hpp:397 (QbitsIndex :177-191)
 //   getNset / getNup / getNdo, `cq = v`  conservation.hpp:89-115, 233-246
 //   getNd / getNh                         conservation.hpp:117-151, 248-256
+//   SingleModeIndex, Indices (q * m)      index.hpp:94-184
 //   Constrain(idx, cq = v, ...)          conservation.hpp:212-231 -> SubIndex
 //   cursor protocol idx[i], ++idx, (idxv_type)idx, idx.range()
 // The sector table is produced on the GPU (qd_basis_create); the host keeps
@@ -49,6 +50,62 @@ class QbitsIndex {
 };
 
 inline QbitsIndex<> getQbits(int N) { return QbitsIndex<>(N); }
+
+// One boson mode with n_mode levels (SingleModeIndex, index.hpp:94-110).
+template <class T = idx_size_t>
+class SingleModeIndex {
+  T n_;
+  T v_ = 0;
+
+ public:
+  struct is_index {
+    using type = int;
+  };
+  explicit SingleModeIndex(int n_mode) : n_((T)n_mode) {}
+  T range() const { return n_; }
+  operator T() const { return v_; }
+  void operator[](T v) { v_ = v; }
+  SingleModeIndex& operator++() {
+    ++v_;
+    return *this;
+  }
+};
+
+// Tensor product of a QbitsIndex and a SingleModeIndex (Indices,
+// index.hpp:112-175): value q * mdim + m, the mode the least significant
+// digit; `q * m` (:177-184).
+template <class L, class R>
+class Indices {
+  L& lhs_;
+  R& rhs_;
+
+ public:
+  struct is_index {
+    using type = int;
+  };
+  Indices(L& l, R& r) : lhs_(l), rhs_(r) {}
+  idxv_type range() const { return (idxv_type)lhs_.range() * (idxv_type)rhs_.range(); }
+  operator idxv_type() const { return (idxv_type)lhs_ * (idxv_type)rhs_.range() + (idxv_type)rhs_; }
+  void operator[](idxv_type i) {  // :158-161
+    lhs_[i / rhs_.range()];
+    rhs_[i % rhs_.range()];
+  }
+  Indices& operator++() {  // :164-172
+    if ((idxv_type)rhs_ < (idxv_type)rhs_.range() - 1) {
+      ++rhs_;
+    } else {
+      rhs_[0];
+      ++lhs_;
+    }
+    return *this;
+  }
+  int nbits() const { return lhs_.nbits(); }
+  int mdim() const { return (int)rhs_.range(); }
+};
+template <class T>
+Indices<QbitsIndex<T>, SingleModeIndex<T>> operator*(QbitsIndex<T>& q, SingleModeIndex<T>& m) {
+  return Indices<QbitsIndex<T>, SingleModeIndex<T>>(q, m);
+}
 
 // popcount(x & mask) == value  (Nset, conservation.hpp:89-115)
 template <class T1 = idx_size_t>
@@ -109,6 +166,7 @@ namespace detail {
 struct BasisHolder {
   qd_basis* b = nullptr;
   int nbits = 0;
+  int mdim = 1;  // >= 2: composite index with a boson mode
   std::vector<uint64_t> host;  // lazily downloaded mapping
   ~BasisHolder() {
     if (b) qd_basis_destroy(b);
@@ -127,13 +185,20 @@ inline std::shared_ptr<BasisHolder> make_basis(int nbits, const std::vector<qd_q
   check(qd_basis_create_q(ctx(), nbits, q.data(), (int)q.size(), &h->b), "qd_basis_create_q");
   return h;
 }
+inline std::shared_ptr<BasisHolder> make_basis_modes(int nbits, int mdim, const std::vector<qd_quantity>& q) {
+  auto h = std::make_shared<BasisHolder>();
+  h->nbits = nbits;
+  h->mdim = mdim;
+  check(qd_basis_create_modes(ctx(), nbits, mdim, q.data(), (int)q.size(), &h->b), "qd_basis_create_modes");
+  return h;
+}
 }  // namespace detail
 
 // Constrained index (SubIndex, index.hpp:284-358): ascending sector table,
 // unrank via idx[i], rank via (idxv_type)idx, a miss ranks to range().
-template <class T = QbitsIndex<>&>
+template <class E = QbitsIndex<>>
 class SubIndex {
-  QbitsIndex<>& earless_;
+  E& earless_;
   std::shared_ptr<detail::BasisHolder> basis_;
   idxv_type idx_value_ = 0;
 
@@ -141,10 +206,10 @@ class SubIndex {
   struct is_index {
     using type = int;
   };
-  SubIndex(QbitsIndex<>& idx, std::shared_ptr<detail::BasisHolder> b) : earless_(idx), basis_(std::move(b)) {}
+  SubIndex(E& idx, std::shared_ptr<detail::BasisHolder> b) : earless_(idx), basis_(std::move(b)) {}
 
   idxv_type range() const { return (idxv_type)qd_basis_size(basis_->b); }
-  QbitsIndex<>& earless() const { return earless_; }
+  E& earless() const { return earless_; }
   const std::shared_ptr<detail::BasisHolder>& basis() const { return basis_; }
   int nbits() const { return basis_->nbits; }
 
@@ -176,6 +241,17 @@ SubIndex<> Constrain(QbitsIndex<>& index, CQs&&... cqs) {
   (q.push_back(cqs.quantity()), ...);
   SubIndex<> sub(index, detail::make_basis(index.nbits(), q));
   sub[0];  // :223
+  return sub;
+}
+
+// Constrain over a qubits-times-mode index: the quantities act on the qubits.
+template <class L, class R, class... CQs>
+SubIndex<Indices<L, R>> Constrain(Indices<L, R>& index, CQs&&... cqs) {
+  std::cout << "Constrain: original range " << index.range() << std::endl;
+  std::vector<qd_quantity> q;
+  (q.push_back(cqs.quantity()), ...);
+  SubIndex<Indices<L, R>> sub(index, detail::make_basis_modes(index.nbits(), index.mdim(), q));
+  sub[0];
   return sub;
 }
 

@@ -50,6 +50,7 @@ struct OpRec {
   int site_a, site_b;
   int string;
   value_type U[4];
+  std::vector<value_type> M;  // mode gate: U_[i*mdim+j] = U(i,j)
 };
 using op_ptr = std::shared_ptr<const OpRec>;
 
@@ -102,6 +103,30 @@ class hopBitOp {
     return Operator(std::make_shared<OpRec>(OpRec{QD_OP_HOP, (int)site2, (int)site1, string_, {}}));
   }
 };
+
+// Gate on a boson mode (modeOp, operators/mode.hpp:6-35): `gate << U`
+// stores the mdim x mdim matrix, gate(site) a fresh instance (site unused);
+// the instance takes mode m to sum_j U(m, j) |j> (mode.hpp:80-103).
+template <class T = idx_size_t>
+class modeOp {
+  int mdim_;
+  std::vector<value_type> U_;
+
+ public:
+  explicit modeOp(SingleModeIndex<T>& index) : mdim_((int)index.range()), U_((size_t)mdim_ * mdim_) {}
+  void operator<<(const Matrix& U) {  // operator.hpp:24-29
+    for (int i = 0; i < mdim_; ++i)
+      for (int j = 0; j < mdim_; ++j) U_[(size_t)i * mdim_ + j] = U(i, j);
+  }
+  Operator operator()(size_t = 0) const {
+    auto r = std::make_shared<OpRec>(OpRec{QD_OP_MODE, 0, 0, 1, {}, U_});
+    return Operator(r);
+  }
+};
+template <class T = idx_size_t>
+modeOp<T> getModeOp(SingleModeIndex<T>& index) {  // mode.hpp:31-34
+  return modeOp<T>(index);
+}
 
 template <class T = idx_size_t>
 bitOp<T> getFermiGate(QbitsIndex<T>& index) {
@@ -245,10 +270,14 @@ class OperatorSum {
     return r;
   }
 
-  // flattened term program (include/qudrip_hip.h data format)
-  void program(std::vector<qd_term>& terms, std::vector<qd_op>& ops, int* nbits_used) const {
+  // flattened term program (include/qudrip_hip.h data format); the matrices
+  // of mode gates go to mats (a mode gate's site_a is its position there)
+  void program(std::vector<qd_term>& terms, std::vector<qd_op>& ops, int* nbits_used,
+               std::vector<double>* mats = nullptr) const {
     terms.clear();
     ops.clear();
+    if (mats) mats->clear();
+    int nmat = 0;
     int nb = 0;
     for (size_t k = 0; k < ops_.size(); ++k) {
       qd_term t{};
@@ -262,6 +291,16 @@ class OperatorSum {
         o.site_a = r->site_a;
         o.site_b = r->site_b;
         o.string = r->string;
+        if (r->kind == QD_OP_MODE) {
+          o.site_a = o.site_b = nmat++;
+          if (mats)
+            for (const auto& u : r->M) {
+              mats->push_back(u.real());
+              mats->push_back(u.imag());
+            }
+          ops.push_back(o);
+          continue;
+        }
         for (int q = 0; q < 4; ++q) {
           o.U[2 * q] = r->U[q].real();
           o.U[2 * q + 1] = r->U[q].imag();
@@ -274,15 +313,22 @@ class OperatorSum {
     if (nbits_used) *nbits_used = nb;
   }
 
-  qd_termprog* make_program(int nbits) const {
+  qd_termprog* make_program(int nbits, int mdim = 1) const {
     std::vector<qd_term> terms;
     std::vector<qd_op> ops;
+    std::vector<double> mats;
     int nb = 0;
-    program(terms, ops, &nb);
+    program(terms, ops, &nb, &mats);
     qd_termprog* p = nullptr;
-    detail::check(qd_termprog_create(detail::ctx(), nbits, terms.data(), (int)terms.size(), ops.data(),
-                                     (int)ops.size(), &p),
-                  "qd_termprog_create");
+    if (mdim > 1)
+      detail::check(qd_termprog_create_modes(detail::ctx(), nbits, mdim, terms.data(), (int)terms.size(), ops.data(),
+                                             (int)ops.size(), mats.data(), (int)(mats.size() / (2 * mdim * mdim)),
+                                             &p),
+                    "qd_termprog_create_modes");
+    else
+      detail::check(qd_termprog_create(detail::ctx(), nbits, terms.data(), (int)terms.size(), ops.data(),
+                                       (int)ops.size(), &p),
+                    "qd_termprog_create");
     return p;
   }
 
@@ -291,7 +337,7 @@ class OperatorSum {
   SpMatrix operator>>(IDX& index) const {
     auto h = std::make_shared<detail::CsrHolder>();
     h->basis = index.basis();
-    h->prog = make_program(index.nbits());
+    h->prog = make_program(index.nbits(), index.basis()->mdim);
     detail::check(qd_csr_build(detail::ctx(), h->basis->b, h->prog, &h->csr), "qd_csr_build");
     return SpMatrix(h);
   }

@@ -13,6 +13,7 @@
 #include "runtime.hpp"
 #include "index.hpp"
 #include "operator.hpp"
+#include "mode.hpp"
 #include "state.hpp"
 #include "sparse_algorithm.hpp"
 #include "mc_manager.hpp"

@@ -73,9 +73,16 @@ typedef struct {
  *   kind == QD_OP_HOP : hopBitOp (operators/hopbit.hpp:6-84), gate(site2,
  *       site1) = c+_{site2} c_{site1}: site_a = site2, site_b = site1.
  *       U is unused.
+ *   kind == QD_OP_MODE : modeOp (operators/mode.hpp:6-30,80-103) on the
+ *       boson mode of a composite index (a QbitsIndex times a
+ *       SingleModeIndex of dimension mdim, index.hpp:94-184).  site_a is the
+ *       position of its matrix in the table given to
+ *       qd_termprog_create_modes; the matrix is stored as elOpBase stores it
+ *       (U_[i*mdim+j] = U(i,j)) and takes mode m to sum_j U(m,j) |j>
+ *       (mode.hpp:86-90).  string and U are unused.
  *   string = +1 (getBoseGate / getHopBoseGate) or -1 (fermionic, Jordan-
  *       Wigner sign from the bits above the site). */
-enum { QD_OP_BIT = 0, QD_OP_HOP = 1 };
+enum { QD_OP_BIT = 0, QD_OP_HOP = 1, QD_OP_MODE = 2 };
 typedef struct {
   int32_t kind;
   int32_t site_a;
@@ -95,6 +102,7 @@ typedef struct {
 
 #define QD_MAX_OPS_PER_TERM 8
 #define QD_MAX_BITS 64
+#define QD_MAX_MODE_DIM 16
 
 /* ------------------------------------------------------------------ context */
 int qd_ctx_create(int device, qd_ctx** out);
@@ -142,6 +150,12 @@ typedef struct {
  * not a disjoint set of bit ranges, all 2^nbits states are scanned, nbits <=
  * 40) and filtered in order; such a basis ranks by binary search. */
 int qd_basis_create_q(qd_ctx* ctx, int nbits, const qd_quantity* q, int nq, qd_basis** out);
+/* Constrain over a composite index QbitsIndex(nbits) * SingleModeIndex(mdim)
+ * (Indices, index.hpp:112-184): the quantities act on the qubit part q; the
+ * sector is every composite value q * mdim + m with q satisfying them, in
+ * ascending order (q-major), for 2 <= mdim <= QD_MAX_MODE_DIM; rank(x) =
+ * rank(q) * mdim + m, a miss ranks to the size. */
+int qd_basis_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_quantity* q, int nq, qd_basis** out);
 int qd_basis_destroy(qd_basis* basis);
 /* SubIndex::range() (index.hpp:357). */
 int64_t qd_basis_size(const qd_basis* basis);
@@ -160,6 +174,15 @@ int qd_basis_rank(qd_basis* basis, const uint64_t* d_states, int64_t count,
  * the QbitsIndex the gates act on; every site must be < nbits. */
 int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, int nterms,
                        const qd_op* ops, int nops, qd_termprog** out);
+/* The same over a composite index (QbitsIndex * SingleModeIndex(mdim),
+ * index.hpp:112-184): states are the composite values q * mdim + m, bit and
+ * hop gates act on q, QD_OP_MODE gates on m.  mats: nmats matrices of
+ * mdim * mdim interleaved complex numbers, U_[i*mdim+j] = U(i,j) each;
+ * 2 <= mdim <= QD_MAX_MODE_DIM.  Such a program builds stored matrices
+ * (qd_csr_build) over a basis made by qd_basis_create_modes with the same
+ * mdim; it has no matrix-free product. */
+int qd_termprog_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_term* terms, int nterms,
+                             const qd_op* ops, int nops, const double* mats, int nmats, qd_termprog** out);
 int qd_termprog_destroy(qd_termprog* prog);
 
 /* --------------------------------------------------------------- CSR build */

@@ -36,6 +36,9 @@ def lib():
     L.orc_basis_scan.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int64]
     L.orc_basis_scan_q.restype = ctypes.c_int64
     L.orc_basis_scan_q.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int64]
+    L.orc_build_csc_modes.restype = ctypes.c_int64
+    L.orc_build_csc_modes.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, _VP, ctypes.c_int, _VP, _VP, _VP, _VP,
+                                      _VP, ctypes.c_int64]
     L.orc_rank.restype = ctypes.c_int64
     L.orc_rank.argtypes = [_VP, ctypes.c_int64, ctypes.c_uint64]
     L.orc_binom.restype = ctypes.c_uint64
@@ -86,6 +89,15 @@ def _p(a):
 def basis_scan(sector):
     """Constrain / LooseConstrain (conservation.hpp:212-231): ascending scan."""
     from qudrip_amd.abi import qd_constraint
+    if getattr(sector, "mdim", 1) > 1:
+        # composite index (Indices, index.hpp:112-184): LooseConstrain walks
+        # q * mdim + m in order and tests the quantities on q alone
+        import copy
+        qsec = copy.copy(sector)
+        qsec.mdim = 1
+        q = basis_scan(qsec)
+        md = sector.mdim
+        return (q[:, None] * np.uint64(md) + np.arange(md, dtype=np.uint64)[None, :]).reshape(-1)
     if not all(c.plain for c in sector.constraints):
         arr, nq = sector.ctypes_quantities()
         n = lib().orc_basis_scan_q(sector.nbits, arr, nq, None, 0)
@@ -113,6 +125,15 @@ def binom(n, k):
 
 def build_csc(basis, opsum):
     """`ham >> index` (operator_abel.hpp:25-34, operator.hpp:235-267)."""
+    if opsum.has_modes():
+        terms, nt, ops, no, mats, nm, mdim = opsum.program_modes()
+        n = len(basis)
+        colptr = np.empty(n + 1, dtype=np.int32)
+        nnz = lib().orc_build_csc_modes(_p(basis), n, mdim, terms, nt, ops, _p(mats), _p(colptr), None, None, 0)
+        rowidx = np.empty(max(nnz, 1), dtype=np.int32)
+        vals = np.empty(2 * max(nnz, 1), dtype=np.float64)
+        lib().orc_build_csc_modes(_p(basis), n, mdim, terms, nt, ops, _p(mats), _p(colptr), _p(rowidx), _p(vals), nnz)
+        return colptr, rowidx[:nnz], vals[:2 * nnz]
     terms, nt, ops, no = opsum.program()
     n = len(basis)
     colptr = np.empty(n + 1, dtype=np.int32)

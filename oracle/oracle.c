@@ -340,6 +340,186 @@ int64_t orc_build_csc(const uint64_t* basis, int64_t n, const qd_term* terms, in
 }
 
 /* ======================================================================
+ * Composite index QbitsIndex * SingleModeIndex(mdim) (Indices,
+ * index.hpp:112-184): a node holds the composite value q * mdim + m (+1).
+ * Tree as orc_expand_term, level by level; bit and hop gates act on q, a
+ * mode gate (operators/mode.hpp:80-103) gives every node mdim children,
+ * child j with value U_[m * mdim + j] * val (U_ as elOpBase stores it,
+ * operator.hpp:24-29), dead nodes mdim dead children.  mats holds the
+ * program's matrices, mdim * mdim interleaved complex each; a mode gate's
+ * site_a picks one.  Returns the leaf count; *out_idx / *out_val are
+ * malloc'ed (caller frees).
+ * ====================================================================== */
+static int expand_term_modes(const qd_op* ops, int n_ops, int mdim, const double* mats, uint64_t state,
+                             uint64_t** out_idx, oc** out_val) {
+  int64_t total = 1;
+  for (int iop = 0; iop < n_ops; ++iop)
+    total *= ops[iop].kind == QD_OP_BIT ? 2 : ops[iop].kind == QD_OP_MODE ? mdim : 1;
+  uint64_t* ci = (uint64_t*)malloc(sizeof(uint64_t) * total);
+  uint64_t* ni = (uint64_t*)malloc(sizeof(uint64_t) * total);
+  oc* cv = (oc*)malloc(sizeof(oc) * total);
+  oc* nv = (oc*)malloc(sizeof(oc) * total);
+  int n = 1;
+  ci[0] = state + 1;
+  cv[0] = c_make(1.0, 0.0);
+  for (int iop = n_ops - 1; iop >= 0; --iop) {
+    const qd_op* op = &ops[iop];
+    int m = 0;
+    for (int j = 0; j < n; ++j) {
+      const int alive = ci[j] != 0;
+      const uint64_t x = alive ? ci[j] - 1 : 0;
+      const uint64_t q = x / (uint64_t)mdim;
+      const uint64_t md = x % (uint64_t)mdim;
+      if (op->kind == QD_OP_MODE) {
+        const double* U = mats + 2 * (size_t)op->site_a * mdim * mdim;
+        for (int o = 0; o < mdim; ++o) {
+          if (alive) {
+            oc u = c_make(U[2 * (md * mdim + o)], U[2 * (md * mdim + o) + 1]);
+            nv[m + o] = c_mul(u, cv[j]);
+            ni[m + o] = q * mdim + o + 1;
+          } else {
+            nv[m + o] = c_make(0.0, 0.0);
+            ni[m + o] = 0;
+          }
+        }
+        m += mdim;
+      } else if (op->kind == QD_OP_BIT) {
+        int s = op->site_a;
+        if (alive) {
+          int b = (int)((q >> s) & 1ull);
+          int prev_set = popc(q >> (s + 1));
+          oc jw = c_make((prev_set % 2) ? (double)op->string : 1.0, 0.0);
+          oc u0 = c_make(op->U[(b * 2 + 0) * 2], op->U[(b * 2 + 0) * 2 + 1]);
+          oc u1 = c_make(op->U[(b * 2 + 1) * 2], op->U[(b * 2 + 1) * 2 + 1]);
+          nv[m] = c_mul(c_mul(jw, u0), cv[j]);
+          ni[m] = (q & ~(1ull << s)) * mdim + md + 1;
+          nv[m + 1] = c_mul(c_mul(jw, u1), cv[j]);
+          ni[m + 1] = (q | (1ull << s)) * mdim + md + 1;
+        } else {
+          ni[m] = ni[m + 1] = 0;
+          nv[m] = nv[m + 1] = c_make(0.0, 0.0);
+        }
+        m += 2;
+      } else {
+        int a = op->site_a, b = op->site_b;
+        ni[m] = 0;
+        nv[m] = c_make(0.0, 0.0);
+        if (alive) {
+          if (a == b) {
+            ni[m] = x + 1;
+            nv[m] = c_rmul((double)((q >> a) & 1ull), cv[j]);
+          } else if (((q >> b) & 1ull) == 1 && ((q >> a) & 1ull) == 0) {
+            int jw1 = popc(q >> (b + 1));
+            uint64_t q1 = q & ~(1ull << b);
+            int jw2 = popc(q1 >> (a + 1));
+            int sgn = ((jw1 % 2) ? op->string : 1) * ((jw2 % 2) ? op->string : 1);
+            nv[m] = c_mul(c_make((double)sgn, 0.0), cv[j]);
+            ni[m] = (q1 | (1ull << a)) * mdim + md + 1;
+          }
+        }
+        m += 1;
+      }
+    }
+    n = m;
+    uint64_t* ti = ci;
+    ci = ni;
+    ni = ti;
+    oc* tv = cv;
+    cv = nv;
+    nv = tv;
+  }
+  free(ni);
+  free(nv);
+  *out_idx = ci;
+  *out_val = cv;
+  return n;
+}
+
+/* `ham >> index` over a composite index: as orc_build_csc (same triplet
+ * merge and term sum), leaves from expand_term_modes; basis holds composite
+ * values ascending.  Returns nnz or -1. */
+int64_t orc_build_csc_modes(const uint64_t* basis, int64_t n, int mdim, const qd_term* terms, int nterms,
+                            const qd_op* ops, const double* mats, int32_t* colptr, int32_t* rowidx, double* vals,
+                            int64_t cap) {
+  int64_t ecap = 256;
+  orc_entry* ent = (orc_entry*)malloc(sizeof(orc_entry) * ecap);
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t ne = 0;
+    colptr[i] = (int32_t)nnz;
+    for (int k = 0; k < nterms; ++k) {
+      const qd_term* t = &terms[k];
+      uint64_t* lidx;
+      oc* lv;
+      int nl = expand_term_modes(ops + t->op_begin, t->n_ops, mdim, mats, basis[i], &lidx, &lv);
+      int64_t* trow = (int64_t*)malloc(sizeof(int64_t) * (nl ? nl : 1));
+      oc* tval = (oc*)malloc(sizeof(oc) * (nl ? nl : 1));
+      int nt = 0;
+      for (int j = 0; j < nl; ++j) {
+        if (lidx[j] == 0) continue;
+        if (!(c_abs(lv[j]) > ORC_EPS)) continue;
+        int64_t r = orc_rank(basis, n, lidx[j] - 1);
+        if (r >= n) continue;
+        int found = -1;
+        for (int q = 0; q < nt; ++q)
+          if (trow[q] == r) {
+            found = q;
+            break;
+          }
+        if (found >= 0)
+          tval[found] = c_add(tval[found], lv[j]);
+        else {
+          trow[nt] = r;
+          tval[nt] = lv[j];
+          ++nt;
+        }
+      }
+      oc ck = c_make(t->coeff[0], t->coeff[1]);
+      for (int q = 0; q < nt; ++q) {
+        oc prod = c_mul(ck, tval[q]);
+        int64_t e;
+        for (e = 0; e < ne; ++e)
+          if (ent[e].row == trow[q]) break;
+        if (e == ne) {
+          if (ne == ecap) {
+            ecap *= 2;
+            ent = (orc_entry*)realloc(ent, sizeof(orc_entry) * ecap);
+          }
+          ent[ne].row = trow[q];
+          ent[ne].acc = c_make(0.0, 0.0);
+          ++ne;
+        }
+        ent[e].acc = c_add(ent[e].acc, prod);
+      }
+      free(trow);
+      free(tval);
+      free(lidx);
+      free(lv);
+    }
+    for (int64_t a = 1; a < ne; ++a) {
+      orc_entry key = ent[a];
+      int64_t b = a - 1;
+      while (b >= 0 && ent[b].row > key.row) {
+        ent[b + 1] = ent[b];
+        --b;
+      }
+      ent[b + 1] = key;
+    }
+    if (rowidx && vals && nnz + ne <= cap) {
+      for (int64_t e = 0; e < ne; ++e) {
+        rowidx[nnz + e] = (int32_t)ent[e].row;
+        vals[2 * (nnz + e)] = ent[e].acc.re;
+        vals[2 * (nnz + e) + 1] = ent[e].acc.im;
+      }
+    }
+    nnz += ne;
+  }
+  colptr[n] = (int32_t)nnz;
+  free(ent);
+  return nnz;
+}
+
+/* ======================================================================
  * Lazy apply, OperatorSum::map_acc (operator_abel.hpp:36-42) over
  * Operator::map_acc (operator.hpp:215-232): for each term k and source i,
  * expand with init = (c_k * coeff) * from_i and scatter to[target] += leaf,

@@ -18,6 +18,8 @@ STATUS = {
 }
 QD_OP_BIT = 0
 QD_OP_HOP = 1
+QD_OP_MODE = 2
+QD_MAX_MODE_DIM = 16
 QD_MAX_OPS_PER_TERM = 8
 
 
@@ -106,12 +108,14 @@ SIGNATURES = {
     "qd_memcpy_d2h": (I, [VP, VP, VP, ctypes.c_size_t]),
     "qd_basis_create": (I, [VP, I, ctypes.POINTER(qd_constraint), I, PP]),
     "qd_basis_create_q": (I, [VP, I, ctypes.POINTER(qd_quantity), I, PP]),
+    "qd_basis_create_modes": (I, [VP, I, I, ctypes.POINTER(qd_quantity), I, PP]),
     "qd_basis_destroy": (I, [VP]),
     "qd_basis_size": (I64, [VP]),
     "qd_basis_download": (I, [VP, VP]),
     "qd_basis_device_states": (VP, [VP]),
     "qd_basis_rank": (I, [VP, VP, I64, VP]),
     "qd_termprog_create": (I, [VP, I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, PP]),
+    "qd_termprog_create_modes": (I, [VP, I, I, ctypes.POINTER(qd_term), I, ctypes.POINTER(qd_op), I, DP, I, PP]),
     "qd_termprog_destroy": (I, [VP]),
     "qd_csr_build": (I, [VP, VP, VP, PP]),
     "qd_csr_destroy": (I, [VP]),

@@ -99,8 +99,10 @@ static int analyse_sector(int nbits, const qd_constraint* cons, int ncons, Secto
 
 // ------------------------------------------------------------ K1: unrank
 __global__ void __launch_bounds__(256) unrank_kernel(SectorDesc sec, uint64_t* __restrict__ out) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= sec.total) return;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 >= sec.total) return;
+  const bool comp = sec.mdim > 1;  // composite: i0 = rank(q) * mdim + m
+  uint64_t i = comp ? i0 / sec.mdim : i0;
   uint64_t x = 0;
   for (int b = 0; b < sec.nblocks; ++b) {
     uint64_t rb = (i / sec.stride[b]) % sec.size[b];
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(256) unrank_kernel(SectorDesc sec, uint64_t* _
     }
     x |= f << sec.lo[b];
   }
-  out[i] = x;
+  out[i0] = comp ? x * sec.mdim + i0 % sec.mdim : x;
 }
 
 // ------------------------------------------------------------ general path
@@ -225,6 +227,14 @@ static int select_states(qd_ctx* ctx, qd_basis* b, int nbits, const QList& c, co
   b->sec.nbits = nbits;
   b->sec.total = (uint64_t)total;
   return QD_OK;
+}
+
+// composite table of a table-ranked qubit sector: out[i] = q[i / mdim] * mdim + i % mdim
+__global__ void __launch_bounds__(256) expand_modes_kernel(const uint64_t* __restrict__ qs, uint64_t nq,
+                                                           uint64_t mdim, uint64_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq * mdim) return;
+  out[i] = qs[i / mdim] * mdim + i % mdim;
 }
 
 // ------------------------------------------------------------ K2: rank
@@ -352,6 +362,56 @@ extern "C" int qd_basis_create_q(qd_ctx* ctx, int nbits, const qd_quantity* q, i
   if (rc != QD_OK) {
     delete b;
     return rc;
+  }
+  *out = b;
+  return QD_OK;
+}
+
+extern "C" int qd_basis_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_quantity* q, int nq,
+                                     qd_basis** out) {
+  if (!ctx || !out || nbits < 0 || nbits > 48 || mdim < 2 || mdim > QD_MAX_MODE_DIM || nq < 0 || nq > 16 ||
+      (nq > 0 && !q))
+    return qd_set_error(ctx, QD_E_ARG, "qd_basis_create_modes: bad arguments");
+  // the qubit sector first (its table, or its CNS form), then the mode digit
+  qd_basis* qb = nullptr;
+  int rc = qd_basis_create_q(ctx, nbits, q, nq, &qb);
+  if (rc != QD_OK) return rc;
+  *out = nullptr;
+  hipSetDevice(ctx->device);
+  qd_basis* b = new qd_basis();
+  b->ctx = ctx;
+  b->sec = qb->sec;
+  b->sec.mdim = (uint64_t)mdim;
+  const unsigned __int128 total = (unsigned __int128)qb->sec.total * (unsigned)mdim;
+  if (total > ((unsigned __int128)1 << 62)) {
+    qd_basis_destroy(qb);
+    delete b;
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "composite sector too large");
+  }
+  b->sec.total = (uint64_t)total;
+  if (b->sec.total == 0) {
+    qd_basis_destroy(qb);
+    *out = b;
+    return QD_OK;
+  }
+  if (hipMalloc(&b->d_states, sizeof(uint64_t) * b->sec.total) != hipSuccess) {
+    qd_basis_destroy(qb);
+    delete b;
+    return qd_set_error(ctx, QD_E_OOM, "composite basis table");
+  }
+  if (b->sec.cns) {
+    unrank_kernel<<<qd_grid((int64_t)b->sec.total, 256), 256, 0, ctx->stream>>>(b->sec, b->d_states);
+  } else {
+    expand_modes_kernel<<<qd_grid((int64_t)b->sec.total, 256), 256, 0, ctx->stream>>>(qb->d_states, qb->sec.total,
+                                                                                    (uint64_t)mdim, b->d_states);
+  }
+  hipError_t le = hipGetLastError();
+  hipStreamSynchronize(ctx->stream);
+  qd_basis_destroy(qb);
+  if (le != hipSuccess) {
+    hipFree(b->d_states);
+    delete b;
+    return qd_set_error(ctx, QD_E_HIP, "composite basis: %s", hipGetErrorString(le));
   }
   *out = b;
   return QD_OK;

@@ -45,12 +45,18 @@ __device__ inline int list_add(int32_t* key, cplx* acc, int& ne, int32_t k, cplx
 
 #define MAXDUP 32
 
-template <int CAP, bool FILL>
+// mode gates of a composite index (MODES kernels only)
+struct ModeArgs {
+  int mdim;
+  const cplx* mats;
+};
+
+template <int CAP, bool FILL, bool MODES>
 __global__ void __launch_bounds__(128)
     csr_rows_kernel(SectorDesc sec, const uint64_t* __restrict__ states, int64_t n,
                     const DevTerm* __restrict__ terms, int nterms, const DevOp* __restrict__ ops,
                     int32_t* __restrict__ counts, const int64_t* __restrict__ offs, int32_t* __restrict__ inner,
-                    cplx* __restrict__ vals, int* __restrict__ flag) {
+                    cplx* __restrict__ vals, int* __restrict__ flag, ModeArgs ma) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t s = states[i];
@@ -62,12 +68,17 @@ __global__ void __launch_bounds__(128)
     const DevTerm t = terms[k];
     const DevOp* tops = ops + t.op_begin;
     int nbitops = 0;
-    for (int q = 0; q < t.n_ops; ++q) nbitops += (tops[q].kind == QD_OP_BIT);
+    if (!MODES)
+      for (int q = 0; q < t.n_ops; ++q) nbitops += (tops[q].kind == QD_OP_BIT);
+    auto leaf = [&](int L, uint64_t* x, cplx* v) {
+      return MODES ? eval_leaf_modes(tops, t.n_ops, s, L, ma.mdim, ma.mats, x, v)
+                   : eval_leaf(tops, t.n_ops, nbitops, s, L, x, v);
+    };
     if (!t.may_dup) {
       for (int L = 0; L < t.n_leaves; ++L) {
         uint64_t x;
         cplx v;
-        if (!eval_leaf(tops, t.n_ops, nbitops, s, L, &x, &v)) continue;
+        if (!leaf(L, &x, &v)) continue;
         if (!(hypot(v.x, v.y) > QD_LEAF_EPS)) continue;
         uint64_t r = qd_rank(sec, states, x);
         if (r >= (uint64_t)n) continue;
@@ -84,7 +95,7 @@ __global__ void __launch_bounds__(128)
       for (int L = 0; L < t.n_leaves; ++L) {
         uint64_t x;
         cplx v;
-        if (!eval_leaf(tops, t.n_ops, nbitops, s, L, &x, &v)) continue;
+        if (!leaf(L, &x, &v)) continue;
         if (!(hypot(v.x, v.y) > QD_LEAF_EPS)) continue;
         uint64_t r = qd_rank(sec, states, x);
         if (r >= (uint64_t)n) continue;
@@ -170,27 +181,40 @@ __global__ void herm_check_kernel(int64_t n, const int32_t* __restrict__ outer, 
   }
 }
 
-template <int CAP>
+template <int CAP, bool MODES>
 static int run_pass(qd_ctx* ctx, qd_basis* basis, int64_t n, const DevTerm* dterms, int nterms, const DevOp* dops,
-                    bool fill, int32_t* counts, const int64_t* offs, int32_t* inner, cplx* vals, int* dflag) {
+                    bool fill, int32_t* counts, const int64_t* offs, int32_t* inner, cplx* vals, int* dflag,
+                    ModeArgs ma) {
   const int B = 128;
   if (fill)
-    csr_rows_kernel<CAP, true><<<qd_grid(n, B), B, 0, ctx->stream>>>(basis->sec, basis->d_states, n, dterms, nterms,
-                                                                      dops, counts, offs, inner, vals, dflag);
+    csr_rows_kernel<CAP, true, MODES><<<qd_grid(n, B), B, 0, ctx->stream>>>(
+        basis->sec, basis->d_states, n, dterms, nterms, dops, counts, offs, inner, vals, dflag, ma);
   else
-    csr_rows_kernel<CAP, false><<<qd_grid(n, B), B, 0, ctx->stream>>>(basis->sec, basis->d_states, n, dterms,
-                                                                       nterms, dops, counts, offs, inner, vals, dflag);
+    csr_rows_kernel<CAP, false, MODES><<<qd_grid(n, B), B, 0, ctx->stream>>>(
+        basis->sec, basis->d_states, n, dterms, nterms, dops, counts, offs, inner, vals, dflag, ma);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }
 
+template <bool MODES>
+static int pass_cap(int cap, qd_ctx* ctx, qd_basis* basis, int64_t n, const DevTerm* dterms, int nterms,
+                    const DevOp* dops, bool fill, int32_t* counts, const int64_t* offs, int32_t* inner, cplx* vals,
+                    int* dflag, ModeArgs ma) {
+  if (cap <= 32)
+    return run_pass<32, MODES>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
+  if (cap <= 64)
+    return run_pass<64, MODES>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
+  if (cap <= 256)
+    return run_pass<256, MODES>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
+  return run_pass<1024, MODES>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
+}
+
 static int pass_dispatch(int cap, qd_ctx* ctx, qd_basis* basis, int64_t n, const DevTerm* dterms, int nterms,
                          const DevOp* dops, bool fill, int32_t* counts, const int64_t* offs, int32_t* inner,
-                         cplx* vals, int* dflag) {
-  if (cap <= 32) return run_pass<32>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag);
-  if (cap <= 64) return run_pass<64>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag);
-  if (cap <= 256) return run_pass<256>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag);
-  return run_pass<1024>(ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag);
+                         cplx* vals, int* dflag, ModeArgs ma) {
+  if (ma.mdim > 1)
+    return pass_cap<true>(cap, ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
+  return pass_cap<false>(cap, ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
 }
 
 int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoint, qd_csr** out) {
@@ -200,6 +224,10 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "sector of %lld states exceeds int32 CSR indices", (long long)n);
   if (prog->nbits > basis->sec.nbits)
     return qd_set_error(ctx, QD_E_ARG, "program acts on %d bits, basis has %d", prog->nbits, basis->sec.nbits);
+  const int bmdim = basis->sec.mdim > 1 ? (int)basis->sec.mdim : 1;
+  if (bmdim != prog->mdim)
+    return qd_set_error(ctx, QD_E_ARG, "program mode dimension %d, basis %d", prog->mdim, bmdim);
+  const ModeArgs ma{prog->mdim, prog->d_mats};
   const DevTerm* dterms = prog->d_terms;
   const DevOp* dops = prog->d_ops;
   if (adjoint) {
@@ -241,7 +269,7 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
   // pass 1 with growing capacity
   for (;;) {
     hipMemsetAsync(dflag, 0, 2 * sizeof(int), ctx->stream);
-    if (n > 0) rc = pass_dispatch(cap, ctx, basis, n, dterms, nterms, dops, false, counts, nullptr, nullptr, nullptr, dflag);
+    if (n > 0) rc = pass_dispatch(cap, ctx, basis, n, dterms, nterms, dops, false, counts, nullptr, nullptr, nullptr, dflag, ma);
     if (rc) return fail(rc, "count");
     hipMemcpyAsync(hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
     hipStreamSynchronize(ctx->stream);
@@ -269,7 +297,7 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
   if (n > 0) {
     hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream);
     rc = pass_dispatch(hflag[1] > cap ? 1024 : (hflag[1] > 0 ? hflag[1] : 1), ctx, basis, n, dterms, nterms, dops,
-                       true, counts, offs, c->d_inner, c->d_vals, dflag);
+                       true, counts, offs, c->d_inner, c->d_vals, dflag, ma);
     if (rc) return fail(rc, "fill");
     hipMemcpyAsync(hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
     hipStreamSynchronize(ctx->stream);

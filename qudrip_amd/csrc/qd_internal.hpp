@@ -75,6 +75,7 @@ struct SectorDesc {
   uint64_t stride[QD_MAX_BLOCKS];
   uint64_t total;
   int cns;  // 1: CNS rank/unrank valid; 0: table + binary search
+  uint64_t mdim;  // >= 2: composite states q * mdim + m (a boson mode); 0 or 1: none
 };
 
 // rank of x within the sector, or sec.total when x is outside it
@@ -118,6 +119,10 @@ __device__ inline uint64_t qd_rank_table(const uint64_t* states, uint64_t n, uin
 }
 
 __device__ inline uint64_t qd_rank(const SectorDesc& sec, const uint64_t* states, uint64_t x) {
+  if (sec.mdim > 1 && sec.cns) {  // composite CNS sector: rank(q) * mdim + m
+    const uint64_t r = qd_rank_cns(sec, x / sec.mdim);
+    return r >= sec.total ? sec.total : r * sec.mdim + x % sec.mdim;
+  }
   return sec.cns ? qd_rank_cns(sec, x) : qd_rank_table(states, sec.total, x);
 }
 
@@ -177,6 +182,58 @@ __device__ inline bool eval_leaf(const DevOp* __restrict__ ops, int n_ops, int n
   return true;
 }
 
+// Leaves of one term over composite states x = q * mdim + m (a boson mode
+// next to the qubits, Indices, index.hpp:112-184): bit and hop gates act on q
+// as in eval_leaf, a mode gate (operators/mode.hpp:80-103) takes m to
+// output j with value U(m, j) * val, where mats[op.a * mdim * mdim + m *
+// mdim + j] = U(m, j).  Each op contributes a digit of L (bit 2, hop 1, mode
+// mdim children), the first-applied op the most significant, as the tree of
+// operator.hpp:135-206 orders its leaves.
+__device__ inline bool eval_leaf_modes(const DevOp* __restrict__ ops, int n_ops, uint64_t x, int L, int mdim,
+                                       const cplx* __restrict__ mats, uint64_t* out_x, cplx* out_v) {
+  int digit[QD_MAX_OPS_PER_TERM];
+  for (int iop = 0; iop < n_ops; ++iop) {  // least significant: the last-applied op
+    const int r = ops[iop].kind == QD_OP_BIT ? 2 : ops[iop].kind == QD_OP_MODE ? mdim : 1;
+    digit[iop] = L % r;
+    L /= r;
+  }
+  uint64_t q = x / (uint64_t)mdim;
+  int m = (int)(x % (uint64_t)mdim);
+  cplx v = cmake(1.0, 0.0);
+  for (int iop = n_ops - 1; iop >= 0; --iop) {
+    const DevOp& op = ops[iop];
+    if (op.kind == QD_OP_BIT) {
+      const int s = op.a, o = digit[iop];
+      const int b = (int)((q >> s) & 1ull);
+      const int prev_set = __popcll(q >> (s + 1));
+      const cplx jw = cmake((prev_set & 1) ? (double)op.string : 1.0, 0.0);
+      v = cmul(cmul(jw, op.U[b * 2 + o]), v);
+      q = (q & ~(1ull << s)) | ((uint64_t)o << s);
+    } else if (op.kind == QD_OP_MODE) {
+      const int j = digit[iop];
+      v = cmul(mats[((size_t)op.a * mdim + m) * mdim + j], v);
+      m = j;
+    } else {
+      const int a = op.a, b = op.b;
+      if (a == b) {
+        v = crmul((double)((q >> a) & 1ull), v);
+      } else if (((q >> b) & 1ull) == 1ull && ((q >> a) & 1ull) == 0ull) {
+        const int jw1 = __popcll(q >> (b + 1));
+        const uint64_t q1 = q & ~(1ull << b);
+        const int jw2 = __popcll(q1 >> (a + 1));
+        const int sgn = ((jw1 & 1) ? op.string : 1) * ((jw2 & 1) ? op.string : 1);
+        v = cmul(cmake((double)sgn, 0.0), v);
+        q = q1 | (1ull << a);
+      } else {
+        return false;
+      }
+    }
+  }
+  *out_x = q * (uint64_t)mdim + (uint64_t)m;
+  *out_v = v;
+  return true;
+}
+
 // --------------------------------------------------------------- handles
 // Caching pool for large per-solve device buffers (Krylov bases, request
 // vectors).  Blocks freed into it are kept and handed to later requests of a
@@ -226,6 +283,12 @@ struct qd_termprog {
   DevTerm* d_adj_terms = nullptr;
   DevOp* d_adj_ops = nullptr;
   int max_leaves = 1;
+  // boson mode (qd_termprog_create_modes): mdim >= 2, mode-gate matrices
+  // (nmats of the program, then their adjoints), mdim * mdim each
+  int mdim = 1;
+  int nmats = 0;
+  std::vector<cplx> h_mats;
+  cplx* d_mats = nullptr;
 };
 
 struct qd_csr {

@@ -416,6 +416,8 @@ extern "C" int qd_linop_matfree(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog,
   *out = nullptr;
   if (prog->nbits > basis->sec.nbits)
     return qd_set_error(ctx, QD_E_ARG, "program acts on %d bits, basis has %d", prog->nbits, basis->sec.nbits);
+  if (prog->mdim > 1 || basis->sec.mdim > 1)
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "qd_linop_matfree: a boson mode needs the stored matrix (qd_csr_build)");
   int rc = qd_termprog_ensure_adjoint(prog);
   if (rc) return rc;
   qd_linop* op = new qd_linop();

@@ -8,17 +8,24 @@
 // U'(i,j) = conj(U(j,i)) on the same site (the Jordan-Wigner factor only
 // depends on the bits above the site, which the gate leaves unchanged), and
 // for hopBitOp (operators/hopbit.hpp:43-84) (c+_a c_b)^dagger = c+_b c_a
-// carries the same sign: the two popcounts exchange roles.
+// carries the same sign: the two popcounts exchange roles.  A mode gate
+// (operators/mode.hpp) on a composite index gets the conjugate transpose of
+// its matrix, stored after the program's own matrices.
 #include "qd_internal.hpp"
 
-static void finish_terms(std::vector<DevTerm>& terms, const std::vector<DevOp>& ops, int* max_leaves) {
+static void finish_terms(std::vector<DevTerm>& terms, const std::vector<DevOp>& ops, int* max_leaves,
+                         int mdim = 1) {
   int ml = 1;
   for (auto& t : terms) {
     int nl = 1;
     int dup = 0;
+    int nmode = 0;
     for (int i = t.n_ops - 1; i >= 0; --i) {  // application order
       const DevOp& oi = ops[t.op_begin + i];
-      if (oi.kind == QD_OP_BIT) {
+      if (oi.kind == QD_OP_MODE) {
+        nl *= mdim;
+        if (++nmode >= 2) dup = 1;  // two mode gates: several paths reach one mode
+      } else if (oi.kind == QD_OP_BIT) {
         nl *= 2;
         // a later-applied op touching the same site can merge two leaves
         for (int j = i - 1; j >= 0; --j) {
@@ -35,18 +42,33 @@ static void finish_terms(std::vector<DevTerm>& terms, const std::vector<DevOp>& 
   *max_leaves = ml;
 }
 
-extern "C" int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, int nterms, const qd_op* ops,
-                                  int nops, qd_termprog** out) {
+static int termprog_create(qd_ctx* ctx, int nbits, int mdim, const qd_term* terms, int nterms, const qd_op* ops,
+                           int nops, const double* mats, int nmats, qd_termprog** out) {
   if (!ctx || !out || nterms < 0 || nops < 0 || (nterms && !terms) || (nops && !ops) || nbits < 1 ||
-      nbits > QD_MAX_BITS)
+      nbits > QD_MAX_BITS || mdim < 1 || mdim > QD_MAX_MODE_DIM || nmats < 0 || (nmats && !mats) ||
+      (mdim == 1 && nmats > 0))
     return qd_set_error(ctx, QD_E_ARG, "qd_termprog_create: bad arguments");
   *out = nullptr;
   qd_termprog* p = new qd_termprog();
   p->ctx = ctx;
   p->nbits = nbits;
+  p->mdim = mdim;
+  p->nmats = nmats;
   p->h_ops.resize(nops);
   for (int i = 0; i < nops; ++i) {
     const qd_op& o = ops[i];
+    if (o.kind == QD_OP_MODE) {
+      if (mdim < 2 || o.site_a < 0 || o.site_a >= nmats) {
+        delete p;
+        return qd_set_error(ctx, QD_E_ARG, "op %d: mode gate without a matrix (or no mode)", i);
+      }
+      DevOp d{};
+      d.kind = QD_OP_MODE;
+      d.a = d.b = o.site_a;
+      d.string = 1;
+      p->h_ops[i] = d;
+      continue;
+    }
     if ((o.kind != QD_OP_BIT && o.kind != QD_OP_HOP) || o.site_a < 0 || o.site_a >= nbits ||
         (o.kind == QD_OP_HOP && (o.site_b < 0 || o.site_b >= nbits)) || (o.string != 1 && o.string != -1)) {
       delete p;
@@ -73,11 +95,41 @@ extern "C" int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, 
     d.n_ops = t.n_ops;
     p->h_terms[k] = d;
   }
-  finish_terms(p->h_terms, p->h_ops, &p->max_leaves);
+  for (int k = 0; k < nterms; ++k) {  // leaves per term: 2 per bit gate, mdim per mode gate
+    int64_t nl = 1;
+    for (int i = 0; i < p->h_terms[k].n_ops; ++i) {
+      const int kind = p->h_ops[p->h_terms[k].op_begin + i].kind;
+      nl *= kind == QD_OP_BIT ? 2 : kind == QD_OP_MODE ? mdim : 1;
+    }
+    if (nl > (1 << 20)) {
+      delete p;
+      return qd_set_error(ctx, QD_E_UNSUPPORTED, "term %d has %lld leaves (at most 2^20)", k, (long long)nl);
+    }
+  }
+  finish_terms(p->h_terms, p->h_ops, &p->max_leaves, mdim);
   hipSetDevice(ctx->device);
+  if (nmats > 0) {
+    // the matrices, then their conjugate transposes (for the adjoint program)
+    const size_t mm = (size_t)mdim * mdim;
+    p->h_mats.resize(2 * mm * nmats);
+    for (int k = 0; k < nmats; ++k)
+      for (int i = 0; i < mdim; ++i)
+        for (int j = 0; j < mdim; ++j) {
+          const double* u = mats + 2 * (k * mm + (size_t)i * mdim + j);
+          p->h_mats[k * mm + (size_t)i * mdim + j] = cmake(u[0], u[1]);
+          p->h_mats[(nmats + k) * mm + (size_t)j * mdim + i] = cmake(u[0], -u[1]);
+        }
+    if (hipMalloc(&p->d_mats, sizeof(cplx) * p->h_mats.size()) != hipSuccess) {
+      delete p;
+      return qd_set_error(ctx, QD_E_OOM, "mode matrices");
+    }
+    hipMemcpyAsync(p->d_mats, p->h_mats.data(), sizeof(cplx) * p->h_mats.size(), hipMemcpyHostToDevice,
+                   ctx->stream);
+  }
   size_t tb = sizeof(DevTerm) * (nterms ? nterms : 1), ob = sizeof(DevOp) * (nops ? nops : 1);
   if (hipMalloc(&p->d_terms, tb) != hipSuccess || hipMalloc(&p->d_ops, ob) != hipSuccess) {
     hipFree(p->d_terms);
+    hipFree(p->d_mats);
     delete p;
     return qd_set_error(ctx, QD_E_OOM, "term program upload");
   }
@@ -87,11 +139,24 @@ extern "C" int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, 
   if (e != hipSuccess) {
     hipFree(p->d_terms);
     hipFree(p->d_ops);
+    hipFree(p->d_mats);
     delete p;
     return qd_set_error(ctx, QD_E_HIP, "term program upload: %s", hipGetErrorString(e));
   }
   *out = p;
   return QD_OK;
+}
+
+extern "C" int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, int nterms, const qd_op* ops,
+                                  int nops, qd_termprog** out) {
+  return termprog_create(ctx, nbits, 1, terms, nterms, ops, nops, nullptr, 0, out);
+}
+
+extern "C" int qd_termprog_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_term* terms, int nterms,
+                                        const qd_op* ops, int nops, const double* mats, int nmats,
+                                        qd_termprog** out) {
+  if (mdim < 2) return qd_set_error(ctx, QD_E_ARG, "qd_termprog_create_modes: mdim %d", mdim);
+  return termprog_create(ctx, nbits, mdim, terms, nterms, ops, nops, mats, nmats, out);
 }
 
 int qd_termprog_ensure_adjoint(qd_termprog* p) {
@@ -107,7 +172,9 @@ int qd_termprog_ensure_adjoint(qd_termprog* p) {
     d.op_begin = pos;
     for (int i = t.n_ops - 1; i >= 0; --i) {
       DevOp o = p->h_ops[t.op_begin + i];
-      if (o.kind == QD_OP_BIT) {
+      if (o.kind == QD_OP_MODE) {
+        o.a = o.b = p->nmats + o.a;  // its conjugate transpose
+      } else if (o.kind == QD_OP_BIT) {
         DevOp a = o;
         for (int r = 0; r < 2; ++r)
           for (int c = 0; c < 2; ++c) a.U[r * 2 + c] = cconj(o.U[c * 2 + r]);
@@ -122,7 +189,7 @@ int qd_termprog_ensure_adjoint(qd_termprog* p) {
     at[k] = d;
   }
   int ml = 1;
-  finish_terms(at, ao, &ml);
+  finish_terms(at, ao, &ml, p->mdim);
   size_t tb = sizeof(DevTerm) * (at.size() ? at.size() : 1), ob = sizeof(DevOp) * (ao.size() ? ao.size() : 1);
   if (hipMalloc(&p->d_adj_terms, tb) != hipSuccess || hipMalloc(&p->d_adj_ops, ob) != hipSuccess) {
     hipFree(p->d_adj_terms);
@@ -143,6 +210,7 @@ extern "C" int qd_termprog_destroy(qd_termprog* p) {
   hipFree(p->d_ops);
   if (p->d_adj_terms) hipFree(p->d_adj_terms);
   if (p->d_adj_ops) hipFree(p->d_adj_ops);
+  if (p->d_mats) hipFree(p->d_mats);
   delete p;
   return QD_OK;
 }

@@ -36,8 +36,38 @@ class QbitsIndex:
         return 1 << self.nbits
 
 
+    def __mul__(self, other):  # operator* of two indices (index.hpp:177-184)
+        if isinstance(other, SingleModeIndex):
+            return Indices(self, other)
+        return NotImplemented
+
+
 def getQbits(N):
     return QbitsIndex(N)
+
+
+class SingleModeIndex:
+    """SingleModeIndex (index.hpp:94-110): one boson mode, levels 0..n_mode-1."""
+
+    def __init__(self, n_mode):
+        self.mdim = int(n_mode)
+
+    def range(self):
+        return self.mdim
+
+
+class Indices:
+    """Indices (index.hpp:112-175) of a QbitsIndex and a SingleModeIndex:
+    value = q * mdim + m, the mode the least significant digit."""
+
+    def __init__(self, lhs, rhs):
+        self.lhs = lhs
+        self.rhs = rhs
+        self.nbits = lhs.nbits
+        self.mdim = rhs.mdim
+
+    def range(self):
+        return (1 << self.nbits) * self.mdim
 
 
 class Constraint:
@@ -115,6 +145,7 @@ class Sector:
     def __init__(self, index, constraints):
         self.index = index
         self.nbits = index.nbits
+        self.mdim = getattr(index, "mdim", 1)
         self.constraints = list(constraints)
 
     @property
@@ -178,6 +209,59 @@ class Gate:
         if site1 is None:  # hopbit.hpp:17-22: gate(site2, site1) = c+_{site2} c_{site1}
             raise TypeError("hop gate takes (site2, site1)")
         return Operator([_OpRecord(self.kind, int(site), int(site1), self.string, self.U.copy())])
+
+
+class ModeGate:
+    """modeOp prototype (operators/mode.hpp:6-35): `gate << U` stores an
+    mdim x mdim matrix, gate(site) makes a fresh instance (the site is not
+    used); the instance takes mode m to sum_j U(m, j) |j> (mode.hpp:80-103)."""
+
+    def __init__(self, index):
+        self.index = index
+        self.mdim = index.mdim
+        self.U = np.zeros((self.mdim, self.mdim), dtype=np.complex128)
+
+    def __lshift__(self, U):
+        self.U = np.array(U, dtype=np.complex128).reshape(self.mdim, self.mdim).copy()
+        return self
+
+    def __call__(self, site=0):
+        return Operator([_OpRecord(abi.QD_OP_MODE, 0, 0, 1, self.U.copy())])
+
+
+def getModeOp(index):
+    return ModeGate(index)
+
+
+MODE_UPPER, MODE_LOWER = 1, -1  # enum class Mode (mode.hpp:36)
+
+
+def LadderMatrix(ndim, mode=MODE_UPPER):
+    """mode.hpp:38-52: Upper puts sqrt(i+1) at (i+1, i), Lower at (i, i+1)."""
+    m = np.zeros((ndim, ndim), dtype=np.complex128)
+    for i in range(ndim - 1):
+        if mode == MODE_UPPER:
+            m[i + 1, i] = np.sqrt(i + 1)
+        else:
+            m[i, i + 1] = np.sqrt(i + 1)
+    return m
+
+
+def CoherentProjector(ndim, amp):
+    """mode.hpp:55-67: |a><a| of the truncated coherent state."""
+    a = np.zeros(ndim, dtype=np.complex128)
+    a[0] = 1.0
+    for i in range(1, ndim):
+        a[i] = a[i - 1] * amp / np.sqrt(float(i))
+    a *= np.exp(-abs(amp) * abs(amp) / 2.0)
+    return np.outer(a, a.conj())
+
+
+def FockProjector(ndim, idx):
+    """mode.hpp:70-77: |idx><idx|."""
+    m = np.zeros((ndim, ndim), dtype=np.complex128)
+    m[idx, idx] = 1.0
+    return m
 
 
 def getBoseGate(index):
@@ -293,8 +377,27 @@ class OperatorSum:
             return self * other
         return NotImplemented
 
-    def program(self):
-        """Flatten to (terms, ops) ctypes arrays in the reference's order."""
+    def has_modes(self):
+        return any(r.kind == abi.QD_OP_MODE for op, _ in self.terms for r in op.ops)
+
+    def program_modes(self):
+        """(terms, nterms, ops, nops, mats, nmats, mdim): as program(), with
+        every mode gate's matrix appended to mats (its site_a points there)."""
+        recs = [r for op, _ in self.terms for r in op.ops if r.kind == abi.QD_OP_MODE]
+        mdim = recs[0].U.shape[0] if recs else 1
+        if any(r.U.shape != (mdim, mdim) for r in recs):
+            raise ValueError("mode gates of one program must share the mode dimension")
+        mats = []
+        terms, nt, ops, no = self.program(allow_modes=True, mats=mats)
+        flat = np.ascontiguousarray(np.array(mats, dtype=np.complex128).reshape(-1)).view(np.float64) \
+            if mats else np.zeros(2)
+        return terms, nt, ops, no, flat, len(mats), mdim
+
+    def program(self, allow_modes=False, mats=None):
+        """Flatten to (terms, ops) ctypes arrays in the reference's order; a
+        mode gate's matrix is appended to mats, its site fields point there."""
+        if not allow_modes and self.has_modes():
+            raise ValueError("a program with mode gates: use program_modes()")
         nops = sum(len(op.ops) for op, _ in self.terms)
         terms = (abi.qd_term * max(1, len(self.terms)))()
         ops = (abi.qd_op * max(1, nops))()
@@ -312,10 +415,14 @@ class OperatorSum:
                 o.site_a = rec.site_a
                 o.site_b = rec.site_b
                 o.string = rec.string
-                for q in range(4):
-                    u = rec.U[q // 2, q % 2]
-                    o.U[2 * q] = u.real
-                    o.U[2 * q + 1] = u.imag
+                if rec.kind == abi.QD_OP_MODE:
+                    o.site_a = o.site_b = len(mats)
+                    mats.append(rec.U)
+                else:
+                    for q in range(4):
+                        u = rec.U[q // 2, q % 2]
+                        o.U[2 * q] = u.real
+                        o.U[2 * q + 1] = u.imag
                 pos += 1
         return terms, len(self.terms), ops, nops
 
@@ -323,7 +430,8 @@ class OperatorSum:
         m = 0
         for op, _ in self.terms:
             for r in op.ops:
-                m = max(m, r.site_a + 1, r.site_b + 1)
+                if r.kind != abi.QD_OP_MODE:
+                    m = max(m, r.site_a + 1, r.site_b + 1)
         return m
 
 

@@ -61,7 +61,11 @@ class Basis:
     def __init__(self, ctx, sector):
         self.ctx = ctx
         p = ctypes.c_void_p()
-        if sector.plain:
+        if sector.mdim > 1:
+            arr, n = sector.ctypes_quantities()
+            ctx.check(ctx.lib.qd_basis_create_modes(ctx.ptr, sector.nbits, sector.mdim, arr, n, ctypes.byref(p)),
+                      "basis_create_modes")
+        elif sector.plain:
             arr, n = sector.ctypes_constraints()
             ctx.check(ctx.lib.qd_basis_create(ctx.ptr, sector.nbits, arr, n, ctypes.byref(p)), "basis_create")
         else:
@@ -95,10 +99,17 @@ class Basis:
 class TermProgram:
     def __init__(self, ctx, opsum, nbits):
         self.ctx = ctx
-        terms, nt, ops, no = opsum.program()
-        self._keep = (terms, ops)
         p = ctypes.c_void_p()
-        ctx.check(ctx.lib.qd_termprog_create(ctx.ptr, nbits, terms, nt, ops, no, ctypes.byref(p)), "termprog")
+        if opsum.has_modes():
+            terms, nt, ops, no, mats, nm, mdim = opsum.program_modes()
+            self._keep = (terms, ops, mats)
+            ctx.check(ctx.lib.qd_termprog_create_modes(ctx.ptr, nbits, mdim, terms, nt, ops, no,
+                                                       mats.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nm,
+                                                       ctypes.byref(p)), "termprog_modes")
+        else:
+            terms, nt, ops, no = opsum.program()
+            self._keep = (terms, ops)
+            ctx.check(ctx.lib.qd_termprog_create(ctx.ptr, nbits, terms, nt, ops, no, ctypes.byref(p)), "termprog")
         self.ptr = p
 
     def close(self):

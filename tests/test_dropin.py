@@ -95,6 +95,42 @@ def test_dropin_matches_python_mirror_and_oracle(tmp_path, model, N, a, b):
 
 
 @pytest.mark.gpu
+def test_dropin_boson_mode_jaynes_cummings(tmp_path):
+    """SingleModeIndex / Indices / getModeOp / LadderMatrix from C++: the
+    program (terms, ops, mode matrices), the `>>` arrays and H x against the
+    Python mirror and the oracle."""
+    prefix = str(tmp_path / "jc")
+    nd, g, D = 7, 0.37, 0.83
+    _run([os.path.join(BIN, "dropin_dump"), "jc", str(nd), prefix, repr(g), repr(D)])
+    q = Q.getQbits(1)
+    md = Q.SingleModeIndex(nd)
+    idx = q * md
+    sp, sm, ne = Q.getBoseGate(q), Q.getBoseGate(q), Q.getBoseGate(q)
+    sp << [[0, 1], [0, 0]]
+    sm << [[0, 0], [1, 0]]
+    ne << [[0, 0], [0, 1]]
+    ad, a = Q.getModeOp(md), Q.getModeOp(md)
+    ad << Q.LadderMatrix(nd, Q.MODE_LOWER)
+    a << Q.LadderMatrix(nd, Q.MODE_UPPER)
+    ham = Q.getEmptyOperator() + ad(0) * a(0)
+    ham = ham + D * ne(0)
+    ham = ham + g * (sp(0) * a(0)) + g * (sm(0) * ad(0))
+    terms, nt, ops, no, mats, nm, mdim = ham.program_modes()
+    assert _load(prefix, ".terms", np.uint8).tobytes() == bytes(terms)[: nt * 24]
+    assert _load(prefix, ".ops", np.uint8).tobytes() == bytes(ops)[: no * 80]
+    assert np.array_equal(_load(prefix, ".mats", np.float64), mats)
+    basis = oracle.basis_scan(Q.Constrain(idx))
+    ref = oracle.build_csc(basis, ham)
+    assert np.array_equal(_load(prefix, ".outer", np.int32), ref[0])
+    assert np.array_equal(_load(prefix, ".inner", np.int32), ref[1])
+    assert np.array_equal(_load(prefix, ".vals", np.uint64), ref[2].view(np.uint64))
+    x = _load(prefix, ".x", np.complex128)
+    y = _load(prefix, ".y", np.complex128)
+    yr = oracle.spmv(ref, x)
+    assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14
+
+
+@pytest.mark.gpu
 def test_dropin_lazy_product_matches_map_acc(tmp_path):
     """psi2 = ham * psi1 (the lazy path, operator.hpp:57-66 / operator_abel.hpp:36-49)."""
     prefix = str(tmp_path / "lazy")

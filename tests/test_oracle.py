@@ -386,3 +386,72 @@ def test_basis_scan_nd_nh_matches_site_definition(L, nup, ndn, nd, nh):
             continue
         ref.append(x)
     assert np.array_equal(got, np.array(ref, dtype=np.uint64))
+
+
+# ------------------------------------------------ boson mode (modeOp, Indices)
+def jaynes_cummings(nd, w, D, g):
+    """One qubit (bit 1 = excited) times a mode of nd levels:
+    H = w a+a + D |e><e| + g (s+ a + s- a+), written with mode gates (a+ a as
+    a product of two of them) in the reference's matrix convention."""
+    q = Q.getQbits(1)
+    md = Q.SingleModeIndex(nd)
+    idx = q * md
+    sp, sm, ne = Q.getBoseGate(q), Q.getBoseGate(q), Q.getBoseGate(q)
+    sp << [[0, 1], [0, 0]]
+    sm << [[0, 0], [1, 0]]
+    ne << [[0, 0], [0, 1]]
+    ad, a = Q.getModeOp(md), Q.getModeOp(md)
+    ad << Q.LadderMatrix(nd, Q.MODE_LOWER)
+    a << Q.LadderMatrix(nd, Q.MODE_UPPER)
+    ham = w * (ad(0) * a(0)) + D * ne(0) + g * (sp(0) * a(0)) + g * (sm(0) * ad(0))
+    return idx, ham
+
+
+def _kron_jc(nd, w, D, g):
+    a = np.diag(np.sqrt(np.arange(1, nd)), k=1).astype(np.complex128)  # a|m> = sqrt(m)|m-1>
+    sp = np.array([[0, 0], [1, 0]], dtype=np.complex128)                # |0> -> |1>
+    e = np.diag([0.0, 1.0]).astype(np.complex128)
+    return (w * np.kron(np.eye(2), a.conj().T @ a) + D * np.kron(e, np.eye(nd))
+            + g * (np.kron(sp, a) + np.kron(sp.conj().T, a.conj().T)))
+
+
+@pytest.mark.parametrize("nd", [2, 5, 9])
+def test_modes_jaynes_cummings_closed_form(nd):
+    """Composite index value q * nd + m; spectrum: 0, the 2x2 blocks of N
+    excitations [[w N, g sqrt N], [g sqrt N, w (N-1) + D]] for N < nd, and
+    w (nd-1) + D (the truncated manifold)."""
+    w, D, g = 1.0, 0.83, 0.37
+    idx, ham = jaynes_cummings(nd, w, D, g)
+    basis = oracle.basis_scan(Q.Constrain(idx))
+    assert np.array_equal(basis, np.arange(2 * nd, dtype=np.uint64))
+    H = oracle.csc_to_dense(oracle.build_csc(basis, ham))
+    assert np.allclose(H, _kron_jc(nd, w, D, g), atol=1e-15)
+    ref = [0.0, w * (nd - 1) + D]
+    for N in range(1, nd):
+        ref += list(np.linalg.eigvalsh(np.array([[w * N, g * np.sqrt(N)], [g * np.sqrt(N), w * (N - 1) + D]])))
+    assert np.allclose(np.linalg.eigvalsh(H), sorted(ref), atol=1e-13)
+
+
+def test_modes_constrained_sector_and_non_hermitian_term():
+    """Two qubits with one excitation (Nset = 1) times a 3-level mode: the
+    sector is {q * 3 + m : popcount(q) = 1}; a lone a+ term (not Hermitian)
+    and a hop, against np.kron on the full space restricted to the sector."""
+    nd = 3
+    q = Q.getQbits(2)
+    md = Q.SingleModeIndex(nd)
+    idx = q * md
+    gc, ga = Q.getBoseGate(q), Q.getBoseGate(q)
+    gc << [[0, 1], [0, 0]]
+    ga << [[0, 0], [1, 0]]
+    ad = Q.getModeOp(md)
+    ad << Q.LadderMatrix(nd, Q.MODE_LOWER)
+    ham = 0.7 * (gc(1) * ga(0)) + 0.7 * (gc(0) * ga(1)) + (0.25 - 0.5j) * ad(0)
+    basis = oracle.basis_scan(Q.Constrain(idx, Q.getNset(q) == 1))
+    assert np.array_equal(basis, np.array([3, 4, 5, 6, 7, 8], dtype=np.uint64))
+    H = oracle.csc_to_dense(oracle.build_csc(basis, ham))
+    a = np.diag(np.sqrt(np.arange(1, nd)), k=1)
+    up = np.array([[0, 0], [1, 0]])  # bit |0> -> |1>
+    dn = up.T
+    Hq = 0.7 * (np.kron(up, dn) + np.kron(dn, up))  # bit 1 is the high bit of q
+    full = np.kron(Hq, np.eye(nd)) + (0.25 - 0.5j) * np.kron(np.eye(4), a.T)
+    assert np.allclose(H, full[np.ix_(basis.astype(int), basis.astype(int))], atol=1e-15)
