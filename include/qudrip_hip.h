@@ -180,7 +180,8 @@ int qd_termprog_create(qd_ctx* ctx, int nbits, const qd_term* terms, int nterms,
  * mdim * mdim interleaved complex numbers, U_[i*mdim+j] = U(i,j) each;
  * 2 <= mdim <= QD_MAX_MODE_DIM.  Such a program builds stored matrices
  * (qd_csr_build) over a basis made by qd_basis_create_modes with the same
- * mdim; it has no matrix-free product. */
+ * mdim; it has no matrix-free product.  A program without mode gates
+ * (qd_termprog_create) over such a basis acts as the identity on the mode. */
 int qd_termprog_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_term* terms, int nterms,
                              const qd_op* ops, int nops, const double* mats, int nmats, qd_termprog** out);
 int qd_termprog_destroy(qd_termprog* prog);

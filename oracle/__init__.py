@@ -123,10 +123,13 @@ def binom(n, k):
     return lib().orc_binom(n, k)
 
 
-def build_csc(basis, opsum):
-    """`ham >> index` (operator_abel.hpp:25-34, operator.hpp:235-267)."""
-    if opsum.has_modes():
-        terms, nt, ops, no, mats, nm, mdim = opsum.program_modes()
+def build_csc(basis, opsum, mdim=1):
+    """`ham >> index` (operator_abel.hpp:25-34, operator.hpp:235-267); mdim > 1:
+    basis holds composite values q * mdim + m (a program without mode gates
+    leaves the mode alone)."""
+    if opsum.has_modes() or mdim > 1:
+        terms, nt, ops, no, mats, nm, md = opsum.program_modes()
+        mdim = md if opsum.has_modes() else mdim
         n = len(basis)
         colptr = np.empty(n + 1, dtype=np.int32)
         nnz = lib().orc_build_csc_modes(_p(basis), n, mdim, terms, nt, ops, _p(mats), _p(colptr), None, None, 0)

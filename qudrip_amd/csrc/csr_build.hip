@@ -224,10 +224,11 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "sector of %lld states exceeds int32 CSR indices", (long long)n);
   if (prog->nbits > basis->sec.nbits)
     return qd_set_error(ctx, QD_E_ARG, "program acts on %d bits, basis has %d", prog->nbits, basis->sec.nbits);
+  // a program without mode gates acts as the identity on a basis's mode
   const int bmdim = basis->sec.mdim > 1 ? (int)basis->sec.mdim : 1;
-  if (bmdim != prog->mdim)
+  if (prog->mdim > 1 && bmdim != prog->mdim)
     return qd_set_error(ctx, QD_E_ARG, "program mode dimension %d, basis %d", prog->mdim, bmdim);
-  const ModeArgs ma{prog->mdim, prog->d_mats};
+  const ModeArgs ma{bmdim, prog->d_mats};
   const DevTerm* dterms = prog->d_terms;
   const DevOp* dops = prog->d_ops;
   if (adjoint) {

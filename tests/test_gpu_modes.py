@@ -59,7 +59,16 @@ def _sector(case):
         ad << Q.LadderMatrix(3, Q.MODE_LOWER)
         ham = 0.7 * (gc(1) * ga(0)) + 0.7 * (gc(0) * ga(1)) + (0.25 - 0.5j) * ad(0)
         return Q.Constrain(idx, Q.getNset(q) == 1), ham
-    # table-ranked qubit sector (double occupancy fixed) times a 4-level mode
+    if case == "hub4_nd1_mode4":  # table-ranked qubit sector (double occupancy fixed) times 4 levels
+        return _hub4_mode4()
+    # no mode gate at all: the program acts as the identity on the mode
+    q = Q.getQbits(6)
+    md = Q.SingleModeIndex(5)
+    _, ham = Q.xxz_chain(6, 0.8, 1.3, pbc=True)
+    return Q.Constrain(q * md, Q.getNset(q) == 3), ham
+
+
+def _hub4_mode4():
     L = 4
     q = Q.getQbits(2 * L)
     md = Q.SingleModeIndex(4)
@@ -72,14 +81,14 @@ def _sector(case):
     return Q.Constrain(idx, Q.getNup(q, L) == 2, Q.getNdo(q, L) == 2, Q.getNd(q, L) == 1), ham
 
 
-CASES = ["jc9", "holstein10_nd8", "two_qubits_nonherm", "hub4_nd1_mode4"]
+CASES = ["jc9", "holstein10_nd8", "two_qubits_nonherm", "hub4_nd1_mode4", "qubits_only_on_mode"]
 
 
 @pytest.mark.parametrize("case", CASES)
 def test_modes_basis_and_csr_bit_exact(ctx, case):
     sec, ham = _sector(case)
     ref_basis = oracle.basis_scan(sec)
-    r_outer, r_inner, r_vals = oracle.build_csc(ref_basis, ham)
+    r_outer, r_inner, r_vals = oracle.build_csc(ref_basis, ham, mdim=sec.mdim)
     b = Basis(ctx, sec)
     p = TermProgram(ctx, ham, sec.nbits)
     c = CSR(ctx, b, p)

@@ -455,3 +455,15 @@ def test_modes_constrained_sector_and_non_hermitian_term():
     Hq = 0.7 * (np.kron(up, dn) + np.kron(dn, up))  # bit 1 is the high bit of q
     full = np.kron(Hq, np.eye(nd)) + (0.25 - 0.5j) * np.kron(np.eye(4), a.T)
     assert np.allclose(H, full[np.ix_(basis.astype(int), basis.astype(int))], atol=1e-15)
+
+
+def test_modes_qubit_program_is_identity_on_the_mode():
+    """A program without mode gates over a qubits-times-mode index is the
+    qubit matrix Kronecker the identity (composite index q * mdim + m)."""
+    q = Q.getQbits(6)
+    md = Q.SingleModeIndex(5)
+    _, ham = Q.xxz_chain(6, 0.8, 1.3, pbc=True)
+    b = oracle.basis_scan(Q.Constrain(q * md, Q.getNset(q) == 3))
+    H = oracle.csc_to_dense(oracle.build_csc(b, ham, mdim=5))
+    Hq = oracle.csc_to_dense(oracle.build_csc(oracle.basis_scan(Q.Constrain(q, Q.getNset(q) == 3)), ham))
+    assert np.array_equal(H, np.kron(Hq, np.eye(5)))
