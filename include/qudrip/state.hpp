@@ -166,18 +166,25 @@ OPsiType<State<T>, OperatorSum> operator*(const OperatorSum& op, State<T>& psi) 
 }
 
 // OperatorSum::map_acc (operator_abel.hpp:36-42): to(t1) += c * H from(t2),
-// H evaluated on the fly from the term program (no stored matrix).
+// H evaluated on the fly from the term program (no stored matrix); over a
+// qubits-times-mode index through a temporary stored matrix (the same leaf
+// filter as `>>`).
 template <class T>
 void map_acc(const OperatorSum& op, State<T>& to, int t1, State<T>& from, int t2, value_type c = 1.0) {
   auto& sub = from.get_index();
-  qd_termprog* p = op.make_program(sub.nbits());
+  const int mdim = sub.basis()->mdim;
+  qd_termprog* p = op.make_program(sub.nbits(), mdim);
   qd_linop* lo = nullptr;
-  int rc = qd_linop_matfree(detail::ctx(), sub.basis()->b, p, &lo);
+  qd_csr* csr = nullptr;
+  int rc = mdim > 1 ? qd_csr_build(detail::ctx(), sub.basis()->b, p, &csr) : QD_OK;
+  if (rc == QD_OK)
+    rc = mdim > 1 ? qd_linop_from_csr(detail::ctx(), csr, &lo) : qd_linop_matfree(detail::ctx(), sub.basis()->b, p, &lo);
   if (rc == QD_OK) {
     double cc[2] = {c.real(), c.imag()};
     rc = qd_spmv_acc(detail::ctx(), lo, cc, from.device(t2), to.device(t1, true));
   }
   if (lo) qd_linop_destroy(lo);
+  if (csr) qd_csr_destroy(csr);
   qd_termprog_destroy(p);
   detail::check(rc, "map_acc");
 }

@@ -63,6 +63,8 @@ static int jaynes_cummings(int nd, double g, double D, const std::string& out) {
   qd_spmv(detail::ctx(), H.linop(), psi.device(0), psi.device(1, true));
   dump(out + ".x", psi.host(0, false), sub.range());
   dump(out + ".y", psi.host(1, false), sub.range());
+  psi(1) = ham * psi(0);  // the lazy product on the composite index
+  dump(out + ".ylazy", psi.host(1, false), sub.range());
   std::cout << "dumped " << terms.size() << " terms, n = " << sub.range() << std::endl;
   return 0;
 }

@@ -128,6 +128,8 @@ def test_dropin_boson_mode_jaynes_cummings(tmp_path):
     y = _load(prefix, ".y", np.complex128)
     yr = oracle.spmv(ref, x)
     assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14
+    yl = _load(prefix, ".ylazy", np.complex128)  # psi(1) = ham * psi(0)
+    assert np.linalg.norm(yl - yr) / np.linalg.norm(yr) < 1e-14
 
 
 @pytest.mark.gpu
