@@ -211,7 +211,7 @@ def main():
         from qudrip_amd import workloads as WL
         log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
         q = WL.hubbard_quench(ctx, 4, 4, 5, 5, U0=4.0, U1=2.0, k_gs=120, k_rt=30, dt=0.05,
-                              steps=args.quench_steps, matfree=True, log=log)
+                              steps=args.quench_steps, matfree=True, log=log, two_vector_gs=True)
         d = q["docc"]
         result["hubbard4x4_quench"] = {
             "n": q["n"], "E0_U4": q["E0"], "gs_s": round(q["gs_s"], 3), "k_gs": 120,
@@ -221,6 +221,12 @@ def main():
                        if int(round(t / 0.05)) < len(d)},
             "energy_drift": abs(q["energy_end"] - q["energy_start"]), "norm_end": q["norm_end"],
             "operator": "matrix-free (two-block Kronecker form)"}
+        tv = q["two_vector_gs"]
+        if tv:
+            result["hubbard4x4_quench"]["two_vector_gs"] = {
+                "E0": tv["E0"], "solve_s": round(tv["solve_s"], 3), "coefficients": tv["coefficients"],
+                "rel_diff_vs_dcgs2": abs(tv["E0"] - q["E0"]) / abs(q["E0"]),
+                "rayleigh_rel_diff": abs(tv["rayleigh"] - tv["E0"]) / abs(tv["E0"])}
         log(f"quench: {q['steps']} steps in {q['rt_s']:.2f} s ({q['rt_step_ms']:.1f} ms/step), "
             f"energy drift {abs(q['energy_end'] - q['energy_start']):.2e}")
         del q

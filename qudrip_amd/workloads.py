@@ -30,7 +30,7 @@ def make_operator(ctx, basis, opsum, nbits, matfree):
 
 
 def hubbard_quench(ctx, Lx, Ly, nup, ndn, U0, U1, t=1.0, k_gs=120, k_rt=30, dt=0.05, steps=200, seed=3,
-                   matfree=True, sync_each_step=False, log=None):
+                   matfree=True, sync_each_step=False, log=None, two_vector_gs=False):
     import torch
     nsite = Lx * Ly
     bonds = Q.hubbard_bonds_square(Lx, Ly)
@@ -59,6 +59,24 @@ def hubbard_quench(ctx, Lx, Ly, nup, ndn, U0, U1, t=1.0, k_gs=120, k_rt=30, dt=0
     energy = [op_expect(op1, psi).real / nrm]
     if log:
         log(f"hubbard {Lx}x{Ly} n={n}: E0(U={U0}) = {e0:.12f} in {gs_s:.3f} s; docc(0) = {docc[0]:.10f}")
+    two = None
+    if two_vector_gs:
+        # the same ground state through Hermitian::getSolver (the reference
+        # driver's own call, programs/hubbard_diss.cpp:382-384): two-vector
+        # Lanczos, no reorthogonalisation, two passes of the recurrence
+        from .runtime import HermitianLanczos
+        hl = HermitianLanczos(ctx, n, k_gs)
+        psi2 = ctx.seeded_state(n, seed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e2 = hl.ground_state(op0, psi2)
+        torch.cuda.synchronize()
+        two = {"E0": e2, "solve_s": time.perf_counter() - t0, "coefficients": hl.info()["dim"],
+               "rayleigh": op_expect(op0, psi2) / ctx.norm2(psi2)}
+        hl.close()
+        del psi2
+        if log:
+            log(f"hubbard {Lx}x{Ly}: two-vector Lanczos E0 = {e2:.12f} in {two['solve_s']:.3f} s")
     kr_rt = Krylov(ctx, n, k_rt, herm=False)
     nxt = ctx.zeros(n)
     torch.cuda.synchronize()
@@ -77,7 +95,7 @@ def hubbard_quench(ctx, Lx, Ly, nup, ndn, U0, U1, t=1.0, k_gs=120, k_rt=30, dt=0
         h.close()
     return {"n": n, "E0": e0, "docc": docc, "energy_start": energy[0], "energy_end": energy[-1],
             "norm_end": nrm, "setup_s": setup_s, "gs_s": gs_s, "rt_s": rt_s, "steps": steps,
-            "rt_step_ms": rt_s * 1e3 / max(steps, 1), "psi": psi}
+            "rt_step_ms": rt_s * 1e3 / max(steps, 1), "psi": psi, "two_vector_gs": two}
 
 
 def op_expect(op, psi):

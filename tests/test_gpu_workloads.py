@@ -15,7 +15,7 @@ def test_hubbard_quench_matches_dense(ctx, matfree):
     """3x2 periodic Hubbard, 2 up / 2 down: GS by Lanczos, quench U 4 -> 2, Arnoldi dt=0.05."""
     Lx, Ly, nup, ndn = 3, 2, 2, 2
     res = WL.hubbard_quench(ctx, Lx, Ly, nup, ndn, U0=4.0, U1=2.0, k_gs=60, k_rt=20, dt=0.05, steps=20,
-                            matfree=matfree)
+                            matfree=matfree, two_vector_gs=True)
     nsite = Lx * Ly
     bonds = Q.hubbard_bonds_square(Lx, Ly)
     idx, h0 = Q.hubbard(nsite, bonds, 1.0, 4.0)
@@ -27,6 +27,8 @@ def test_hubbard_quench_matches_dense(ctx, matfree):
     D = oracle.csc_to_dense(oracle.build_csc(basis, WL.docc_operator(nsite)))
     w, z = np.linalg.eigh(H0)
     assert abs(res["E0"] - w[0]) / abs(w[0]) < 1e-10
+    tv = res["two_vector_gs"]  # Hermitian::getSolver on the same operator
+    assert abs(tv["E0"] - w[0]) / abs(w[0]) < 1e-10 and abs(tv["rayleigh"] - w[0]) / abs(w[0]) < 1e-9
     g = z[:, 0]
     U = scipy.linalg.expm(-1j * 0.05 * H1)
     docc = [np.vdot(g, D @ g).real]
@@ -40,8 +42,11 @@ def test_hubbard_quench_matches_dense(ctx, matfree):
 
 def test_hubbard_4x4_free_fermions_closed_form(ctx):
     """U = 0, 5 up / 5 down on the 4x4 torus: E0 = -24 (closed shell), n = 19,079,424, matrix-free."""
-    res = WL.hubbard_quench(ctx, 4, 4, 5, 5, U0=0.0, U1=0.0, k_gs=120, k_rt=4, dt=0.01, steps=1, matfree=True)
+    res = WL.hubbard_quench(ctx, 4, 4, 5, 5, U0=0.0, U1=0.0, k_gs=120, k_rt=4, dt=0.01, steps=1, matfree=True,
+                            two_vector_gs=True)
     assert res["n"] == 19079424
     assert abs(res["E0"] - (-24.0)) / 24.0 < 1e-10
+    # the two-vector solver (Hermitian::getSolver) through the two-block product
+    assert abs(res["two_vector_gs"]["E0"] - (-24.0)) / 24.0 < 1e-10
     # free fermions: <n_up n_dn> factorises, docc = sum_i <n_i,up><n_i,dn> = 16 * (5/16)^2
     assert abs(res["docc"][0] - 16 * (5 / 16) ** 2) < 1e-8
