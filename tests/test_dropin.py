@@ -35,7 +35,7 @@ def test_headers_compile_with_nd_nh(tmp_path):
 
 
 def test_driver_programs_built():
-    for p in ("spin", "dropin_dump"):
+    for p in ("spin", "dropin_dump", "mc_dump"):
         assert os.access(os.path.join(BIN, p), os.X_OK), p
 
 
@@ -145,3 +145,44 @@ def test_dropin_lazy_product_matches_map_acc(tmp_path):
     oracle.map_acc(basis, ham, x, to)
     y = _load(prefix, ".y", np.complex128)
     assert np.linalg.norm(y - to) / np.linalg.norm(to) < 1e-14
+
+
+@pytest.mark.gpu
+def test_dropin_mc_manager_out_file(tmp_path):
+    """McManager::outFile (mc_manager.hpp:147-196): '#time' header with
+    'name(real)\tname(imag)' for complex and 'name' for real observables,
+    '#time\tindex' and a blank line after each time for vector observables,
+    std::to_string numbers; values = the oracle's psi.eval along the same
+    Arnoldi steps (averaged over two identical samples)."""
+    N, nt, h = 8, 5, 0.1
+    prefix = str(tmp_path / "mc")
+    _run([os.path.join(BIN, "mc_dump"), str(N), str(nt), repr(h), prefix])
+    idx, ham = Q.xy_chain(N)
+    basis = oracle.basis_scan(Q.Constrain(idx, Q.getNset(idx) == N // 2))
+    csc = oracle.build_csc(basis, ham)
+    psi = oracle.seeded_state(len(basis), 5)
+    states = [psi]
+    for _ in range(nt - 1):
+        states.append(oracle.arnoldi_evolve(csc, 12, h, states[-1]))
+    occ = np.array([[((int(b) >> i) & 1) for i in range(N)] for b in basis], dtype=float)
+    E = open(prefix + ".E.txt").read().split("\n")
+    assert E[0] == "#time\tE(real)\tE(imag)"
+    for t in range(nt):
+        f = E[1 + t].split("\t")
+        assert f[0] == f"{t * h:f}" and len(f) == 3
+        e = oracle.eval_expect(csc, states[t])
+        assert abs(float(f[1]) - e.real) < 2e-6 and abs(float(f[2]) - e.imag) < 2e-6
+        assert re.fullmatch(r"-?\d+\.\d{6}", f[1])
+    lines = open(prefix + ".n.txt").read().split("\n")
+    assert lines[0] == "#time\tindex\tn"
+    pos = 1
+    for t in range(nt):
+        dens = (np.abs(states[t]) ** 2) @ occ
+        for i in range(N):
+            f = lines[pos].split("\t")
+            assert f[0] == f"{t * h:f}" and f[1] == str(i) and len(f) == 3
+            assert abs(float(f[2]) - dens[i]) < 2e-6
+            pos += 1
+        assert lines[pos] == ""  # blank line after each time of a vector observable
+        pos += 1
+
