@@ -63,8 +63,8 @@ def main():
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "multi-GPU sharding is designed, not built", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); "
+              "sharding over several GPUs is designed, not implemented", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
