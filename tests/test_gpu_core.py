@@ -537,3 +537,41 @@ def test_stream_bandwidth_probe(ctx):
     assert ctx.lib.qd_stream_bandwidth(ctx.ptr, 1 << 30, 3, ctypes.byref(rd), ctypes.byref(cp)) == 0
     assert 500.0 < rd.value < 16000.0 and 500.0 < cp.value < 16000.0, (rd.value, cp.value)
     assert ctx.lib.qd_stream_bandwidth(ctx.ptr, 16, 3, ctypes.byref(rd), ctypes.byref(cp)) != 0
+
+
+@pytest.mark.parametrize("nd,nh", [(2, None), (None, 8), (1, 7)])
+def test_basis_nd_nh_full_size_hubbard4x4(ctx, nd, nh):
+    """4x4 Hubbard, 5 up / 5 down (19,079,424 states) filtered by the number
+    of doubly occupied (d) and empty (h) sites: the size matches the closed
+    form (choose the d doublons, then the remaining up and down sites, with
+    h = L - N_up - N_dn + d), the table is strictly ascending, sampled states
+    carry the right counts, and sampled ranks round-trip."""
+    from math import comb
+    L, nu, ndn = 16, 5, 5
+    idx = Q.getQbits(2 * L)
+    cons = [Q.getNup(idx, L) == nu, Q.getNdo(idx, L) == ndn]
+    if nd is not None:
+        cons.append(Q.getNd(idx, L) == nd)
+    if nh is not None:
+        cons.append(Q.getNh(idx, L) == nh)
+    d = nd if nd is not None else nh - (L - nu - ndn)
+    expect = comb(L, d) * comb(L - d, nu - d) * comb(L - nu, ndn - d)
+    if nd is not None and nh is not None and nh != L - nu - ndn + nd:
+        expect = 0
+    b = Basis(ctx, Q.Constrain(idx, *cons))
+    try:
+        assert b.size == expect
+        if expect == 0:
+            return
+        st = b.states()
+        assert np.all(st[1:] > st[:-1])
+        rng = np.random.default_rng(3)
+        pick = rng.choice(len(st), size=2000, replace=False)
+        mask = (1 << L) - 1
+        for x in st[pick[:200]]:
+            up, dn = int(x) & mask, (int(x) >> L) & mask
+            assert bin(up).count("1") == nu and bin(dn).count("1") == ndn
+            assert bin(up & dn).count("1") == d
+        assert np.array_equal(b.rank(st[pick]), pick)
+    finally:
+        b.close()
