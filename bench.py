@@ -15,8 +15,12 @@ and the oracle's single-thread CPU product timed on a bounded sample.
 Further sections (each can be skipped): the Hubbard 4x4 quench (ground state
 then real-time steps with the matrix-free Kronecker product), 4096 batched
 quantum-jump trajectories of the dissipative Hubbard chain L=8 against the
-one-at-a-time loop and the oracle, and the served request stream (partial and
-full reorthogonalisation) against the plain one-at-a-time path.
+one-at-a-time loop and the oracle, the served request stream (partial and
+full reorthogonalisation) against the plain one-at-a-time path, and extras:
+a basis filtered by double occupancy and a qubits-times-mode matrix (build
+and product).  The Lanczos section also runs the reference's two-vector
+solver (Hermitian::getSolver) on the same operator and start, and the quench
+section the same for its ground state.
 """
 import argparse
 import json
@@ -49,6 +53,8 @@ def main():
     ap.add_argument("--quench-steps", type=int, default=200)
     ap.add_argument("--no-traj", action="store_true", help="skip the dissipative trajectory workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Nd-filtered basis and qubits-times-mode measurements")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the Poisson stream")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--traj", type=int, default=4096, help="number of trajectories")
@@ -237,10 +243,73 @@ def main():
     if not args.no_serve:
         result["serving"] = serving(ctx, args)
 
+    if not args.no_extras:
+        result["extras"] = extras(ctx)
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(csr, n, nnz, B, x, args.cpu_seconds)
 
     print(json.dumps(result), flush=True)
+
+
+def extras(ctx):
+    """Rows beyond the Krylov core, measured: the 4x4 Hubbard (5,5) sector
+    filtered to two doubly occupied sites (getNd), and a qubits-times-mode
+    matrix (XY chain N=20 at half filling times 16 boson levels, 2,956,096
+    states): its `>>` build and its product (algorithmic bytes as for any
+    CSR, 20 nnz + 4 (n+1) + 32 n)."""
+    from math import comb
+
+    import torch
+
+    from qudrip_amd import operators as Q
+    from qudrip_amd.runtime import CSR, Basis, LinOp, TermProgram
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+    out = {}
+    L = 16
+    idx = Q.getQbits(2 * L)
+    sec = Q.Constrain(idx, Q.getNup(idx, L) == 5, Q.getNdo(idx, L) == 5, Q.getNd(idx, L) == 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = Basis(ctx, sec)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    out["basis_hub4x4_55_nd2"] = {"states": b.size, "closed_form": comb(16, 2) * comb(14, 3) * comb(11, 3),
+                                  "of_sector": 19079424, "build_ms": round(ms, 2)}
+    log(f"Nd-filtered basis: {b.size} of 19079424 states in {ms:.1f} ms")
+    b.close()
+    q, idxm, ham = Q.holstein_chain(20, 16, 0.9, 0.45)
+    sec = Q.Constrain(idxm, Q.getNset(q) == 10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, 20)
+    c = CSR(ctx, b, p)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    op = LinOp.from_csr(ctx, c)
+    n, nnz = c.n, c.nnz
+    x = ctx.seeded_state(n, 6)
+    y = ctx.zeros(n)
+    for _ in range(5):
+        op.apply(x, y)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    reps = 50
+    for _ in range(reps):
+        op.apply(x, y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    Bm = 20 * nnz + 4 * (n + 1) + 32 * n
+    out["holstein20_mode16"] = {"n": n, "nnz": nnz, "build_ms": round(build_ms, 2), "spmv_us": round(us, 2),
+                                "spmv_GBs": round(Bm / (us * 1e-6) / 1e9, 1)}
+    log(f"qubits x mode: n={n} nnz={nnz} build {build_ms:.1f} ms, product {us:.1f} us "
+        f"= {Bm / (us * 1e-6) / 1e9:.0f} GB/s")
+    for h in (op, c, p, b):
+        h.close()
+    return out
 
 
 def trajectories(ctx, args):

@@ -516,6 +516,31 @@ def hubbard(nsite, bonds, t=1.0, U=4.0):
     return idx, ham
 
 
+def holstein_chain(N, nd, w, g):
+    """XY chain of N qubits next to one boson mode of nd levels, coupled to
+    the total occupation: sum_i (S+_{i+1} S-_i + h.c.) + w a+a
+    + g sum_i n_i (a + a+), with mode gates (operators/mode.hpp) in the
+    reference's matrix convention.  Returns (qubit index, composite index, H)."""
+    q = getQbits(N)
+    md = SingleModeIndex(nd)
+    idx = q * md
+    gc, ga, gn = getBoseGate(q), getBoseGate(q), getBoseGate(q)
+    gc << [[0, 1], [0, 0]]
+    ga << [[0, 0], [1, 0]]
+    gn << [[0, 0], [0, 1]]
+    ad, a, num = getModeOp(md), getModeOp(md), getModeOp(md)
+    ad << LadderMatrix(nd, MODE_LOWER)
+    a << LadderMatrix(nd, MODE_UPPER)
+    num << np.diag(np.arange(nd, dtype=float))
+    ham = getEmptyOperator()
+    for i in range(N - 1):
+        ham = ham + gc(i + 1) * ga(i) + gc(i) * ga(i + 1)
+    ham = ham + w * num(0)
+    for i in range(N):
+        ham = ham + g * (gn(i) * a(0)) + g * (gn(i) * ad(0))
+    return q, idx, ham
+
+
 def hubbard_sector(idx, nsite, nup, ndn):
     return Constrain(idx, getNup(idx, nsite) == nup, getNdo(idx, nsite) == ndn)
 

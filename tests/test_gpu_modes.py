@@ -19,26 +19,7 @@ def bits(a):
 
 
 def holstein(N, nd, w, g):
-    """XY chain of N qubits, one mode of nd levels coupled to every site's
-    occupation: sum (S+S- + h.c.) + w a+a + g sum_i n_i (a + a+)."""
-    q = Q.getQbits(N)
-    md = Q.SingleModeIndex(nd)
-    idx = q * md
-    gc, ga, gn = Q.getBoseGate(q), Q.getBoseGate(q), Q.getBoseGate(q)
-    gc << [[0, 1], [0, 0]]
-    ga << [[0, 0], [1, 0]]
-    gn << [[0, 0], [0, 1]]
-    ad, a, num = Q.getModeOp(md), Q.getModeOp(md), Q.getModeOp(md)
-    ad << Q.LadderMatrix(nd, Q.MODE_LOWER)
-    a << Q.LadderMatrix(nd, Q.MODE_UPPER)
-    num << np.diag(np.arange(nd, dtype=float))
-    ham = Q.getEmptyOperator()
-    for i in range(N - 1):
-        ham = ham + gc(i + 1) * ga(i) + gc(i) * ga(i + 1)
-    ham = ham + w * num(0)
-    for i in range(N):
-        ham = ham + g * (gn(i) * a(0)) + g * (gn(i) * ad(0))
-    return q, idx, ham
+    return Q.holstein_chain(N, nd, w, g)
 
 
 def _sector(case):
