@@ -34,6 +34,47 @@ def test_headers_compile_with_nd_nh(tmp_path):
                    check=True)
 
 
+def test_headers_compile_solvers_over_a_mode_index(tmp_path):
+    """Both solver families, McManager and the lazy product instantiate over a
+    qubits-times-mode SubIndex (index.hpp:94-184, operators/mode.hpp)."""
+    src = tmp_path / "t.cpp"
+    src.write_text('''#include "qudrip/qudrip.hpp"
+using namespace qudrip;
+int main() {
+  auto q = getQbits(4);
+  SingleModeIndex<> md(5);
+  auto idx = q * md;
+  auto nset = getNset(q);
+  auto sub = Constrain(idx, nset = 2);
+  auto gc = getBoseGate(q), ga = getBoseGate(q);
+  gc << 0.5 * (pauli_x + II * pauli_y);
+  ga << 0.5 * (pauli_x - II * pauli_y);
+  auto a = getModeOp(md), ad = getModeOp(md);
+  a << LadderMatrix(5, Mode::Upper);
+  ad << LadderMatrix(5, Mode::Lower);
+  auto ham = getEmptyOperator();
+  for (int i = 0; i < 3; ++i) ham = ham + gc(i + 1) * ga(i) + gc(i) * ga(i + 1);
+  ham = ham + 0.3 * (a(0) * ad(0));
+  auto H = ham >> sub;
+  auto psi = getState(sub, 3);
+  shuffle_state(psi(0));
+  auto hs = Hermitian::getSolver(sub, 20);
+  hs.evolve(H, psi, 0, 0.1);
+  hs.getGroundState(H, psi, 0);
+  auto ar = Arnoldi::getLanczosSolver(sub, 20);
+  ar.getGroundState(H, psi, 1);
+  auto mc = getMC(psi);
+  mc.addObs("E", 1);
+  mc.updateMatrix("E", 0, H);
+  mc.incrAllObs(0);
+  psi(2) = ham * psi(0);
+  return (int)psi.eval(0, H).real();
+}
+''')
+    subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"), str(src)],
+                   check=True)
+
+
 def test_driver_programs_built():
     for p in ("spin", "dropin_dump", "mc_dump"):
         assert os.access(os.path.join(BIN, p), os.X_OK), p
