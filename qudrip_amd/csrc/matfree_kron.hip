@@ -681,14 +681,26 @@ struct SlabShape {
   unsigned grid;
 };
 
-SlabShape slab_shape(const KronDesc& d, int W) {
+// One column slab is split into `chunks` blocks of rows.  With target 128 an
+// XCD that holds more than 128 blocks at once works on two or three slabs
+// together; `target` = the blocks one XCD holds keeps one slab (W columns of
+// x over all nB rows) live per XCD, so it stays in that XCD's L2.
+SlabShape slab_shape(const KronDesc& d, int W, int target = 128) {
   SlabShape s;
   const int64_t nslabs = (d.nA + W - 1) / W;
-  s.chunks = (int)std::min<int64_t>(128, std::max<int64_t>(1, d.nB / 8));
-  s.rows_per_chunk = (int)((d.nB + s.chunks - 1) / s.chunks);
+  const int64_t c = std::min<int64_t>(target, std::max<int64_t>(1, d.nB / 8));
+  s.rows_per_chunk = (int)((d.nB + c - 1) / c);
+  s.chunks = (int)((d.nB + s.rows_per_chunk - 1) / s.rows_per_chunk);  // no empty chunks
   const int64_t slab_groups = (nslabs + 7) / 8;  // slabs per XCD
   s.grid = (unsigned)(slab_groups * s.chunks * 8);
   return s;
+}
+
+// blocks of a 256-thread kernel with `lds` dynamic LDS that one XCD holds at once
+int xcd_resident(const void* fn, size_t lds, int ncu) {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
+  return occ * std::max(1, ncu / 8);
 }
 
 int kron_optin(int device) {
@@ -713,6 +725,7 @@ int kron_optin(int device) {
 // 2 = rows2<1,8>, 3 = rows2<4,4>; a variant whose LDS does not fit returns
 // false.  Without the packed table: kron_rows_kernel.
 bool launch_rows(const KronDesc& d, int v, int optin, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  if (v < 0 || v > 3) return false;
   const size_t lds1 = sizeof(cplx) * (size_t)d.nA;
   if (!d.ell) {
     if (v != 0) return false;
@@ -743,19 +756,35 @@ bool launch_rows(const KronDesc& d, int v, int optin, const cplx* x, cplx* y, cp
 }
 
 // slab pass (adds into y): variant 0 = kron_slab_packed<32,4>, 1 =
-// slab2<32,8>, 2 = slab2<32,16>.  Without the packed table: kron_slab_kernel.
-bool launch_slab(const KronDesc& d, int v, const cplx* x, cplx* y, hipStream_t st) {
-  const SlabShape sh = slab_shape(d, KRON_SLAB_W);
+// slab2<32,8>, 2 = slab2<32,16>, each at 128 chunks per slab; 3 = slab2<32,8>
+// and 4 = slab2<32,16> with one slab per XCD at a time.  Without the packed
+// table: kron_slab_kernel.
+constexpr int KRON_SLAB_VARIANTS = 5;
+
+bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipStream_t st) {
   if (!d.ellB) {
     if (v != 0) return false;
+    const SlabShape sh = slab_shape(d, KRON_SLAB_W);
     kron_slab_kernel<KRON_SLAB_W><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk);
     return true;
   }
-  const size_t lds = sizeof(uint32_t) * (size_t)sh.rows_per_chunk * (size_t)d.ellB_w;
+  if (v < 0 || v >= KRON_SLAB_VARIANTS) return false;
+  const void* fn = v == 0 ? (const void*)kron_slab_packed_kernel<KRON_SLAB_W, 4>
+                   : (v == 1 || v == 3) ? (const void*)kron_slab2_kernel<KRON_SLAB_W, 8>
+                                        : (const void*)kron_slab2_kernel<KRON_SLAB_W, 16>;
+  SlabShape sh = slab_shape(d, KRON_SLAB_W);
+  size_t lds = sizeof(uint32_t) * (size_t)sh.rows_per_chunk * (size_t)d.ellB_w;
+  if (v >= 3) {  // the LDS per block shrinks with the chunk, which can raise residency: take the fixed point
+    for (int it = 0; it < 3; ++it) {
+      sh = slab_shape(d, KRON_SLAB_W, xcd_resident(fn, lds, ncu));
+      lds = sizeof(uint32_t) * (size_t)sh.rows_per_chunk * (size_t)d.ellB_w;
+    }
+  }
   if (v != 0 && lds > 32768) return false;
   switch (v) {
     case 0: kron_slab_packed_kernel<KRON_SLAB_W, 4><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
-    case 1: kron_slab2_kernel<KRON_SLAB_W, 8><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+    case 1:
+    case 3: kron_slab2_kernel<KRON_SLAB_W, 8><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
     default: kron_slab2_kernel<KRON_SLAB_W, 16><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
   }
   return true;
@@ -795,6 +824,17 @@ float time_launches(hipStream_t st, F&& f, int reps) {
 
 int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   const KronDesc& d = k->desc;
+  // QD_KRON_VARIANTS="rows,slab" fixes both variants instead of timing them
+  // (tests run every variant through it); an unfit variant then makes the
+  // product return QD_E_UNSUPPORTED.
+  if (const char* f = std::getenv("QD_KRON_VARIANTS")) {
+    int r = -1, s = -1;
+    if (std::sscanf(f, "%d,%d", &r, &s) == 2 && r >= 0 && s >= 0) {
+      k->rows_variant = r;
+      k->slab_variant = s;
+      return QD_OK;
+    }
+  }
   const KronKey key{d.nA, d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, d.nvtab};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -818,9 +858,9 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
       best_r = v;
     }
   }
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < KRON_SLAB_VARIANTS; ++v) {
     bool ok = true;
-    float ms = time_launches(st, [&] { ok = ok && launch_slab(d, v, x, y, st); }, 3);
+    float ms = time_launches(st, [&] { ok = ok && launch_slab(d, v, ctx->num_cus, x, y, st); }, 3);
     if (ok && std::getenv("QD_TUNE_LOG")) std::fprintf(stderr, "[qd]   slab variant %d: %.1f us\n", v, 1000.0 * ms / 3);
     if (ok && ms < ts) {
       ts = ms;
@@ -852,7 +892,7 @@ int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, c
   if (!launch_rows(d, k->rows_variant, optin, x, y, shift, ctx->stream))
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", k->rows_variant);
   QD_CHECK_LAUNCH(ctx);
-  if (!launch_slab(d, k->slab_variant, x, y, ctx->stream))
+  if (!launch_slab(d, k->slab_variant, ctx->num_cus, x, y, ctx->stream))
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker slab variant %d", k->slab_variant);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
+from qudrip_amd import abi
 from qudrip_amd import operators as Q
 from qudrip_amd.runtime import CSR, Basis, Krylov, LinOp, TermProgram
 from tests import physics as P
@@ -389,9 +390,7 @@ def test_evolve_tol_matches_dense_propagator(ctx):
         _close(kr, op, c, p, b)
 
 
-@pytest.mark.parametrize("case", ["chain8_44", "sq3x4_33", "sq3x4_66", "chain6_nonherm", "phase_hops"])
-def test_matfree_structured_matches_oracle(ctx, case):
-    """Two-block Kronecker form of the matrix-free product (Hubbard layout)."""
+def _structured_case(case):
     if case == "chain8_44":
         idx, ham = Q.hubbard(8, Q.hubbard_bonds_chain(8, False), 1.0, 4.0)
         sec = Q.hubbard_sector(idx, 8, 4, 4)
@@ -424,6 +423,13 @@ def test_matfree_structured_matches_oracle(ctx, case):
         for i in range(6):
             ham = ham + 3.0 * (g(i, i) * g(i + 6, i + 6)) + (-0.4) * nb(i + 6)
         sec = Q.hubbard_sector(idx, 6, 3, 3)
+    return ham, sec
+
+
+@pytest.mark.parametrize("case", ["chain8_44", "sq3x4_33", "sq3x4_66", "chain6_nonherm", "phase_hops"])
+def test_matfree_structured_matches_oracle(ctx, case):
+    """Two-block Kronecker form of the matrix-free product (Hubbard layout)."""
+    ham, sec = _structured_case(case)
     basis = oracle.basis_scan(sec)
     ref = oracle.build_csc(basis, ham)
     b = Basis(ctx, sec)
@@ -453,6 +459,43 @@ def test_matfree_structured_matches_oracle(ctx, case):
             assert np.linalg.norm(yv - yd) / np.linalg.norm(yd) < 1e-14, structured
     finally:
         _close(mf, p, b)
+
+
+@pytest.mark.parametrize("nup,ndn", [(7, 1), (1, 7)])
+def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
+    """Every rows-pass and slab-pass variant of the two-block product, fixed
+    through QD_KRON_VARIANTS, against the oracle: a 14-site chain with one
+    block of 3432 states (more than 128 chunks of 8 rows, so the slab
+    variants that size chunks to one XCD's resident blocks differ from the
+    128-chunk ones) and one of 14."""
+    ham, sec = hub(14, Q.hubbard_bonds_chain(14, True), nup, ndn, U=3.0)[1:]
+    basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(basis, ham)
+    x = oracle.seeded_state(len(basis), 23)
+    yr = oracle.spmv(ref, x)
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, sec.nbits)
+    xd = to_dev(ctx, x)
+    ran = []
+    try:
+        for rv in range(4):
+            for sv in range(5):
+                monkeypatch.setenv("QD_KRON_VARIANTS", f"{rv},{sv}")
+                mf = LinOp.matfree(ctx, b, p)
+                try:
+                    assert mf.set_structured(True)
+                    try:
+                        y = mf.apply(xd).cpu().numpy()
+                    except abi.QDError as e:  # a variant whose LDS does not fit this shape
+                        assert abi.STATUS[e.code] == "QD_E_UNSUPPORTED"
+                        continue
+                finally:
+                    mf.close()
+                assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14, (rv, sv)
+                ran.append((rv, sv))
+    finally:
+        _close(p, b)
+    assert {s for r, s in ran} == set(range(5)) and (0, 0) in ran
 
 
 @pytest.mark.parametrize("method", [0, 1])
