@@ -10,7 +10,7 @@
 #include "qd_internal.hpp"
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
-#define QD_STREAM_VARIANTS 13
+#define QD_STREAM_VARIANTS 15
 
 // ---------------------------------------------------------------- K5
 // G lanes cooperate on one row: each lane strides the row's entries by G,
@@ -254,6 +254,112 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Same partition and summation order as spmv_stream_kernel, software
+// pipelined: a slice is exactly one block of 64 x U entries (CAPW = 64 U),
+// and the value/index streams of slice s+1 are issued before the wait on the
+// x gathers of slice s, so each wave keeps two memory phases in flight
+// instead of alternating index load -> gather -> products.  Indices are
+// clamped into the slice, so every load stays inside [wbeg, wend).
+template <int U>
+__global__ void __launch_bounds__(256)
+    spmv_stream3_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                        const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
+                        const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
+                        cplx shift) {
+  constexpr int CAPW = 64 * U;
+  __shared__ cplx lds[4][CAPW];
+  const unsigned nb = gridDim.x;
+  const unsigned per = nb >> 3;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)lb * 4 + w) * 64;
+  if (r0 >= nrows) return;  // whole wave leaves together
+  const int64_t row = r0 + lane;
+  const int32_t nnz_end = outer[nrows];
+  int32_t rs = (row < nrows) ? outer[row] : nnz_end;
+  int32_t re = __shfl_down(rs, 1, 64);
+  if (lane == 63) re = (r0 + 64 < nrows) ? outer[r0 + 64] : nnz_end;
+  const int32_t wbeg = __shfl(rs, 0, 64);
+  const int32_t wend = __shfl(re, 63, 64);
+  cplx* __restrict__ buf = lds[w];
+  const dbl2* vp = reinterpret_cast<const dbl2*>(vals);
+  double sx = 0.0, sy = 0.0;
+  dbl2 v[U];
+  int32_t c[U];
+  if (wbeg < wend) {
+    const int32_t cn = min(wend - wbeg, (int32_t)CAPW);
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(inner + wbeg + min(64 * u + lane, cn - 1));
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(vp + wbeg + min(64 * u + lane, cn - 1));
+  }
+  for (int32_t c0 = wbeg; c0 < wend; c0 += CAPW) {
+    const int32_t cn = min(wend - c0, (int32_t)CAPW);
+    cplx xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+    // the next slice's streams go out behind the gathers (wave-uniform branch)
+    const int32_t c1 = c0 + CAPW;
+    dbl2 vn[U];
+    int32_t cnx[U];
+    if (c1 < wend) {
+      const int32_t cn1 = min(wend - c1, (int32_t)CAPW);
+#pragma unroll
+      for (int u = 0; u < U; ++u) cnx[u] = __builtin_nontemporal_load(inner + c1 + min(64 * u + lane, cn1 - 1));
+#pragma unroll
+      for (int u = 0; u < U; ++u) vn[u] = __builtin_nontemporal_load(vp + c1 + min(64 * u + lane, cn1 - 1));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int32_t kk = 64 * u + lane;
+      if (kk < cn) buf[kk] = cmake(v[u].x * xv[u].x + v[u].y * xv[u].y, v[u].x * xv[u].y - v[u].y * xv[u].x);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int32_t a = max(rs, c0) - c0, b = min(re, c0 + cn) - c0;
+    int32_t e = a;
+    for (; e + 4 <= b; e += 4) {
+      const cplx p0 = buf[e], p1 = buf[e + 1], p2 = buf[e + 2], p3 = buf[e + 3];
+      sx += p0.x;
+      sy += p0.y;
+      sx += p1.x;
+      sy += p1.y;
+      sx += p2.x;
+      sy += p2.y;
+      sx += p3.x;
+      sy += p3.y;
+    }
+    for (; e < b; ++e) {
+      const cplx p = buf[e];
+      sx += p.x;
+      sy += p.y;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c1 < wend) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = cnx[u];
+        v[u] = vn[u];
+      }
+    }
+  }
+  if (row < nrows) {
+    double s = xscale ? xscale[xscale_idx] : 1.0;
+    cplx r = cmake(sx * s, sy * s);
+    if (diag || shift.x != 0.0 || shift.y != 0.0) {
+      cplx xr = x[row];
+      cplx d = diag ? cadd(diag[row], shift) : shift;
+      r = cadd(r, crmul(s, cmul(d, xr)));
+    }
+    __builtin_nontemporal_store(r.x, &y[row].x);
+    __builtin_nontemporal_store(r.y, &y[row].y);
+  }
+}
+
 // ---------------------------------------------------------------- K6
 // Matrix-free pull product from the adjoint program: expanding column r of
 // A^dagger = sum conj(c_k) P_k^dagger from state r gives leaves (t, w) with
@@ -345,6 +451,14 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
       QD_STREAM2_CASE(11, 512, 4)
       QD_STREAM2_CASE(12, 1024, 8)
 #undef QD_STREAM2_CASE
+#define QD_STREAM3_CASE(V, U)                                                                          \
+  case V:                                                                                           \
+    spmv_stream3_kernel<U><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y,      \
+                                                          d_xscale, xscale_idx, d_diag, shift);         \
+    break;
+      QD_STREAM3_CASE(13, 8)
+      QD_STREAM3_CASE(14, 4)
+#undef QD_STREAM3_CASE
       default:
         return qd_set_error(ctx, QD_E_ARG, "stream variant %d", -op->lanes_per_row);
     }
