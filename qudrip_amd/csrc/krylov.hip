@@ -31,7 +31,16 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
 
 namespace {
 
-constexpr int KT = 256;  // threads per block for the tall-skinny kernels
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+constexpr int KT = 256;
+
+// Basis columns are streamed once per pass and are far larger than the
+// caches; loading (and writing) them non-temporally leaves L2/MALL to the
+// vectors that are read again (u, w, the product's x).
+__device__ __forceinline__ cplx ld_nt(const cplx* p) {
+  const dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(p));
+  return cmake(v.x, v.y);
+}  // threads per block for the tall-skinny kernels
 
 
 // gfx950 cross-half swaps: permlane32 trades lanes 32-63 of a with lanes
@@ -261,7 +270,7 @@ __global__ void __launch_bounds__(KT)
     for (int q = 0; q < RPT; ++q) {
       int64_t r = base + (int64_t)q * KT + threadIdx.x;
       if (r < n) {
-        cplx v = Vc[r];
+        const cplx v = ld_nt(Vc + r);
         acc[q].x += v.x * yc.x - v.y * yc.y;
         acc[q].y += v.x * yc.y + v.y * yc.x;
       }
@@ -378,7 +387,7 @@ __global__ void __launch_bounds__(KT)
         for (int q = 0; q < RPT; ++q) {
           const int64_t r = t0 + (int64_t)q * KT + threadIdx.x;
           if (r < r1) {
-            const cplx v0 = Vc[r], v1 = Vd[r];
+            const cplx v0 = ld_nt(Vc + r), v1 = ld_nt(Vd + r);
             a0 = cadd(a0, cdotmul(v0, ur[q]));
             a1 = cadd(a1, cdotmul(v1, ur[q]));
             b0 = cadd(b0, cdotmul(v0, wr[q]));
@@ -459,8 +468,8 @@ __global__ void __launch_bounds__(KT)
       for (int q = 0; q < RPT; ++q) {
         int64_t r = base + (int64_t)q * KT + threadIdx.x;
         if (r < n) {
-          a0 = cadd(a0, cdotmul(Vc[r], ur[q]));
-          a1 = cadd(a1, cdotmul(Vd[r], ur[q]));
+          a0 = cadd(a0, cdotmul(ld_nt(Vc + r), ur[q]));
+          a1 = cadd(a1, cdotmul(ld_nt(Vd + r), ur[q]));
         }
       }
       a0 = wave_sum(a0);
@@ -476,7 +485,7 @@ __global__ void __launch_bounds__(KT)
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         int64_t r = base + (int64_t)q * KT + threadIdx.x;
-        if (r < n) a = cadd(a, cdotmul(Vc[r], ur[q]));
+        if (r < n) a = cadd(a, cdotmul(ld_nt(Vc + r), ur[q]));
       }
       a = wave_sum(a);
       if (lane == 0) sh[wv][c - c0] = a;
@@ -672,7 +681,7 @@ __global__ void __launch_bounds__(KT)
     for (int q = 0; q < RPT; ++q) {
       int64_t r = base + (int64_t)q * KT + threadIdx.x;
       if (r < n) {
-        cplx v = Vc[r];
+        const cplx v = ld_nt(Vc + r);
         s1[q].x += v.x * a.x - v.y * a.y;
         s1[q].y += v.x * a.y + v.y * a.x;
         if (!FINAL) {
@@ -690,7 +699,8 @@ __global__ void __launch_bounds__(KT)
     int64_t r = base + (int64_t)q * KT + threadIdx.x;
     if (r < n) {
       cplx ur = u[r];
-      Vj[r] = cmake((ur.x - s1[q].x) * inv, (ur.y - s1[q].y) * inv);
+      __builtin_nontemporal_store((ur.x - s1[q].x) * inv, &Vj[r].x);
+      __builtin_nontemporal_store((ur.y - s1[q].y) * inv, &Vj[r].y);
       if (!FINAL) {
         cplx wr = w[r];
         cplx t = cmul(ur, us);
