@@ -696,10 +696,22 @@ SlabShape slab_shape(const KronDesc& d, int W, int target = 128) {
   return s;
 }
 
-// blocks of a 256-thread kernel with `lds` dynamic LDS that one XCD holds at once
+// blocks of a 256-thread kernel with `lds` dynamic LDS that one XCD holds at
+// once; the occupancy query runs once per (kernel, LDS size) and process
 int xcd_resident(const void* fn, size_t lds, int ncu) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, size_t>, int> memo;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = memo.find({fn, lds});
+    if (it != memo.end()) occ = it->second;
+  }
+  if (occ == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
+    std::lock_guard<std::mutex> lk(mu);
+    memo[{fn, lds}] = occ;
+  }
   return occ * std::max(1, ncu / 8);
 }
 
