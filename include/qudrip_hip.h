@@ -222,10 +222,12 @@ int64_t qd_linop_dim(const qd_linop* op);
  * matrices; on = 0 forces the generic per-row kernel.  *has_structured
  * (may be NULL) reports whether the operator has that form. */
 int qd_linop_set_structured(qd_linop* op, int on, int* has_structured);
-/* Row mapping of the stored-matrix product: 0, -1, ..., -8 select a
+/* Row mapping of the stored-matrix product: 0, -1, ..., -14 select a
  * CSR-stream variant (a wave streams 64 rows' nonzeros through LDS; the
- * variants differ in LDS chunk, loads in flight and cache hint), 2..64 the
- * lanes per row of the vector kernel.  Chosen from nnz/n at creation. */
+ * variants differ in LDS chunk, loads in flight, cache hint and software
+ * pipelining; all give the same bits), 2..64 the lanes per row of the vector
+ * kernel.  Chosen from nnz/n at creation: -13 (pipelined, 512 entries per
+ * slice) for rows of up to 48 entries on average. */
 int qd_linop_set_lanes(qd_linop* op, int lanes);
 int qd_linop_get_lanes(const qd_linop* op);
 /* Tile autotuner: times every row mapping on d_x -> d_y (reps launches
