@@ -33,6 +33,21 @@
 
 #include "kron.hpp"
 
+namespace {
+typedef double kdbl2 __attribute__((ext_vector_type(2)));
+// y is written by the rows pass and read-modified-written by the slab pass,
+// once each per product, and is larger than the caches for the sizes where
+// the product matters: stream it non-temporally so L2/MALL keep x.
+__device__ __forceinline__ cplx y_ld(const cplx* p) {
+  const kdbl2 v = __builtin_nontemporal_load(reinterpret_cast<const kdbl2*>(p));
+  return cmake(v.x, v.y);
+}
+__device__ __forceinline__ void y_st(cplx* p, double re, double im) {
+  __builtin_nontemporal_store(re, &p->x);
+  __builtin_nontemporal_store(im, &p->y);
+}
+}  // namespace
+
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 
@@ -104,7 +119,7 @@ __global__ void __launch_bounds__(1024)
     cplx d = cadd(diag_value(k, sB | k.statesA[a]), shift);
     if (k.ext_diag) d = cadd(d, k.ext_diag[b * k.nA + a]);
     cplx dx = cmul(d, xa);
-    y[b * k.nA + a] = cmake(sx + dx.x, sy + dx.y);
+    y_st(y + b * k.nA + a, sx + dx.x, sy + dx.y);
   }
 }
 
@@ -149,7 +164,7 @@ __global__ void __launch_bounds__(1024)
         cplx d = cadd(diag_value(k, sB[r] | sA), shift);
         if (k.ext_diag) d = cadd(d, k.ext_diag[(b0 + r) * k.nA + a]);
         cplx dx = cmul(d, xr[r * k.nA + a]);
-        y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
+        y_st(y + (b0 + r) * k.nA + a, sx[r] + dx.x, sy[r] + dx.y);
       }
     }
   }
@@ -187,8 +202,8 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (col_ok) {
-      cplx o = y[b * k.nA + a];
-      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+      const cplx o = y_ld(y + b * k.nA + a);
+      y_st(y + b * k.nA + a, o.x + sx, o.y + sy);
     }
   }
 }
@@ -235,8 +250,8 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (col_ok) {
-      cplx o = y[b * k.nA + a];
-      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+      const cplx o = y_ld(y + b * k.nA + a);
+      y_st(y + b * k.nA + a, o.x + sx, o.y + sy);
     }
   }
 }
@@ -301,7 +316,7 @@ __global__ void __launch_bounds__(1024)
         cplx d = cadd(diag_value(k, sB[r] | sA), shift);
         if (k.ext_diag) d = cadd(d, k.ext_diag[(b0 + r) * k.nA + a]);
         cplx dx = cmul(d, xr[r * k.nA + a]);
-        y[(b0 + r) * k.nA + a] = cmake(sx[r] + dx.x, sy[r] + dx.y);
+        y_st(y + (b0 + r) * k.nA + a, sx[r] + dx.x, sy[r] + dx.y);
       }
     }
   }
@@ -358,8 +373,8 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (col_ok) {
-      cplx o = y[b * k.nA + a];
-      y[b * k.nA + a] = cmake(o.x + sx, o.y + sy);
+      const cplx o = y_ld(y + b * k.nA + a);
+      y_st(y + b * k.nA + a, o.x + sx, o.y + sy);
     }
   }
 }
