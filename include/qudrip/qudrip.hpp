@@ -10,6 +10,7 @@
 #include <complex>
 
 #include "matrix.hpp"
+#include "utils.hpp"
 #include "runtime.hpp"
 #include "index.hpp"
 #include "operator.hpp"

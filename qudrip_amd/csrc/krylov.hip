@@ -918,13 +918,15 @@ struct RowsGrid {
 };
 template <int RT>
 static RowsGrid rows_grid(int64_t n, int ncu) {
-  static int per_cu = 0;
   const size_t lds = (size_t)(KT / 64) * RowsGrid::kCmax * 4 * sizeof(double);
-  if (per_cu <= 0) {
+  // queried once per process; the initialisation of a function-local static
+  // is thread-safe, so concurrent solves (server workers) never see a
+  // half-written value
+  static const int per_cu = [lds]() {
     int b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dc_red_rows_kernel<RT>, KT, lds) != hipSuccess || b < 1) b = 1;
-    per_cu = b;
-  }
+    return b;
+  }();
   const int64_t tile = (int64_t)KT * RT;
   const int64_t tiles = std::max<int64_t>(1, (n + tile - 1) / tile);
   const int64_t slots = std::max<int64_t>(1, (int64_t)per_cu * ncu);

@@ -25,6 +25,7 @@
 //                     on one XCD so the slab (n_B * W * 16 B) stays in its L2.
 // HBM traffic ~ X twice + Y three times (vs. 32 n algorithmic).
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -68,8 +69,11 @@ struct qd_kron {
   DevOp* d_dops = nullptr;
   int ndterms = 0;
   KronDesc desc{};
-  int rows_variant = -1;  // chosen by timing at the first product (-1: not yet)
-  int slab_variant = -1;
+  // launch variants, chosen by timing at the first product: rows * 16 + slab
+  // in one word, so a product on another thread reads both or neither
+  // (-1: not chosen yet); tune_mu lets one thread time them
+  std::atomic<int> variants{-1};
+  std::mutex tune_mu;
 };
 
 __device__ inline cplx diag_value(const KronDesc& k, uint64_t s) {
@@ -849,16 +853,15 @@ float time_launches(hipStream_t st, F&& f, int reps) {
   return ms;
 }
 
-int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
+int kron_tune_locked(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   const KronDesc& d = k->desc;
   // QD_KRON_VARIANTS="rows,slab" fixes both variants instead of timing them
   // (tests run every variant through it); an unfit variant then makes the
   // product return QD_E_UNSUPPORTED.
   if (const char* f = std::getenv("QD_KRON_VARIANTS")) {
     int r = -1, s = -1;
-    if (std::sscanf(f, "%d,%d", &r, &s) == 2 && r >= 0 && s >= 0) {
-      k->rows_variant = r;
-      k->slab_variant = s;
+    if (std::sscanf(f, "%d,%d", &r, &s) == 2 && r >= 0 && s >= 0 && r < 16 && s < 16) {
+      k->variants.store(r * 16 + s);
       return QD_OK;
     }
   }
@@ -867,8 +870,7 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      k->rows_variant = it->second.first;
-      k->slab_variant = it->second.second;
+      k->variants.store(it->second.first * 16 + it->second.second);
       return QD_OK;
     }
   }
@@ -899,33 +901,43 @@ int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
     std::fprintf(stderr, "[qd] Kronecker nA=%lld nB=%lld w=%d wB=%d: rows variant %d (%.1f us), slab variant %d (%.1f us)\n",
                  (long long)d.nA, (long long)d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, best_r,
                  1000.0 * tr / 3, best_s, 1000.0 * ts / 3);
-  k->rows_variant = best_r;
-  k->slab_variant = best_s;
+  k->variants.store(best_r * 16 + best_s);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = {best_r, best_s};
   return QD_OK;
+}
+
+// one thread times the variants of a form; others that arrive meanwhile wait
+// for its choice (x, y: n-vector scratch of the calling thread)
+int kron_tune(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
+  std::lock_guard<std::mutex> lk(k->tune_mu);
+  if (k->variants.load() >= 0) return QD_OK;
+  return kron_tune_locked(ctx, k, x, y);
 }
 }  // namespace
 
 int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, const cplx* ext_diag) {
   if (k->desc.nA == 0 || k->desc.nB == 0) return QD_OK;
-  if (k->rows_variant < 0) {
+  int var = k->variants.load();
+  if (var < 0) {
     int rc = kron_tune(ctx, k, x, y);  // leaves y overwritten: the product below rewrites it
     if (rc) return rc;
+    var = k->variants.load();
   }
+  const int rows_variant = var / 16, slab_variant = var % 16;
   KronDesc d = k->desc;
   d.ext_diag = ext_diag;
   const int optin = kron_optin(ctx->device);
-  if (!launch_rows(d, k->rows_variant, optin, x, y, shift, ctx->stream))
-    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", k->rows_variant);
+  if (!launch_rows(d, rows_variant, optin, x, y, shift, ctx->stream))
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", rows_variant);
   QD_CHECK_LAUNCH(ctx);
-  if (!launch_slab(d, k->slab_variant, ctx->num_cus, x, y, ctx->stream))
-    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker slab variant %d", k->slab_variant);
+  if (!launch_slab(d, slab_variant, ctx->num_cus, x, y, ctx->stream))
+    return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker slab variant %d", slab_variant);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }
 
 int qd_kron_prepare(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
-  if (!k || k->rows_variant >= 0 || k->desc.nA == 0 || k->desc.nB == 0) return QD_OK;
+  if (!k || k->variants.load() >= 0 || k->desc.nA == 0 || k->desc.nB == 0) return QD_OK;
   return kron_tune(ctx, k, x, y);
 }

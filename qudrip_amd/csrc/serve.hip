@@ -205,14 +205,17 @@ size_t request_footprint(const Model& M, const qd_request& r) {
 // <D> in the normalised state psi and |psi|^2
 int diag_expect(qd_ctx* ctx, int64_t n, const cplx* d, const cplx* psi, double* dexp, double* nrm2) {
   const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + DX_THREADS - 1) / DX_THREADS));
+  // partials from the server's pool: a hipMalloc / hipFree per request would
+  // synchronise the whole device and stall the other workers' streams
   double2* part = nullptr;
-  QD_HIP(ctx, hipMalloc(&part, sizeof(double2) * (nb + 1)));
+  if (qd_dev_alloc(ctx, sizeof(double2) * (nb + 1), (void**)&part) != QD_OK)
+    return qd_set_error(ctx, QD_E_OOM, "diag_expect partials");
   diag_expect_partial<<<nb, DX_THREADS, 0, ctx->stream>>>(n, d, psi, part);
   diag_expect_final<<<1, 64, 0, ctx->stream>>>(nb, part, part + nb);
   double2 h;
   hipError_t e1 = hipMemcpyAsync(&h, part + nb, sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
   hipError_t e2 = hipStreamSynchronize(ctx->stream);
-  hipFree(part);
+  qd_dev_free(ctx, part);
   if (e1 != hipSuccess || e2 != hipSuccess) return qd_set_error(ctx, QD_E_HIP, "diag_expect");
   *dexp = h.y > 0.0 ? h.x / h.y : 0.0;
   *nrm2 = h.y;
@@ -441,7 +444,7 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
   bool ok = true;
   auto dalloc = [&](size_t bytes) -> void* {
     void* p = nullptr;
-    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
+    if (qd_dev_alloc(ctx, bytes ? bytes : 16, &p) != QD_OK) {
       ok = false;
       return nullptr;
     }
@@ -639,7 +642,7 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
     S->batch_active[model] = 0;
   }
   hipStreamSynchronize(st);
-  for (void* p : allocs) hipFree(p);
+  for (void* p : allocs) qd_dev_free(ctx, p);
   {
     std::lock_guard<std::mutex> lk(S->mu);
     S->used -= (size_t)C * first.footprint;

@@ -110,3 +110,33 @@ def test_admission_defers_and_rejects(ctx):
     for i in range(5):
         e = _dense_e0(12, 0.5 + 0.1 * i)
         assert abs(res[i]["energy"] - e) <= 1e-10 * abs(e)
+
+
+def test_served_config5_sizes_match_plain_and_oracle(ctx):
+    """Requests of the serving workload's own sizes: XXZ ground states at N = 16
+    and N = 24 (n = 12,870 and 2,704,156; the workload's Lanczos k = 150,
+    chunks of 4, Ritz-bound stop 1e-12) and one 3x4 Hubbard real-time solve
+    (n = 853,776, 100 steps of dt = 0.05, Krylov dimension by Saad's
+    estimate), against the plain one-at-a-time path and, for N = 16, the
+    oracle's Lanczos (sparse_algorithm.hpp:395-425,564-585) on the oracle's
+    own matrix."""
+    import oracle
+    reqs = [{"tag": 0, "arrival": 0.0, "kind": "gs", "N": 16, "coeff": 0.8321, "seed": 41},
+            {"tag": 1, "arrival": 0.001, "kind": "rt", "coeff": 5.4, "seed": 42},
+            {"tag": 2, "arrival": 0.002, "kind": "gs", "N": 24, "coeff": 1.37, "seed": 43}]
+    print("serving at the workload's sizes", flush=True)
+    served = SV.run_served(reqs, workers=4, hbm_budget=64 << 30)
+    plain = SV.run_plain(reqs, ctx)
+    for r in reqs:
+        s, p = served["results"][r["tag"]], plain["results"][r["tag"]]
+        assert s["status"] == 0, r
+        assert abs(s["energy"] - p["energy"]) <= 1e-10 * abs(p["energy"]), (r, s["energy"], p["energy"])
+        if r["kind"] == "rt":
+            assert abs(s["energy_start"] - p["energy_start"]) <= 1e-12 * abs(p["energy_start"])
+            assert abs(s["observable"] - p["observable"]) <= 1e-9
+            assert abs(s["energy"] - s["energy_start"]) <= 1e-9 * abs(s["energy_start"])
+    idx, ham = Q.xxz_chain(16, 1.0, 0.8321, pbc=True)
+    basis = oracle.basis_scan(Q.sz_sector(idx, 16, 8))
+    csc = oracle.build_csc(basis, ham)
+    e_orc = oracle.lanczos_gs(csc, 150, oracle.seeded_state(len(basis), 41))["E0"]
+    assert abs(served["results"][0]["energy"] - e_orc) <= 1e-10 * abs(e_orc), (served["results"][0]["energy"], e_orc)
