@@ -14,6 +14,7 @@
 
 #include <complex>
 #include <memory>
+#include <stdexcept>
 #include <vector>
 
 #include "index.hpp"
@@ -148,7 +149,7 @@ hopBitOp<T> getHopBoseGate(QbitsIndex<T>& index) {
 // ------------------------------------------------------------------ SpMatrix
 namespace detail {
 struct CsrHolder {
-  std::shared_ptr<BasisHolder> basis;
+  std::shared_ptr<detail::BasisHolder> basis;
   qd_termprog* prog = nullptr;
   qd_csr* csr = nullptr;
   qd_linop* op = nullptr;
@@ -173,14 +174,54 @@ struct CsrHolder {
 }  // namespace detail
 
 // Device-resident matrix returned by `>>` (the reference's SpMatrix,
-// Eigen ColMajor with int32 indices, matrix.hpp:11).
+// Eigen ColMajor with int32 indices, matrix.hpp:11).  The arithmetic the
+// reference's callers write on it (programs/hubbard_diss.cpp:106-159,
+// 443-446, 514-529) runs on the device with Eigen's element rules:
+//   SpMatrix(n, n), setZero(), setIdentity()
+//   A + B, A - B, A += B, A -= B, -A      union of the patterns, a lone
+//                                         entry is a + 0 / 0 + b / 0 - b
+//   c * A, A * c (complex or double c)    per entry; nothing pruned
+//   A.adjoint()                           conjugate transpose
+//   A * B                                 conservative sparse product
+// Every operation returns a new matrix; operands are never modified except
+// by the compound assignments, which rebind the left-hand side.
 class SpMatrix {
   std::shared_ptr<detail::CsrHolder> h_;
+
+  static SpMatrix wrap(qd_csr* c, const std::shared_ptr<detail::BasisHolder>& basis) {
+    auto h = std::make_shared<detail::CsrHolder>();
+    h->basis = basis;
+    h->csr = c;
+    return SpMatrix(h);
+  }
+  const std::shared_ptr<detail::BasisHolder>& basis_of(const SpMatrix& o) const { return h_->basis ? h_->basis : o.h_->basis; }
+  SpMatrix scaled(value_type c, bool real) const {
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_scale(detail::ctx(), h_->csr, c.real(), real ? 0.0 : c.imag(), real ? 1 : 0, &r),
+                  "qd_csr_scale");
+    return wrap(r, h_->basis);
+  }
+  SpMatrix combined(const SpMatrix& o, int sign) const {
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_add(detail::ctx(), h_->csr, o.h_->csr, sign, &r), "qd_csr_add");
+    return wrap(r, basis_of(o));
+  }
 
  public:
   SpMatrix() = default;
   explicit SpMatrix(std::shared_ptr<detail::CsrHolder> h) : h_(std::move(h)) {}
-  long rows() const { return h_ ? (long)qd_basis_size(h_->basis->b) : 0; }
+  // SparseMatrix(rows, cols): square, no stored entries
+  SpMatrix(long rows, long cols) {
+    if (rows != cols) throw std::runtime_error("SpMatrix: only square matrices are supported");
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_zero(detail::ctx(), rows, &r), "qd_csr_zero");
+    *this = wrap(r, nullptr);
+  }
+  long rows() const {
+    int64_t n = 0;
+    if (h_) qd_csr_info(h_->csr, &n, nullptr, nullptr);
+    return (long)n;
+  }
   long cols() const { return rows(); }
   long nonZeros() const {
     int64_t n = 0, nnz = 0;
@@ -204,6 +245,37 @@ class SpMatrix {
     if (!h_->op) detail::check(qd_linop_from_csr(detail::ctx(), h_->csr, &h_->op), "qd_linop_from_csr");
     return h_->op;
   }
+
+  // ---- algebra
+  SpMatrix& setZero() {
+    *this = SpMatrix(rows(), rows());
+    return *this;
+  }
+  SpMatrix& setIdentity() {
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_identity(detail::ctx(), rows(), &r), "qd_csr_identity");
+    *this = wrap(r, h_->basis);
+    return *this;
+  }
+  SpMatrix adjoint() const {
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_adjoint(detail::ctx(), h_->csr, &r), "qd_csr_adjoint");
+    return wrap(r, h_->basis);
+  }
+  SpMatrix operator+(const SpMatrix& o) const { return combined(o, 1); }
+  SpMatrix operator-(const SpMatrix& o) const { return combined(o, -1); }
+  SpMatrix& operator+=(const SpMatrix& o) { return *this = combined(o, 1); }
+  SpMatrix& operator-=(const SpMatrix& o) { return *this = combined(o, -1); }
+  SpMatrix operator-() const { return scaled(value_type(-1.0, 0.0), true); }  // exact negation
+  SpMatrix operator*(const SpMatrix& o) const {
+    qd_csr* r = nullptr;
+    detail::check(qd_csr_multiply(detail::ctx(), h_->csr, o.h_->csr, &r), "qd_csr_multiply");
+    return wrap(r, basis_of(o));
+  }
+  friend SpMatrix operator*(value_type c, const SpMatrix& m) { return m.scaled(c, false); }
+  friend SpMatrix operator*(double c, const SpMatrix& m) { return m.scaled(value_type(c, 0.0), true); }
+  friend SpMatrix operator*(const SpMatrix& m, value_type c) { return m.scaled(c, false); }
+  friend SpMatrix operator*(const SpMatrix& m, double c) { return m.scaled(value_type(c, 0.0), true); }
 };
 
 // ------------------------------------------------------------------ sums

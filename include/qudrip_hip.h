@@ -202,6 +202,34 @@ int qd_csr_download(qd_csr* csr, int32_t* h_outer, int32_t* h_inner, double* h_v
 int qd_csr_device_arrays(qd_csr* csr, const int32_t** d_outer, const int32_t** d_inner,
                          const double** d_vals);
 
+/* ------------------------------------------------------ SpMatrix algebra */
+/* Arithmetic between stored n x n matrices, on the device, each returning a
+ * new matrix in the same ColMajor storage, with the element rules of the
+ * Eigen expressions the reference's callers write on SpMatrix
+ * (programs/hubbard_diss.cpp:106-159,443-446,514-517).  Operands must have
+ * the same n; they are read only.  A result without a term program gets
+ * its adjoint (for the product kernels) from qd_csr_adjoint when needed. */
+/* SpMatrix(n, n) / setZero(): no stored entries (hubbard_diss.cpp:106-116). */
+int qd_csr_zero(qd_ctx* ctx, int64_t n, qd_csr** out);
+/* setIdentity(): one entry (1, 0) per column (hubbard_diss.cpp:117). */
+int qd_csr_identity(qd_ctx* ctx, int64_t n, qd_csr** out);
+/* c * A: every value times c (the complex product; real_scalar != 0 scales
+ * both parts by c_re, Eigen's double * complex), pattern and explicit zeros
+ * kept (hubbard_diss.cpp:135,443,516). */
+int qd_csr_scale(qd_ctx* ctx, const qd_csr* a, double c_re, double c_im, int real_scalar, qd_csr** out);
+/* A + B (sign = 1) or A - B (sign = -1), also A += B and A -= B
+ * (hubbard_diss.cpp:135-156,443,516): the union of the patterns; an entry
+ * of one side only is a + 0, 0 + b or 0 - b; no pruning. */
+int qd_csr_add(qd_ctx* ctx, const qd_csr* a, const qd_csr* b, int sign, qd_csr** out);
+/* SpMatrix(A.adjoint()) (hubbard_diss.cpp:156,526): the conjugate transpose,
+ * rows ascending in each column. */
+int qd_csr_adjoint(qd_ctx* ctx, const qd_csr* a, qd_csr** out);
+/* A * B (hubbard_diss.cpp:156,158,526): the conservative sparse product;
+ * column j visits B's column j in order and for each (k, y) A's column k in
+ * order, the first visit of a row stores x * y and later ones add x * y.
+ * QD_E_CAPACITY if a column visits more than 1024 entries of A. */
+int qd_csr_multiply(qd_ctx* ctx, const qd_csr* a, const qd_csr* b, qd_csr** out);
+
 /* ------------------------------------------------------- linear operators */
 /* y = H x through the stored matrix. */
 int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out);

@@ -78,6 +78,14 @@ def lib():
     L.orc_traj_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
     L.orc_arnoldi_evolve.restype = ctypes.c_int
     L.orc_arnoldi_evolve.argtypes = [ctypes.c_int64, _VP, _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, _VP]
+    L.orc_csr_add.restype = ctypes.c_int64
+    L.orc_csr_add.argtypes = [ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int, _VP, _VP, _VP]
+    L.orc_csr_scale.restype = None
+    L.orc_csr_scale.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _VP, _VP]
+    L.orc_csr_adjoint.restype = None
+    L.orc_csr_adjoint.argtypes = [ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP]
+    L.orc_csr_multiply.restype = ctypes.c_int64
+    L.orc_csr_multiply.argtypes = [ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
     _lib = L
     return L
 
@@ -298,3 +306,71 @@ def traj_run(opsum, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, 
     if rc != 0:
         raise RuntimeError("oracle trajectories failed")
     return dsum.reshape(nsteps + 1, M), log.reshape(ntraj, max(nsteps, 1))
+
+
+# ------------------------------------------------------------ SpMatrix algebra
+# Matrices are (outer int32[n+1], inner int32[nnz], vals complex128[nnz]),
+# the arrays build_csc returns (spalg.c restates Eigen's rules).
+def _c(csc):
+    """(outer, inner, vals) with vals complex128, or interleaved float64 as
+    build_csc returns them."""
+    o, i, v = csc
+    v = np.ascontiguousarray(v)
+    v = v.view(np.complex128) if v.dtype == np.float64 else v.astype(np.complex128, copy=False)
+    return (np.ascontiguousarray(o, dtype=np.int32), np.ascontiguousarray(i, dtype=np.int32),
+            np.ascontiguousarray(v))
+
+
+def csr_add(a, b, sign=1):
+    """A + B (sign 1) / A - B (sign -1) (hubbard_diss.cpp:135-156,443,516)."""
+    ao, ai, av = _c(a)
+    bo, bi, bv = _c(b)
+    n = len(ao) - 1
+    nnz = lib().orc_csr_add(n, _p(ao), _p(ai), _p(av), _p(bo), _p(bi), _p(bv), sign, None, None, None)
+    co = np.empty(n + 1, dtype=np.int32)
+    ci = np.empty(max(nnz, 1), dtype=np.int32)
+    cv = np.empty(max(nnz, 1), dtype=np.complex128)
+    lib().orc_csr_add(n, _p(ao), _p(ai), _p(av), _p(bo), _p(bi), _p(bv), sign, _p(co), _p(ci), _p(cv))
+    return co, ci[:nnz], cv[:nnz]
+
+
+def csr_scale(a, c):
+    """c * A; a Python float is a real scalar (Eigen's double * complex)."""
+    ao, ai, av = _c(a)
+    real = not isinstance(c, complex)
+    c = complex(c)
+    cv = np.empty_like(av)
+    lib().orc_csr_scale(len(ai), c.real, c.imag, 1 if real else 0, _p(av), _p(cv))
+    return ao.copy(), ai.copy(), cv
+
+
+def csr_adjoint(a):
+    """SpMatrix(A.adjoint()) (hubbard_diss.cpp:156,526)."""
+    ao, ai, av = _c(a)
+    n = len(ao) - 1
+    co = np.empty(n + 1, dtype=np.int32)
+    ci = np.empty(max(len(ai), 1), dtype=np.int32)
+    cv = np.empty(max(len(ai), 1), dtype=np.complex128)
+    lib().orc_csr_adjoint(n, _p(ao), _p(ai), _p(av), _p(co), _p(ci), _p(cv))
+    return co, ci[:len(ai)], cv[:len(ai)]
+
+
+def csr_multiply(a, b):
+    """A * B, Eigen's conservative sparse product (hubbard_diss.cpp:156,526)."""
+    ao, ai, av = _c(a)
+    bo, bi, bv = _c(b)
+    n = len(ao) - 1
+    nnz = lib().orc_csr_multiply(n, _p(ao), _p(ai), _p(av), _p(bo), _p(bi), _p(bv), None, None, None)
+    co = np.empty(n + 1, dtype=np.int32)
+    ci = np.empty(max(nnz, 1), dtype=np.int32)
+    cv = np.empty(max(nnz, 1), dtype=np.complex128)
+    lib().orc_csr_multiply(n, _p(ao), _p(ai), _p(av), _p(bo), _p(bi), _p(bv), _p(co), _p(ci), _p(cv))
+    return co, ci[:nnz], cv[:nnz]
+
+
+def csr_identity(n):
+    return (np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n, dtype=np.complex128))
+
+
+def csr_zero(n):
+    return np.zeros(n + 1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0, dtype=np.complex128)

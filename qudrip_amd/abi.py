@@ -122,6 +122,12 @@ SIGNATURES = {
     "qd_csr_info": (I, [VP, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I)]),
     "qd_csr_download": (I, [VP, VP, VP, VP]),
     "qd_csr_device_arrays": (I, [VP, PP, PP, PP]),
+    "qd_csr_zero": (I, [VP, I64, PP]),
+    "qd_csr_identity": (I, [VP, I64, PP]),
+    "qd_csr_scale": (I, [VP, VP, ctypes.c_double, ctypes.c_double, I, PP]),
+    "qd_csr_add": (I, [VP, VP, VP, I, PP]),
+    "qd_csr_adjoint": (I, [VP, VP, PP]),
+    "qd_csr_multiply": (I, [VP, VP, VP, PP]),
     "qd_linop_from_csr": (I, [VP, VP, PP]),
     "qd_linop_matfree": (I, [VP, VP, VP, PP]),
     "qd_linop_destroy": (I, [VP]),

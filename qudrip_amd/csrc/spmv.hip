@@ -509,9 +509,10 @@ extern "C" int qd_linop_from_csr(qd_ctx* ctx, qd_csr* csr, qd_linop** out) {
   qd_csr* use = csr;
   if (csr->hermitian != 1) {
     if (!csr->adj) {
-      if (!csr->basis || !csr->prog)
-        return qd_set_error(ctx, QD_E_UNSUPPORTED, "non-Hermitian matrix without its term program");
-      int rc = qd_csr_build_impl(ctx, csr->basis, csr->prog, 1, &csr->adj);
+      // the adjoint's CSC: from the adjoint term program, or (a matrix made
+      // by the SpMatrix algebra has none) the conjugate transpose
+      int rc = (csr->basis && csr->prog) ? qd_csr_build_impl(ctx, csr->basis, csr->prog, 1, &csr->adj)
+                                         : qd_csr_adjoint(ctx, csr, &csr->adj);
       if (rc) return rc;
     }
     use = csr->adj;

@@ -121,14 +121,47 @@ class TermProgram:
 class CSR:
     """`ham >> index` on the GPU (reference ColMajor arrays)."""
 
-    def __init__(self, ctx, basis, prog):
+    def __init__(self, ctx, basis, prog, _ptr=None):
         self.ctx = ctx
         p = ctypes.c_void_p()
-        ctx.check(ctx.lib.qd_csr_build(ctx.ptr, basis.ptr, prog.ptr, ctypes.byref(p)), "csr_build")
+        if _ptr is None:
+            ctx.check(ctx.lib.qd_csr_build(ctx.ptr, basis.ptr, prog.ptr, ctypes.byref(p)), "csr_build")
+        else:
+            p = _ptr
         self.ptr = p
         n, nnz, herm = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
         ctx.check(ctx.lib.qd_csr_info(p, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(herm)), "csr_info")
         self.n, self.nnz, self.hermitian = n.value, nnz.value, herm.value
+
+    # SpMatrix algebra on the device (qd_csr_zero ... qd_csr_multiply)
+    @classmethod
+    def _made(cls, ctx, fn, *args):
+        p = ctypes.c_void_p()
+        ctx.check(fn(ctx.ptr, *args, ctypes.byref(p)), fn.__name__)
+        return cls(ctx, None, None, _ptr=p)
+
+    @classmethod
+    def zero(cls, ctx, n):
+        return cls._made(ctx, ctx.lib.qd_csr_zero, n)
+
+    @classmethod
+    def identity(cls, ctx, n):
+        return cls._made(ctx, ctx.lib.qd_csr_identity, n)
+
+    def scale(self, c):
+        """c * A; a Python float is a real scalar (Eigen's double * complex)."""
+        real = not isinstance(c, complex)
+        c = complex(c)
+        return CSR._made(self.ctx, self.ctx.lib.qd_csr_scale, self.ptr, c.real, c.imag, 1 if real else 0)
+
+    def add(self, other, sign=1):
+        return CSR._made(self.ctx, self.ctx.lib.qd_csr_add, self.ptr, other.ptr, sign)
+
+    def adjoint(self):
+        return CSR._made(self.ctx, self.ctx.lib.qd_csr_adjoint, self.ptr)
+
+    def multiply(self, other):
+        return CSR._made(self.ctx, self.ctx.lib.qd_csr_multiply, self.ptr, other.ptr)
 
     def download(self):
         outer = np.empty(self.n + 1, dtype=np.int32)

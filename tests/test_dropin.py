@@ -76,7 +76,7 @@ int main() {
 
 
 def test_driver_programs_built():
-    for p in ("spin", "dropin_dump", "mc_dump"):
+    for p in ("spin", "dropin_dump", "mc_dump", "hubbard_quench"):
         assert os.access(os.path.join(BIN, p), os.X_OK), p
 
 
