@@ -1,7 +1,7 @@
 // qudrip/index.hpp -- Hilbert-space indices and conserved quantities.
 //
 // Same call surface as the reference:
-//   getQbits(N)                         index.hpp:397 (QbitsIndex :177-191)
+//   getQbits(N)                         index.hpp:300 (QbitsIndex :80-95)
 //   getNset / getNup / getNdo, `cq = v`  conservation.hpp:89-115, 233-246
 //   getNd / getNh                         conservation.hpp:117-151, 248-256
 //   SingleModeIndex, Indices (q * m)      index.hpp:94-184
@@ -72,7 +72,7 @@ class SingleModeIndex {
 };
 
 // Tensor product of a QbitsIndex and a SingleModeIndex (Indices,
-// index.hpp:112-175): value q * mdim + m, the mode the least significant
+// index.hpp:112-177): value q * mdim + m, the mode the least significant
 // digit; `q * m` (:177-184).
 template <class L, class R>
 class Indices {
@@ -194,7 +194,7 @@ inline std::shared_ptr<BasisHolder> make_basis_modes(int nbits, int mdim, const 
 }
 }  // namespace detail
 
-// Constrained index (SubIndex, index.hpp:284-358): ascending sector table,
+// Constrained index (SubIndex, index.hpp:187-261): ascending sector table,
 // unrank via idx[i], rank via (idxv_type)idx, a miss ranks to range().
 template <class E = QbitsIndex<>>
 class SubIndex {

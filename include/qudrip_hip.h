@@ -157,14 +157,14 @@ int qd_basis_create_q(qd_ctx* ctx, int nbits, const qd_quantity* q, int nq, qd_b
  * rank(q) * mdim + m, a miss ranks to the size. */
 int qd_basis_create_modes(qd_ctx* ctx, int nbits, int mdim, const qd_quantity* q, int nq, qd_basis** out);
 int qd_basis_destroy(qd_basis* basis);
-/* SubIndex::range() (index.hpp:357). */
+/* SubIndex::range() (index.hpp:261). */
 int64_t qd_basis_size(const qd_basis* basis);
 /* SubIndex::mapping, i.e. the unrank table (index.hpp:242-248); host buffer
  * of qd_basis_size() entries. */
 int qd_basis_download(qd_basis* basis, uint64_t* h_states);
 /* Device pointer to the same table (valid while the basis lives). */
 const uint64_t* qd_basis_device_states(qd_basis* basis);
-/* SubIndex::operator idxv_type() (index.hpp:331-337): rank of each state;
+/* SubIndex::operator idxv_type() (index.hpp:234-240): rank of each state;
  * a state outside the sector ranks to qd_basis_size().  Device arrays. */
 int qd_basis_rank(qd_basis* basis, const uint64_t* d_states, int64_t count,
                   int64_t* d_ranks);

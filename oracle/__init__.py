@@ -123,7 +123,7 @@ def basis_scan(sector):
 
 
 def rank(basis, x):
-    """SubIndex -> inv_mapping lookup (index.hpp:331-337); miss -> len(basis)."""
+    """SubIndex -> inv_mapping lookup (index.hpp:234-240); miss -> len(basis)."""
     return lib().orc_rank(_p(basis), len(basis), int(x))
 
 

@@ -108,7 +108,7 @@ int64_t orc_basis_scan_q(int nbits, const orc_quantity* q, int nq, uint64_t* out
   return count;
 }
 
-/* index.hpp:331-337: inv_mapping lookup; a miss returns range() == n.  The
+/* index.hpp:234-240: inv_mapping lookup; a miss returns range() == n.  The
  * mapping is ascending, so a binary search finds the same entry. */
 int64_t orc_rank(const uint64_t* basis, int64_t n, uint64_t x) {
   int64_t lo = 0, hi = n;

@@ -79,7 +79,7 @@ struct SectorDesc {
 };
 
 // rank of x within the sector, or sec.total when x is outside it
-// (index.hpp:331-337 semantics).  CNS form.
+// (index.hpp:234-240 semantics).  CNS form.
 __device__ inline uint64_t qd_rank_cns(const SectorDesc& sec, uint64_t x) {
   if (sec.nbits < 64 && (x >> sec.nbits) != 0) return sec.total;
   uint64_t r = 0;

@@ -27,7 +27,7 @@ id2 = np.array([[1, 0], [0, 1]], dtype=np.complex128)           # bit.hpp:45
 
 
 class QbitsIndex:
-    """QbitsIndex (index.hpp:177-191): range 1 << N."""
+    """QbitsIndex (index.hpp:80-95): range 1 << N."""
 
     def __init__(self, nbits):
         self.nbits = int(nbits)
@@ -36,7 +36,7 @@ class QbitsIndex:
         return 1 << self.nbits
 
 
-    def __mul__(self, other):  # operator* of two indices (index.hpp:177-184)
+    def __mul__(self, other):  # operator* of two indices (index.hpp:179-184)
         if isinstance(other, SingleModeIndex):
             return Indices(self, other)
         return NotImplemented
@@ -57,7 +57,7 @@ class SingleModeIndex:
 
 
 class Indices:
-    """Indices (index.hpp:112-175) of a QbitsIndex and a SingleModeIndex:
+    """Indices (index.hpp:112-177) of a QbitsIndex and a SingleModeIndex:
     value = q * mdim + m, the mode the least significant digit."""
 
     def __init__(self, lhs, rhs):
