@@ -12,12 +12,17 @@
 //   pass 2 (getKrylovGS, :166-233) reruns the recurrence from psi with the
 //     real part of a and accumulates out = sum_i c_i q_i.
 // Two raw vectors hold the recurrence (q = s * raw with a device-side scale
-// s, so no normalisation pass), a third the product.  Per step: the product,
-// one dot pass (2 vectors), one update pass (3 read, 1 written; pass 2 also
-// accumulates out), and two one-block scalar kernels; every kernel returns at
-// once after the recurrence has stopped, so a whole pass is launched without
-// a host round trip.
+// s, so no normalisation pass), a third the product.  Per step three
+// launches: the product fused with the block sums of <raw|H raw> (stored
+// matrices: the CSR-stream kernel writes them; other operators: the product
+// then a dot pass), one one-block kernel that closes the previous step's
+// |r|^2 (next scale, b, stopping rule) and this step's a, and the update
+// pass (3 read, 1 written; pass 2 also accumulates out) with the block sums
+// of |r|^2.  Kernel boundaries order the sums; every kernel returns at once
+// after the recurrence has stopped, so a whole pass is launched without a
+// host round trip.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -132,25 +137,78 @@ __global__ void __launch_bounds__(HT) hlz_start_scalars(const cplx* __restrict__
   }
 }
 
-// a = s_x^2 sum(part) (the product was applied to the raw vector);
-// pass 1 records it, pass 2 keeps its real part.
-__global__ void __launch_bounds__(HT) hlz_alpha_scalars(const cplx* __restrict__ part, int nb,
-                                                        HlzDev* __restrict__ dev, cplx* __restrict__ alpha,
-                                                        int step, int cx, int pass2) {
+// One-block kernel between the product and the update of step `step`
+// (FT threads, fixed summation order):
+//  - nb_norm > 0: |r|^2 = sum pnorm (the previous step's update) gives the
+//    scale of raw vector cx; pass 1 also records b = |r| of step - 1,
+//    error() and the stopping rule of getKrylovReprAd: stop when
+//    |r|^2 <= eps, or, from step 1 on, when the count of a exceeds kry_dim;
+//  - nb_dot > 0 (and not stopped): a = s_x^2 sum pdot (the product was
+//    applied to the raw vector); pass 1 records it, pass 2 keeps its real
+//    part.
+constexpr int FT = 1024;
+__device__ inline cplx block_sum_ft(cplx v, cplx* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  cplx t = sh[0];
+  for (int q = 1; q < FT / 64; ++q) t = cadd(t, sh[q]);
+  __syncthreads();
+  return t;
+}
+// this thread's share of part[0 .. m): entries t, t + FT, ... in four
+// interleaved accumulators (four loads in flight), combined in a fixed order
+__device__ inline cplx strided_sum(const cplx* __restrict__ part, int m) {
+  cplx a0 = cmake(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+  int b = threadIdx.x;
+  for (; b + 3 * FT < m; b += 4 * FT) {
+    a0 = cadd(a0, part[b]);
+    a1 = cadd(a1, part[b + FT]);
+    a2 = cadd(a2, part[b + 2 * FT]);
+    a3 = cadd(a3, part[b + 3 * FT]);
+  }
+  for (; b < m; b += FT) a0 = cadd(a0, part[b]);
+  return cadd(cadd(a0, a1), cadd(a2, a3));
+}
+__global__ void __launch_bounds__(FT)
+    hlz_finish_kernel(const cplx* __restrict__ pnorm, int nb_norm, const cplx* __restrict__ pdot, int nb_dot,
+                      HlzDev* __restrict__ dev, cplx* __restrict__ alpha, double* __restrict__ beta, int step, int cx,
+                      int kry_dim, int pass2) {
   if (dev->stop) return;
-  __shared__ cplx sh[HT / 64];
-  cplx acc = cmake(0.0, 0.0);
-  for (int b = threadIdx.x; b < nb; b += HT) acc = cadd(acc, part[b]);
-  const cplx t = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    const double s2 = dev->s[cx] * dev->s[cx];
-    const cplx a = crmul(s2, t);
-    if (pass2) {
-      dev->a = cmake(a.x, 0.0);
-    } else {
-      dev->a = a;
-      alpha[step] = a;
-      dev->dim = step + 1;
+  __shared__ cplx sh[FT / 64];
+  __shared__ int stopped;
+  if (nb_norm > 0) {
+    const cplx t = block_sum_ft(strided_sum(pnorm, nb_norm), sh);
+    if (threadIdx.x == 0) {
+      const double n2 = t.x;
+      const int prev = step - 1;
+      dev->nrm2 = n2;
+      dev->s[cx] = 1.0 / sqrt(n2);
+      int st = 0;
+      if (!pass2) {
+        beta[prev] = sqrt(n2);
+        dev->err = sqrt(n2);
+        if (!(n2 > QD_LEAF_EPS) || (prev >= 1 && prev + 1 > kry_dim)) st = 1;
+      }
+      dev->stop = st;
+      stopped = st;
+    }
+    __syncthreads();
+    if (stopped) return;
+  }
+  if (nb_dot > 0) {
+    const cplx t = block_sum_ft(strided_sum(pdot, nb_dot), sh);
+    if (threadIdx.x == 0) {
+      const double s2 = dev->s[cx] * dev->s[cx];
+      const cplx a = crmul(s2, t);
+      if (pass2) {
+        dev->a = cmake(a.x, 0.0);
+      } else {
+        dev->a = a;
+        alpha[step] = a;
+        dev->dim = step + 1;
+      }
     }
   }
 }
@@ -190,29 +248,6 @@ __global__ void __launch_bounds__(HT)
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// |r|^2 -> scale of the new vector; pass 1 also records b = |r|, error()
-// and the stopping rule of getKrylovReprAd: stop when |r|^2 <= eps, or, from
-// step 1 on, when the count of a exceeds kry_dim.
-__global__ void __launch_bounds__(HT) hlz_norm_scalars(const cplx* __restrict__ part, int nb,
-                                                       HlzDev* __restrict__ dev, double* __restrict__ beta,
-                                                       int step, int cx, int kry_dim, int pass2) {
-  if (dev->stop) return;
-  __shared__ cplx sh[HT / 64];
-  cplx acc = cmake(0.0, 0.0);
-  for (int b = threadIdx.x; b < nb; b += HT) acc = cadd(acc, part[b]);
-  const cplx t = block_sum(acc, sh);
-  if (threadIdx.x == 0) {
-    const double n2 = t.x;
-    dev->nrm2 = n2;
-    dev->s[cx ^ 1] = 1.0 / sqrt(n2);
-    if (!pass2) {
-      beta[step] = sqrt(n2);
-      dev->err = sqrt(n2);
-      if (!(n2 > QD_LEAF_EPS) || (step >= 1 && step + 1 > kry_dim)) dev->stop = 1;
-    }
-  }
-}
-
 // out (+)= c s_x x: the last term of pass 2, whose next vector is not needed
 // (set: the only term, out not read)
 __global__ void __launch_bounds__(HT) hlz_acc_kernel(int64_t n, const cplx* __restrict__ x, cplx* __restrict__ out,
@@ -237,7 +272,9 @@ struct qd_hlanczos {
   cplx* d_alpha = nullptr;
   double* d_beta = nullptr;
   cplx* d_coef = nullptr;
-  cplx* d_part = nullptr;
+  cplx* d_part = nullptr;    // block sums of the start and update passes (nb)
+  cplx* d_pdot = nullptr;    // block sums of <raw|H raw> (dot_blocks)
+  int dot_blocks = 0;        // blocks of the fused product (stored matrices)
   // host copies of the last solve
   int dim = 0;
   double err = 0.0;
@@ -251,21 +288,54 @@ namespace {
 
 int hlz_fail(qd_hlanczos* lz, int code, const char* what) { return qd_set_error(lz->ctx, code, "%s", what); }
 
-// launches of the step kernels with the access form chosen by vector size
-void launch_dot(int64_t n, int nb, hipStream_t st, const cplx* x, const cplx* y, cplx* part, const HlzDev* dev) {
-  if (n * (int64_t)sizeof(cplx) > kNtBytes)
-    hlz_dot_kernel<true><<<nb, HT, 0, st>>>(n, x, y, part, dev);
+// access form of the vector kernels: non-temporal above the MALL size;
+// QD_HLZ_NT=0 / 1 overrides the size rule (the two forms give the same bits)
+bool use_nt(int64_t n) {
+  if (const char* e = std::getenv("QD_HLZ_NT")) return e[0] == '1';
+  return n * (int64_t)sizeof(cplx) > kNtBytes;
+}
+
+// y = H raw_x and the block sums of <raw_x|y> in d_pdot: one fused launch
+// for stored matrices, else the product and a dot pass.  Returns the number
+// of block sums through *npart.
+int step_product(qd_hlanczos* lz, qd_linop* op, int cx, int* npart) {
+  qd_ctx* ctx = lz->ctx;
+  const int64_t n = lz->n;
+  int rc = qd_linop_apply_dot(op, lz->d_raw[cx], lz->d_y, lz->d_pdot, true);
+  if (rc == QD_OK) {
+    *npart = lz->dot_blocks;
+    return QD_OK;
+  }
+  if (rc != QD_E_UNSUPPORTED) return rc;
+  rc = qd_linop_apply(op, lz->d_raw[cx], lz->d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+  if (rc) return rc;
+  if (use_nt(n))
+    hlz_dot_kernel<true><<<lz->nb, HT, 0, ctx->stream>>>(n, lz->d_raw[cx], lz->d_y, lz->d_pdot, lz->d_dev);
   else
-    hlz_dot_kernel<false><<<nb, HT, 0, st>>>(n, x, y, part, dev);
+    hlz_dot_kernel<false><<<lz->nb, HT, 0, ctx->stream>>>(n, lz->d_raw[cx], lz->d_y, lz->d_pdot, lz->d_dev);
+  QD_CHECK_LAUNCH(ctx);
+  *npart = lz->nb;
+  return QD_OK;
+}
+
+void launch_finish(qd_hlanczos* lz, int nb_norm, int nb_dot, int step, int cx, int pass2) {
+  hlz_finish_kernel<<<1, FT, 0, lz->ctx->stream>>>(lz->d_part, nb_norm, lz->d_pdot, nb_dot, lz->d_dev, lz->d_alpha,
+                                                   lz->d_beta, step, cx, lz->kry_dim, pass2);
 }
 
 template <bool FIRST, bool ACC, bool ACC_SET>
-void launch_update(int64_t n, int nb, hipStream_t st, const cplx* y, const cplx* x, cplx* p, cplx* out,
-                   const cplx* coef, int step, int cx, cplx* part, const HlzDev* dev) {
-  if (n * (int64_t)sizeof(cplx) > kNtBytes)
-    hlz_update_kernel<FIRST, ACC, ACC_SET, true><<<nb, HT, 0, st>>>(n, y, x, p, out, coef, step, cx, part, dev);
+void launch_update(qd_hlanczos* lz, int step, int cx, cplx* out) {
+  const int64_t n = lz->n;
+  hipStream_t st = lz->ctx->stream;
+  cplx* y = lz->d_y;
+  const cplx* x = lz->d_raw[cx];
+  cplx* p = lz->d_raw[cx ^ 1];
+  if (use_nt(n))
+    hlz_update_kernel<FIRST, ACC, ACC_SET, true><<<lz->nb, HT, 0, st>>>(n, y, x, p, out, lz->d_coef, step, cx,
+                                                                         lz->d_part, lz->d_dev);
   else
-    hlz_update_kernel<FIRST, ACC, ACC_SET, false><<<nb, HT, 0, st>>>(n, y, x, p, out, coef, step, cx, part, dev);
+    hlz_update_kernel<FIRST, ACC, ACC_SET, false><<<lz->nb, HT, 0, st>>>(n, y, x, p, out, lz->d_coef, step, cx,
+                                                                          lz->d_part, lz->d_dev);
 }
 
 // pass 1: the tridiagonal coefficients of psi's Krylov space
@@ -279,19 +349,19 @@ int hlz_pass1(qd_hlanczos* lz, qd_linop* op, const cplx* psi) {
   QD_CHECK_LAUNCH(ctx);
   for (int i = 0; i <= lz->kry_dim; ++i) {
     const int cx = i & 1;
-    int rc = qd_linop_apply(op, lz->d_raw[cx], lz->d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    int npart = 0;
+    int rc = step_product(lz, op, cx, &npart);
     if (rc) return rc;
-    launch_dot(n, nb, st, lz->d_raw[cx], lz->d_y, lz->d_part, lz->d_dev);
-    hlz_alpha_scalars<<<1, HT, 0, st>>>(lz->d_part, nb, lz->d_dev, lz->d_alpha, i, cx, 0);
+    launch_finish(lz, i == 0 ? 0 : nb, npart, i, cx, 0);
     if (i == 0)
-      launch_update<true, false, false>(n, nb, st, lz->d_y, lz->d_raw[cx], lz->d_raw[cx ^ 1], nullptr, nullptr, i,
-                                        cx, lz->d_part, lz->d_dev);
+      launch_update<true, false, false>(lz, i, cx, nullptr);
     else
-      launch_update<false, false, false>(n, nb, st, lz->d_y, lz->d_raw[cx], lz->d_raw[cx ^ 1], nullptr, nullptr, i,
-                                         cx, lz->d_part, lz->d_dev);
-    hlz_norm_scalars<<<1, HT, 0, st>>>(lz->d_part, nb, lz->d_dev, lz->d_beta, i, cx, lz->kry_dim, 0);
+      launch_update<false, false, false>(lz, i, cx, nullptr);
     QD_CHECK_LAUNCH(ctx);
   }
+  // the last step's |r|^2: b, error() and the stop flag
+  launch_finish(lz, nb, 0, lz->kry_dim + 1, (lz->kry_dim + 1) & 1, 0);
+  QD_CHECK_LAUNCH(ctx);
   HlzDev h{};
   QD_HIP(ctx, hipMemcpyAsync(&h, lz->d_dev, sizeof(h), hipMemcpyDeviceToHost, st));
   QD_HIP(ctx, hipStreamSynchronize(st));
@@ -333,20 +403,19 @@ int hlz_pass2(qd_hlanczos* lz, qd_linop* op, const cplx* psi, cplx* out, const s
   hlz_start_scalars<<<1, HT, 0, st>>>(lz->d_part, nb, lz->d_dev);
   for (int i = 0; i + 1 < m; ++i) {
     const int cx = i & 1;
-    int rc = qd_linop_apply(op, lz->d_raw[cx], lz->d_y, nullptr, 0, nullptr, cmake(0.0, 0.0));
+    int npart = 0;
+    int rc = step_product(lz, op, cx, &npart);
     if (rc) return rc;
     ++lz->n_spmv;
-    launch_dot(n, nb, st, lz->d_raw[cx], lz->d_y, lz->d_part, lz->d_dev);
-    hlz_alpha_scalars<<<1, HT, 0, st>>>(lz->d_part, nb, lz->d_dev, lz->d_alpha, i, cx, 1);
+    launch_finish(lz, i == 0 ? 0 : nb, npart, i, cx, 1);
     if (i == 0)
-      launch_update<true, true, true>(n, nb, st, lz->d_y, lz->d_raw[cx], lz->d_raw[cx ^ 1], out, lz->d_coef, i, cx,
-                                      lz->d_part, lz->d_dev);
+      launch_update<true, true, true>(lz, i, cx, out);
     else
-      launch_update<false, true, false>(n, nb, st, lz->d_y, lz->d_raw[cx], lz->d_raw[cx ^ 1], out, lz->d_coef, i,
-                                        cx, lz->d_part, lz->d_dev);
-    hlz_norm_scalars<<<1, HT, 0, st>>>(lz->d_part, nb, lz->d_dev, lz->d_beta, i, cx, lz->kry_dim, 1);
+      launch_update<false, true, false>(lz, i, cx, out);
     QD_CHECK_LAUNCH(ctx);
   }
+  // the scale of the last vector
+  if (m > 1) launch_finish(lz, nb, 0, m - 1, (m - 1) & 1, 1);
   const int cl = (m - 1) & 1;
   hlz_acc_kernel<<<nb, HT, 0, st>>>(n, lz->d_raw[cl], out, lz->d_coef, m - 1, cl, m == 1, lz->d_dev);
   QD_CHECK_LAUNCH(ctx);
@@ -368,7 +437,12 @@ extern "C" int qd_hlanczos_create(qd_ctx* ctx, int64_t n, int kry_dim, qd_hlancz
   lz->nb = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ctx->num_cus * 8));
   const size_t vb = sizeof(cplx) * (size_t)n;
   const size_t k2 = (size_t)kry_dim + 2;
-  const size_t small = 256 + sizeof(cplx) * k2 * 2 + sizeof(double) * k2 + sizeof(cplx) * (size_t)lz->nb + 1024;
+  // block sums: the start / update passes (nb), and the dot of each step
+  // (the fused product's blocks, or nb from the dot pass)
+  lz->dot_blocks = (int)qd_linop_dot_blocks(n);
+  const int64_t npdot = std::max<int64_t>(lz->nb, lz->dot_blocks);
+  const size_t small = 256 + sizeof(cplx) * k2 * 2 + sizeof(double) * k2 + sizeof(cplx) * (size_t)lz->nb +
+                       sizeof(cplx) * (size_t)npdot + 2 * 256;
   if (qd_dev_alloc(ctx, vb, (void**)&lz->d_raw[0]) != QD_OK || qd_dev_alloc(ctx, vb, (void**)&lz->d_raw[1]) != QD_OK ||
       qd_dev_alloc(ctx, vb, (void**)&lz->d_y) != QD_OK || qd_dev_alloc(ctx, small, (void**)&lz->d_small) != QD_OK) {
     qd_dev_free(ctx, lz->d_raw[0]);
@@ -389,6 +463,9 @@ extern "C" int qd_hlanczos_create(qd_ctx* ctx, int64_t n, int kry_dim, qd_hlancz
   p += sizeof(double) * k2;
   p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
   lz->d_part = (cplx*)p;
+  p += sizeof(cplx) * (size_t)lz->nb;
+  p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+  lz->d_pdot = (cplx*)p;
   *out = lz;
   return QD_OK;
 }

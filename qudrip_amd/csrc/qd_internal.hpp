@@ -389,6 +389,13 @@ int qd_tridiag_eig(int m, const double* d, const double* e, double* w, double* z
 int qd_expm_pade13(int m, const cplx* A, cplx* E);
 int qd_krylov_gs_step(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, double* h_E0, bool want_vector);
 
+// stored-matrix product fused with a dot (spmv.hip): y = (H + diag) x and,
+// per block of 256 rows, the sum of conj(x_r) y_r into part[block] for
+// qd_linop_dot_blocks(n) blocks; QD_E_UNSUPPORTED (nothing launched) for
+// other operators.  write = false skips the store of y.
+int64_t qd_linop_dot_blocks(int64_t n);
+int qd_linop_apply_dot(qd_linop* op, const cplx* x, cplx* y, cplx* part, bool write);
+
 // grid helpers
 static inline unsigned qd_grid(int64_t work, int per_block) {
   int64_t g = (work + per_block - 1) / per_block;

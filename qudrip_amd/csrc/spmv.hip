@@ -7,7 +7,10 @@
 // is a row-parallel "pull" product with no atomics and a fixed summation
 // order.  Non-Hermitian sums use the columns of the adjoint program instead
 // (same formula), built once by qd_linop_from_csr.
+#include <mutex>
+
 #include "qd_internal.hpp"
+#include "wave.hpp"
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 #define QD_STREAM_VARIANTS 15
@@ -264,12 +267,17 @@ __global__ void __launch_bounds__(256)
 // x gathers of slice s, so each wave keeps two memory phases in flight
 // instead of alternating index load -> gather -> products.  Indices are
 // clamped into the slice, so every load stays inside [wbeg, wend).
-template <int U>
+// DOT: the block also sums conj(x_r) y_r over its rows into
+// dot_part[blockIdx.x] (a fixed tree: wavefront butterflies, then the four
+// waves in order); the caller sums the partials in a later kernel.  WRITE =
+// false skips the store of y (an expectation value <x|H|x> without the
+// product vector).
+template <int U, bool DOT = false, bool WRITE = true>
 __global__ void __launch_bounds__(256)
     spmv_stream3_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
                         const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
                         const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
-                        cplx shift) {
+                        cplx shift, cplx* __restrict__ dot_part = nullptr) {
   constexpr int CAPW = 64 * U;
   __shared__ cplx lds[4][CAPW];
   const unsigned nb = gridDim.x;
@@ -277,7 +285,9 @@ __global__ void __launch_bounds__(256)
   const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t r0 = ((int64_t)lb * 4 + w) * 64;
-  if (r0 >= nrows) return;  // whole wave leaves together
+  if (!DOT && r0 >= nrows) return;  // whole wave leaves together
+  cplx dsum = cmake(0.0, 0.0);
+  if (r0 < nrows) {
   const int64_t row = r0 + lane;
   const int32_t nnz_end = outer[nrows];
   int32_t rs = (row < nrows) ? outer[row] : nnz_end;
@@ -359,8 +369,19 @@ __global__ void __launch_bounds__(256)
       cplx d = diag ? cadd(diag[row], shift) : shift;
       r = cadd(r, crmul(s, cmul(d, xr)));
     }
-    __builtin_nontemporal_store(r.x, &y[row].x);
-    __builtin_nontemporal_store(r.y, &y[row].y);
+    if (WRITE) {
+      __builtin_nontemporal_store(r.x, &y[row].x);
+      __builtin_nontemporal_store(r.y, &y[row].y);
+    }
+    if (DOT) dsum = cdotmul(x[row], r);
+  }
+  }
+  if (DOT) {  // every wave of the block reaches this barrier
+    __shared__ cplx wsum[4];
+    dsum = wave_sum(dsum);
+    if (lane == 0) wsum[w] = dsum;
+    __syncthreads();
+    if (threadIdx.x == 0) dot_part[blockIdx.x] = cadd(cadd(wsum[0], wsum[1]), cadd(wsum[2], wsum[3]));
   }
 }
 
@@ -723,6 +744,29 @@ extern "C" int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double*
   if (rc) return rc;
   h_out[0] = r[0];
   h_out[1] = r[1];
+  return QD_OK;
+}
+
+// ------------------------------------------------- product fused with a dot
+int64_t qd_linop_dot_blocks(int64_t n) { return grid8((n + 255) / 256); }
+
+// Stored-matrix operators: y = (H + diag + shift) x in the CSR-stream kernel,
+// whose blocks also write conj(x_r) y_r summed over their rows to
+// part[0 .. qd_linop_dot_blocks(n)).  Other operators: QD_E_UNSUPPORTED,
+// nothing launched.
+int qd_linop_apply_dot(qd_linop* op, const cplx* x, cplx* y, cplx* part, bool write) {
+  qd_ctx* ctx = op->ctx;
+  const int64_t n = op->n;
+  if (op->kind != QD_LINOP_CSR || n == 0) return QD_E_UNSUPPORTED;
+  const unsigned grid = (unsigned)qd_linop_dot_blocks(n);
+  const cplx* diag = op->diag0;
+  if (write)
+    spmv_stream3_kernel<8, true, true><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y, nullptr,
+                                                                      0, diag, cmake(0.0, 0.0), part);
+  else
+    spmv_stream3_kernel<8, true, false><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y,
+                                                                       nullptr, 0, diag, cmake(0.0, 0.0), part);
+  QD_CHECK_LAUNCH(ctx);
   return QD_OK;
 }
 

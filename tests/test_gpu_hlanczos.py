@@ -235,3 +235,57 @@ def test_dropin_hermitian_solver(tmp_path):
     assert np.allclose(E, rgs["evals"], rtol=1e-5, atol=1e-5)  # printed with 6 digits, like the reference
     err = float(r.stdout.split("error = ")[1].split()[0])
     assert abs(err - rgs["err"]) < 1e-9
+
+
+@pytest.mark.parametrize("matfree", [False, True])
+def test_hlanczos_non_temporal_form_is_bit_identical(ctx, matfree, monkeypatch):
+    """The vector kernels' non-temporal form (chosen above 256 MB vectors)
+    forced at n = 924 through QD_HLZ_NT: coefficients, Ritz values and the
+    state bit for bit equal to the ordinary form.  Stored matrix: the
+    product fused with the dot; matrix-free: product then the dot pass."""
+    idx, ham = Q.xxz_chain(12, 1.0, 0.7, pbc=True)
+    sec = Q.Constrain(idx, Q.getNset(idx) == 6)
+    b, p, c = _build(ctx, ham, sec)
+    op = LinOp.matfree(ctx, b, p) if matfree else LinOp.from_csr(ctx, c)
+    out = {}
+    try:
+        for nt in ("0", "1"):
+            monkeypatch.setenv("QD_HLZ_NT", nt)
+            lz = HermitianLanczos(ctx, b.size, 40)
+            psi = ctx.seeded_state(b.size, 3)
+            e0 = lz.ground_state(op, psi)
+            a, be = lz.coefficients()
+            out[nt] = (e0, a, be, psi.cpu().numpy())
+            lz.close()
+    finally:
+        _close(op, c, p, b)
+    e0, a0, b0, s0 = out["0"]
+    e1, a1, b1, s1 = out["1"]
+    assert e0 == e1
+    assert np.array_equal(a0.view(np.uint64), a1.view(np.uint64))
+    assert np.array_equal(b0.view(np.uint64), b1.view(np.uint64))
+    assert np.array_equal(s0.view(np.uint64), s1.view(np.uint64))
+
+
+def test_hlanczos_fused_product_matches_separate_dot(ctx):
+    """The stored-matrix step (product and a = <q|Hq> in one kernel) against
+    the matrix-free step (product, then a dot pass) on the same operator:
+    the leading coefficients agree to rounding, E0 to 1e-12."""
+    idx, ham = Q.xxz_chain(14, 1.0, 1.3, pbc=True)
+    sec = Q.Constrain(idx, Q.getNset(idx) == 7)
+    b, p, c = _build(ctx, ham, sec)
+    ops = [LinOp.from_csr(ctx, c), LinOp.matfree(ctx, b, p)]
+    res = []
+    try:
+        for op in ops:
+            lz = HermitianLanczos(ctx, b.size, 60)
+            psi = ctx.seeded_state(b.size, 9)
+            e0 = lz.ground_state(op, psi)
+            a, be = lz.coefficients()
+            res.append((e0, a, be))
+            lz.close()
+    finally:
+        _close(*ops, c, p, b)
+    (e0, a0, b0), (e1, a1, b1) = res
+    assert abs(e0 - e1) <= 1e-12 * abs(e0)
+    assert np.max(np.abs(a0[:20] - a1[:20])) < 1e-12 and np.max(np.abs(b0[:20] - b1[:20])) < 1e-12
