@@ -273,6 +273,17 @@ int qd_spmv(qd_ctx* ctx, qd_linop* op, const double* d_x, double* d_y);
 int qd_spmv_acc(qd_ctx* ctx, qd_linop* op, const double* h_alpha, const double* d_x, double* d_y);
 /* <psi|H|psi>, unnormalised (State::eval, state.hpp:33-44).  out: host[2]. */
 int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* h_out);
+/* The same value with no device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Stored-
+ * matrix operators (qd_linop_from_csr) of at most 33,554,432 rows
+ * (QD_E_CAPACITY beyond; QD_E_UNSUPPORTED for other operators).  The
+ * product is fused with the dot and never stored; block sums use a fixed
+ * 2 MiB device buffer per GPU held until the process ends.  d_out: device
+ * double[2], written asynchronously on the context's stream (no memset
+ * node: the call may be captured in a graph).  Calls that overlap on one GPU
+ * share that buffer and race: serialise them. */
+int qd_expect_noalloc(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* d_out);
 /* Re(psi^H psi) (Hermitian::norm, sparse_algorithm.hpp:7-16). */
 int qd_norm2(qd_ctx* ctx, int64_t n, const double* d_psi, double* h_out);
 /* Seeded replacement of shuffle_state (utils.hpp:129-140): re, im of entry

@@ -140,6 +140,7 @@ SIGNATURES = {
     "qd_spmv": (I, [VP, VP, VP, VP]),
     "qd_spmv_acc": (I, [VP, VP, DP, VP, VP]),
     "qd_expect": (I, [VP, VP, VP, DP]),
+    "qd_expect_noalloc": (I, [VP, VP, VP, VP]),
     "qd_norm2": (I, [VP, I64, VP, DP]),
     "qd_seeded_state": (I, [VP, I64, ctypes.c_uint64, VP]),
     "qd_krylov_create": (I, [VP, I64, I, I, PP]),
