@@ -38,9 +38,14 @@ struct KronDesc {
   int ellB_w;
   const cplx* vtabB;
   int nvtabB;
+  // T_A compressed for registers (nA < 8192, at most 8 table values, rows of
+  // at most 2 * KRON_WP entries): entry = col << 3 | value id (id 0 pads),
+  // two entries per word, word p of row a at ell16[p * nA + a]
+  const uint32_t* ell16;
   // per product: an extra diagonal (n values, state b nA + a), or nullptr
   const cplx* ext_diag;
 };
+constexpr int KRON_WP = 12;  // words per row of the compressed T_A (24 entries)
 
 // the description of a built Kronecker form (nullptr for nullptr)
 const KronDesc* qd_kron_desc(const qd_kron* k);

@@ -58,6 +58,7 @@ struct qd_kron {
   uint64_t* d_nmask = nullptr;
   cplx* d_ncoef = nullptr;
   uint32_t* d_ell = nullptr;
+  uint32_t* d_ell16 = nullptr;
   cplx* d_vtab = nullptr;
   qd_basis* basisA = nullptr;
   qd_basis* basisB = nullptr;
@@ -323,6 +324,180 @@ __global__ void __launch_bounds__(1024)
         y_st(y + (b0 + r) * k.nA + a, sx[r] + dx.x, sy[r] + dx.y);
       }
     }
+  }
+}
+
+// Persistent rows kernel: each block keeps the compressed T_A entries of its
+// output columns (a = t + 1024 i, i < SLOTS) in registers for all its rows,
+// so T_A is read once per block instead of once per row; the block walks a
+// contiguous range of rows with two LDS row buffers: row r+1 is loaded into
+// registers while row r is computed, then stored to the other buffer.  Same
+// summation order as the packed kernels (entries in order; pad entries,
+// which add 0, are skipped).
+// D(b, a) from the number-operator masks only (GENDIAG = false) or also the
+// general diagonal terms
+template <bool GENDIAG>
+__device__ __forceinline__ cplx diag_of(const KronDesc& k, uint64_t s) {
+  if (GENDIAG) return diag_value(k, s);
+  cplx d = cmake(0.0, 0.0);
+  for (int t = 0; t < k.nnum; ++t) {
+    const uint64_t m = k.nmask[t];
+    if ((s & m) == m) d = cadd(d, k.ncoef[t]);
+  }
+  return d;
+}
+
+// Persistent rows kernel: one block per CU walks a contiguous range of rows
+// with two LDS row buffers and one row in registers -- while row r is
+// computed from one buffer, row r+1 (loaded during row r-1) sits in the
+// other and row r+2 is in flight to registers -- so the HBM latency of the
+// staging hides behind a whole row of gathers.  Thread t computes the
+// outputs a = t + 1024 i (i < S = ceil(nA / 1024)) and keeps the compressed
+// T_A words of all S outputs in registers for every row it visits (T_A is
+// read from memory once per block): slot 0 is the one in use, and after
+// each output the S slots rotate by one, so every register index is a
+// compile-time constant while the output loop stays rolled.  Same
+// summation order as the packed kernels (entries in order; pad entries,
+// which add 0, are skipped).
+template <int WP, int S, bool GENDIAG>
+__global__ void __launch_bounds__(1024)
+    kron_rows3_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift, int rows_per_block) {
+  extern __shared__ cplx lds[];
+  cplx* vt = lds;        // 8 values
+  cplx* nc = lds + 8;    // 64 number-product coefficients
+  uint64_t* nmA = reinterpret_cast<uint64_t*>(lds + 72);  // their block-A masks
+  uint64_t* nmB = nmA + 64;                                // block-B masks
+  cplx* buf0 = lds + 136;  // two row buffers of nA
+  const int nA = (int)k.nA;
+  cplx* buf1 = buf0 + nA;
+  const int64_t b0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t b1 = min<int64_t>(k.nB, b0 + rows_per_block);
+  if (b0 >= b1) return;  // block-uniform
+  const int t = threadIdx.x;
+  const int nnum = GENDIAG ? 0 : k.nnum;  // <= 64 when !GENDIAG
+  const uint64_t amask = k.loB >= 64 ? ~0ull : ((1ull << k.loB) - 1ull);
+  if (t < 8) vt[t] = t < k.nvtab ? k.vtab[t] : cmake(0.0, 0.0);
+  if (t < nnum) {
+    nc[t] = k.ncoef[t];
+    nmA[t] = k.nmask[t] & amask;
+    nmB[t] = k.nmask[t] >> k.loB;
+  }
+  {
+    const cplx* __restrict__ xb = x + b0 * nA;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int c = t + 1024 * i;
+      if (c < nA) buf0[c] = xb[c];
+    }
+  }
+  __syncthreads();
+  // per output: its T_A words, and either its block-A state (general
+  // diagonal) or the set F(a) of number products whose A bits it has
+  uint32_t e[S][WP];
+  uint64_t sa[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int a = t + 1024 * i;
+#pragma unroll
+    for (int p = 0; p < WP; ++p) e[i][p] = a < nA ? k.ell16[p * nA + a] : 0u;
+    const uint64_t st = k.statesA[a < nA ? a : nA - 1];
+    if (GENDIAG) {
+      sa[i] = st;
+    } else {
+      uint64_t f = 0;
+      for (int q = 0; q < nnum; ++q)
+        if ((st & nmA[q]) == nmA[q]) f |= 1ull << q;
+      sa[i] = f;
+    }
+  }
+  // row b0 + 1 in registers (plain vector type, so the array stays in
+  // registers across the loop): stored to LDS at the top of the first iteration
+  kdbl2 pre[S];
+  {
+    const kdbl2* __restrict__ xn = reinterpret_cast<const kdbl2*>(x + min<int64_t>(b0 + 1, b1 - 1) * nA);
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int c = t + 1024 * i;
+      pre[i] = xn[c < nA ? c : nA - 1];
+    }
+  }
+  for (int64_t b = b0; b < b1; ++b) {
+    const int par = (int)((b - b0) & 1);
+    const cplx* cur = par ? buf1 : buf0;
+    cplx* nxt = par ? buf0 : buf1;
+    // row b+1, loaded during the previous iteration, goes to the free buffer
+    // (its last reader, row b-1, finished before the previous barrier); then
+    // row b+2 is requested, to arrive while row b is computed
+    const uint64_t sbb = k.statesB[b];  // before the prefetch: loads complete in order
+    if (b + 1 < b1) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int c = t + 1024 * i;
+        if (c < nA) nxt[c] = cmake(pre[i].x, pre[i].y);
+      }
+    }
+    {
+      const kdbl2* __restrict__ xn = reinterpret_cast<const kdbl2*>(x + min<int64_t>(b + 2, b1 - 1) * nA);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int c = t + 1024 * i;
+        pre[i] = xn[c < nA ? c : nA - 1];
+      }
+    }
+    uint64_t g = 0;  // number products whose B bits row b has
+    if (!GENDIAG) {
+#pragma unroll 8
+      for (int q = 0; q < nnum; ++q)
+        if ((sbb & nmB[q]) == nmB[q]) g |= 1ull << q;
+    }
+    const uint64_t sB = sbb << k.loB;
+#pragma nounroll
+    for (int i = 0; i < S; ++i) {
+      const int a = t + 1024 * i;
+      if (a < nA) {
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int p = 0; p < WP; ++p) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const uint32_t w = (e[0][p] >> (16 * h)) & 0xFFFFu;
+            if (w & 7u) {  // pad words (0) are skipped
+              const cplx v = vt[w & 7u];
+              const cplx xv = cur[w >> 3];
+              sx += v.x * xv.x + v.y * xv.y;  // conj(v) * x
+              sy += v.x * xv.y - v.y * xv.x;
+            }
+          }
+        }
+        cplx d;
+        if (GENDIAG) {
+          d = diag_value(k, sB | sa[0]);
+        } else {
+          // the matching products in mask order: the same sum as diag_value
+          d = cmake(0.0, 0.0);
+          for (uint64_t m = sa[0] & g; m; m &= m - 1) d = cadd(d, nc[__builtin_ctzll(m)]);
+        }
+        d = cadd(d, shift);
+        if (k.ext_diag) d = cadd(d, k.ext_diag[b * nA + a]);
+        const cplx dx = cmul(d, cur[a]);
+        y_st(y + b * nA + a, sx + dx.x, sy + dx.y);
+      }
+      // rotate the slots: the next output's words and state move to slot 0
+#pragma unroll
+      for (int p = 0; p < WP; ++p) {
+        const uint32_t w0 = e[0][p];
+#pragma unroll
+        for (int q = 0; q + 1 < S; ++q) e[q][p] = e[q + 1][p];
+        e[S - 1][p] = w0;
+      }
+      {
+        const uint64_t s0 = sa[0];
+#pragma unroll
+        for (int q = 0; q + 1 < S; ++q) sa[q] = sa[q + 1];
+        sa[S - 1] = s0;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -658,6 +833,22 @@ qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
             d.nvtab = (int)tab.size();
           }
         }
+        // the register-resident form of the persistent rows kernel
+        d.ell16 = nullptr;
+        if (d.ell && d.nA < 8192 && tab.size() <= 8 && w <= 2 * KRON_WP) {
+          std::vector<uint32_t> e16((size_t)KRON_WP * d.nA, 0u);
+          for (int64_t a = 0; a < d.nA; ++a)
+            for (int j = 0; j < w; ++j) {
+              const int32_t e = ao[a] + j;
+              if (e >= ao[a + 1]) break;
+              const uint32_t v = ((uint32_t)ai[e] << 3) | vid[e];
+              e16[(size_t)(j / 2) * d.nA + a] |= v << (16 * (j & 1));
+            }
+          if (hipMalloc(&k->d_ell16, sizeof(uint32_t) * e16.size()) == hipSuccess) {
+            hipMemcpyAsync(k->d_ell16, e16.data(), sizeof(uint32_t) * e16.size(), hipMemcpyHostToDevice, ctx->stream);
+            if (hipStreamSynchronize(ctx->stream) == hipSuccess) d.ell16 = k->d_ell16;
+          }
+        }
       }
     }
   }
@@ -675,6 +866,7 @@ const KronDesc* qd_kron_desc(const qd_kron* k) { return k ? &k->desc : nullptr; 
 void qd_kron_destroy(qd_kron* k) {
   if (!k) return;
   hipFree(k->d_ell);
+  hipFree(k->d_ell16);
   hipFree(k->d_vtab);
   hipFree(k->d_nmask);
   hipFree(k->d_ncoef);
@@ -717,7 +909,7 @@ SlabShape slab_shape(const KronDesc& d, int W, int target = 128) {
 
 // blocks of a 256-thread kernel with `lds` dynamic LDS that one XCD holds at
 // once; the occupancy query runs once per (kernel, LDS size) and process
-int xcd_resident(const void* fn, size_t lds, int ncu) {
+int xcd_resident(const void* fn, size_t lds, int ncu, int threads = 256) {
   static std::mutex mu;
   static std::map<std::pair<const void*, size_t>, int> memo;
   int occ = 0;
@@ -727,7 +919,7 @@ int xcd_resident(const void* fn, size_t lds, int ncu) {
     if (it != memo.end()) occ = it->second;
   }
   if (occ == 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, lds) != hipSuccess || occ < 1) occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds) != hipSuccess || occ < 1) occ = 1;
     std::lock_guard<std::mutex> lk(mu);
     memo[{fn, lds}] = occ;
   }
@@ -752,11 +944,58 @@ int kron_optin(int device) {
   return optin;
 }
 
+// persistent rows kernel: SLOTS = ceil(nA / 1024) <= 5, WP words of
+// compressed entries per output, two row buffers in LDS; grid = the blocks
+// the device holds at once (at most one per row)
+template <int WP, int S, bool GD>
+bool launch_rows3(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const size_t lds = sizeof(cplx) * (136 + 2 * (size_t)d.nA);
+  if (lds > (size_t)optin) return false;
+  const void* fn = (const void*)kron_rows3_kernel<WP, S, GD>;
+  static std::once_flag once;
+  std::call_once(once, [fn, optin]() { hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, optin); });
+  const int per_xcd = xcd_resident(fn, lds, ncu, 1024);
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(d.nB, (int64_t)per_xcd * 8));
+  const int rpb = (int)((d.nB + grid - 1) / grid);
+  const unsigned g = (unsigned)((d.nB + rpb - 1) / rpb);
+  kron_rows3_kernel<WP, S, GD><<<g, 1024, lds, st>>>(d, x, y, shift, rpb);
+  return true;
+}
+
+template <int WP, int S>
+bool launch_rows3_d(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  // the number products go through per-output bit sets when there are at
+  // most 64 and no other diagonal terms
+  return (d.ndterms > 0 || d.nnum > 64) ? launch_rows3<WP, S, true>(d, optin, ncu, x, y, shift, st)
+                                        : launch_rows3<WP, S, false>(d, optin, ncu, x, y, shift, st);
+}
+
+template <int S>
+bool launch_rows3_w(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const int wp = (d.ell_w + 1) / 2;  // words holding the widest row
+  if (wp <= 8) return launch_rows3_d<8, S>(d, optin, ncu, x, y, shift, st);
+  if (wp <= 10) return launch_rows3_d<10, S>(d, optin, ncu, x, y, shift, st);
+  return launch_rows3_d<12, S>(d, optin, ncu, x, y, shift, st);
+}
+
 // rows pass (writes y): variant 0 = kron_rows_packed<2>, 1 = rows2<2,8>,
-// 2 = rows2<1,8>, 3 = rows2<4,4>; a variant whose LDS does not fit returns
-// false.  Without the packed table: kron_rows_kernel.
-bool launch_rows(const KronDesc& d, int v, int optin, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
-  if (v < 0 || v > 3) return false;
+// 2 = rows2<1,8>, 3 = rows2<4,4>, 4 = the persistent rows3 (T_A in
+// registers; needs the compressed table); a variant whose LDS does not fit
+// returns false.  Without the packed table: kron_rows_kernel.
+constexpr int KRON_ROWS_VARIANTS = 5;
+bool launch_rows(const KronDesc& d, int v, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  if (v < 0 || v >= KRON_ROWS_VARIANTS) return false;
+  if (v == 4) {
+    if (!d.ell16) return false;
+    switch ((d.nA + 1023) / 1024) {  // row loads per thread when staging
+      case 1: return launch_rows3_w<1>(d, optin, ncu, x, y, shift, st);
+      case 2: return launch_rows3_w<2>(d, optin, ncu, x, y, shift, st);
+      case 3: return launch_rows3_w<3>(d, optin, ncu, x, y, shift, st);
+      case 4: return launch_rows3_w<4>(d, optin, ncu, x, y, shift, st);
+      case 5: return launch_rows3_w<5>(d, optin, ncu, x, y, shift, st);
+      default: return false;  // more slots than the registers of a 1024-thread block hold
+    }
+  }
   const size_t lds1 = sizeof(cplx) * (size_t)d.nA;
   if (!d.ell) {
     if (v != 0) return false;
@@ -878,9 +1117,9 @@ int kron_tune_locked(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   hipStream_t st = ctx->stream;
   int best_r = 0, best_s = 0;
   float tr = 1e30f, ts = 1e30f;
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < KRON_ROWS_VARIANTS; ++v) {
     bool ok = true;
-    float ms = time_launches(st, [&] { ok = ok && launch_rows(d, v, optin, x, y, cmake(0.0, 0.0), st); }, 3);
+    float ms = time_launches(st, [&] { ok = ok && launch_rows(d, v, optin, ctx->num_cus, x, y, cmake(0.0, 0.0), st); }, 3);
     if (ok && std::getenv("QD_TUNE_LOG")) std::fprintf(stderr, "[qd]   rows variant %d: %.1f us\n", v, 1000.0 * ms / 3);
     if (ok && ms < tr) {
       tr = ms;
@@ -928,7 +1167,7 @@ int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, c
   KronDesc d = k->desc;
   d.ext_diag = ext_diag;
   const int optin = kron_optin(ctx->device);
-  if (!launch_rows(d, rows_variant, optin, x, y, shift, ctx->stream))
+  if (!launch_rows(d, rows_variant, optin, ctx->num_cus, x, y, shift, ctx->stream))
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", rows_variant);
   QD_CHECK_LAUNCH(ctx);
   if (!launch_slab(d, slab_variant, ctx->num_cus, x, y, ctx->stream))

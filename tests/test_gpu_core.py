@@ -464,7 +464,8 @@ def test_matfree_structured_matches_oracle(ctx, case):
 @pytest.mark.parametrize("nup,ndn", [(7, 1), (1, 7)])
 def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
     """Every rows-pass and slab-pass variant of the two-block product, fixed
-    through QD_KRON_VARIANTS, against the oracle: a 14-site chain with one
+    through QD_KRON_VARIANTS, against the oracle (rows variant 4 splits the
+    3432 columns into four groups of blocks): a 14-site chain with one
     block of 3432 states (more than 128 chunks of 8 rows, so the slab
     variants that size chunks to one XCD's resident blocks differ from the
     128-chunk ones) and one of 14."""
@@ -478,7 +479,7 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
     xd = to_dev(ctx, x)
     ran = []
     try:
-        for rv in range(4):
+        for rv in range(5):
             for sv in range(5):
                 monkeypatch.setenv("QD_KRON_VARIANTS", f"{rv},{sv}")
                 mf = LinOp.matfree(ctx, b, p)
@@ -496,6 +497,7 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
     finally:
         _close(p, b)
     assert {s for r, s in ran} == set(range(5)) and (0, 0) in ran
+    assert 4 in {r for r, s in ran}  # the persistent rows kernel (T_A in registers)
 
 
 @pytest.mark.parametrize("method", [0, 1])
