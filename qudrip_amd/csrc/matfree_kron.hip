@@ -558,6 +558,66 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Slab kernel as kron_slab2_kernel, with the read of y -- which misses
+// every cache -- issued before the row's gathers instead of after them, and
+// the next row's y requested while this row is gathered, so neither waits
+// for HBM at the end of a row.  Same summation order.
+template <int W, int U>
+__global__ void __launch_bounds__(256)
+    kron_slab3_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, int chunks,
+                      int rows_per_chunk) {
+  extern __shared__ uint32_t se[];  // rows_per_chunk * ellB_w entries
+  __shared__ cplx vt[256];
+  const int xcd = blockIdx.x & 7;
+  const int kk = blockIdx.x >> 3;
+  const int64_t slab = (int64_t)xcd + 8 * (kk / chunks);
+  const int chunk = kk % chunks;
+  const int64_t a0 = slab * W;
+  const int64_t rb = (int64_t)chunk * rows_per_chunk;
+  const int64_t re = min<int64_t>(k.nB, rb + rows_per_chunk);
+  if (a0 >= k.nA || rb >= re) return;  // uniform over the block
+  const int w = k.ellB_w;
+  for (int i = threadIdx.x; i < k.nvtabB; i += blockDim.x) vt[i] = k.vtabB[i];
+  const int64_t ne = (re - rb) * w;
+  const uint32_t* __restrict__ src = k.ellB + rb * w;
+  for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) se[i] = src[i];
+  __syncthreads();
+  const int lane = threadIdx.x % W;
+  const int sub = threadIdx.x / W;
+  const int nsub = blockDim.x / W;
+  const int64_t a = min<int64_t>(a0 + lane, k.nA - 1);  // clamp: duplicate lanes recompute the last column
+  const bool col_ok = a0 + lane < k.nA;
+  int64_t b = rb + sub;
+  cplx o = b < re ? y_ld(y + b * k.nA + a) : cmake(0.0, 0.0);
+  for (; b < re; b += nsub) {
+    const int64_t bn = b + nsub;
+    const cplx on = bn < re ? y_ld(y + bn * k.nA + a) : cmake(0.0, 0.0);  // the next row's y, in flight
+    const uint32_t* er = se + (b - rb) * w;
+    int len = w;  // pad entries (value 0) close the row
+    while (len > 0 && (er[len - 1] & 0xFFFFu) == 0) --len;
+    double sx = 0.0, sy = 0.0;
+    for (int j = 0; j < len; j += U) {
+      uint32_t e[U];
+      cplx xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) e[u] = (j + u < len) ? er[j + u] : 0u;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < len) xv[u] = x[(int64_t)(e[u] >> 16) * k.nA + a];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j + u < len) {
+          const cplx v = vt[e[u] & 0xFFFFu];
+          sx += v.x * xv[u].x + v.y * xv[u].y;
+          sy += v.x * xv[u].y - v.y * xv[u].x;
+        }
+      }
+    }
+    if (col_ok) y_st(y + b * k.nA + a, o.x + sx, o.y + sy);
+    o = on;
+  }
+}
+
 // ------------------------------------------------------------------ host
 // pack a pull-form block matrix: entries (col << 16 | value id); row-major
 // when rowmajor, else entry-major (ELL column j contiguous over rows)
@@ -1026,10 +1086,10 @@ bool launch_rows(const KronDesc& d, int v, int optin, int ncu, const cplx* x, cp
 }
 
 // slab pass (adds into y): variant 0 = kron_slab_packed<32,4>, 1 =
-// slab2<32,8>, 2 = slab2<32,16>, each at 128 chunks per slab; 3 = slab2<32,8>
-// and 4 = slab2<32,16> with one slab per XCD at a time.  Without the packed
-// table: kron_slab_kernel.
-constexpr int KRON_SLAB_VARIANTS = 5;
+// slab2<32,8>, 2 = slab2<32,16>, each at 128 chunks per slab; 3 = slab2<32,8>,
+// 4 = slab2<32,16>, 5 = slab3<32,8> and 6 = slab3<32,4> with one slab per
+// XCD at a time.  Without the packed table: kron_slab_kernel.
+constexpr int KRON_SLAB_VARIANTS = 7;
 
 bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipStream_t st) {
   if (!d.ellB) {
@@ -1039,8 +1099,10 @@ bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipS
     return true;
   }
   if (v < 0 || v >= KRON_SLAB_VARIANTS) return false;
-  const void* fn = v == 0 ? (const void*)kron_slab_packed_kernel<KRON_SLAB_W, 4>
+  const void* fn = v == 0                ? (const void*)kron_slab_packed_kernel<KRON_SLAB_W, 4>
                    : (v == 1 || v == 3) ? (const void*)kron_slab2_kernel<KRON_SLAB_W, 8>
+                   : v == 5             ? (const void*)kron_slab3_kernel<KRON_SLAB_W, 8>
+                   : v == 6             ? (const void*)kron_slab3_kernel<KRON_SLAB_W, 4>
                                         : (const void*)kron_slab2_kernel<KRON_SLAB_W, 16>;
   SlabShape sh = slab_shape(d, KRON_SLAB_W);
   size_t lds = sizeof(uint32_t) * (size_t)sh.rows_per_chunk * (size_t)d.ellB_w;
@@ -1055,6 +1117,8 @@ bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipS
     case 0: kron_slab_packed_kernel<KRON_SLAB_W, 4><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
     case 1:
     case 3: kron_slab2_kernel<KRON_SLAB_W, 8><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+    case 5: kron_slab3_kernel<KRON_SLAB_W, 8><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
+    case 6: kron_slab3_kernel<KRON_SLAB_W, 4><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
     default: kron_slab2_kernel<KRON_SLAB_W, 16><<<sh.grid, 256, lds, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk); break;
   }
   return true;

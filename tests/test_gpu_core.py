@@ -480,7 +480,7 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
     ran = []
     try:
         for rv in range(5):
-            for sv in range(5):
+            for sv in range(7):
                 monkeypatch.setenv("QD_KRON_VARIANTS", f"{rv},{sv}")
                 mf = LinOp.matfree(ctx, b, p)
                 try:
@@ -496,7 +496,7 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
                 ran.append((rv, sv))
     finally:
         _close(p, b)
-    assert {s for r, s in ran} == set(range(5)) and (0, 0) in ran
+    assert {s for r, s in ran} == set(range(7)) and (0, 0) in ran
     assert 4 in {r for r, s in ran}  # the persistent rows kernel (T_A in registers)
 
 
