@@ -279,7 +279,7 @@ int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double* h_out);
  * matrix operators (qd_linop_from_csr) of at most 33,554,432 rows
  * (QD_E_CAPACITY beyond; QD_E_UNSUPPORTED for other operators).  The
  * product is fused with the dot and never stored; block sums use a fixed
- * 2 MiB device buffer per GPU held until the process ends.  d_out: device
+ * 4 MiB device buffer per GPU held until the process ends.  d_out: device
  * double[2], written asynchronously on the context's stream (no memset
  * node: the call may be captured in a graph).  Calls that overlap on one GPU
  * share that buffer and race: serialise them. */

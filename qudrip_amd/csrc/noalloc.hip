@@ -4,10 +4,10 @@
 // (code under HIP graph capture, tight serving loops, harnesses that forbid
 // hipMalloc).  qd_expect_noalloc computes <psi|H|psi> of a stored-matrix
 // operator with the CSR-stream product fused with the dot (the product
-// vector is never stored): each block of 256 rows writes its sum to a
-// file-scope device buffer of fixed size, then one block adds the sums in a
-// fixed order and writes the result.  Operators of at most 2^17 * 256 =
-// 33,554,432 rows are accepted.  Every call on one GPU shares that buffer:
+// vector is never stored): each block of 128 rows writes its sum to a
+// file-scope device buffer of fixed size (4 MiB), then one block adds the
+// sums in a fixed order and writes the result.  Operators of at most
+// 2^18 * 128 = 33,554,432 rows are accepted.  Every call on one GPU shares that buffer:
 // calls that overlap on one GPU (two streams, two threads) race and must be
 // serialised by the caller.  Both kernels run asynchronously on the
 // context's stream and write no memset node, so the call may be captured.
@@ -17,7 +17,7 @@
 #include "wave.hpp"
 
 namespace {
-constexpr int64_t kNoallocBlocks = (int64_t)1 << 17;  // blocks of 256 rows: n <= 2^17 * 256
+constexpr int64_t kNoallocBlocks = (int64_t)1 << 18;  // one partial per block of the fused product
 __device__ __align__(256) cplx scratch[kNoallocBlocks];
 
 // out = sum part[0 .. nb) in a fixed order (one block)
@@ -70,7 +70,7 @@ extern "C" int qd_expect_noalloc(qd_ctx* ctx, qd_linop* op, const double* d_psi,
   const int64_t nb = qd_linop_dot_blocks(n);
   if (nb > kNoallocBlocks)
     return qd_set_error(ctx, QD_E_CAPACITY, "qd_expect_noalloc: dimension %lld exceeds the scratch limit of %lld rows",
-                        (long long)n, (long long)(kNoallocBlocks * 256));
+                        (long long)n, (long long)(kNoallocBlocks * qd_linop_dot_rows_per_block()));
   cplx* part = nullptr;
   int rc = noalloc_scratch(ctx, &part);
   if (rc) return rc;

@@ -394,6 +394,7 @@ int qd_krylov_gs_step(qd_krylov* kr, qd_linop* op, double* d_psi, int add_dim, d
 // qd_linop_dot_blocks(n) blocks; QD_E_UNSUPPORTED (nothing launched) for
 // other operators.  write = false skips the store of y.
 int64_t qd_linop_dot_blocks(int64_t n);
+int qd_linop_dot_rows_per_block();  // rows summed into one partial
 int qd_linop_apply_dot(qd_linop* op, const cplx* x, cplx* y, cplx* part, bool write);
 
 // grid helpers

@@ -267,24 +267,33 @@ __global__ void __launch_bounds__(256)
 // x gathers of slice s, so each wave keeps two memory phases in flight
 // instead of alternating index load -> gather -> products.  Indices are
 // clamped into the slice, so every load stays inside [wbeg, wend).
+//
+// Launch shape: STREAM3_WPB waves (64 rows each) per block, blocks in row
+// order with no XCD remap.  Row ranges differ in their entry counts (XXZ
+// N=24: the eight contiguous eighths hold 4.1 M to 4.8 M entries), so giving
+// each XCD one contiguous eighth left the XCDs finishing up to 24 us apart
+// in a 170 us launch; dealt round-robin, blocks balance the entries over the
+// XCDs and the XCDs finish within 4 us.  Small blocks free their slot as
+// soon as their own waves end: a 4-wave block held its slot until the
+// slowest of its waves finished, which left about a fifth of the wave slots
+// idle.  Together: 171 -> 155 us at XXZ N=24, same bits (per-wave start/end
+// stamps and interleaved timings on one GPU).
 // DOT: the block also sums conj(x_r) y_r over its rows into
-// dot_part[blockIdx.x] (a fixed tree: wavefront butterflies, then the four
-// waves in order); the caller sums the partials in a later kernel.  WRITE =
-// false skips the store of y (an expectation value <x|H|x> without the
-// product vector).
+// dot_part[blockIdx.x] (a fixed tree: wavefront butterflies, then the
+// block's waves in order); the caller sums the partials in a later kernel.
+// WRITE = false skips the store of y (an expectation value <x|H|x> without
+// the product vector).
+constexpr int STREAM3_WPB = 2;
 template <int U, bool DOT = false, bool WRITE = true>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * STREAM3_WPB)
     spmv_stream3_kernel(int64_t nrows, const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
                         const cplx* __restrict__ vals, const cplx* __restrict__ x, cplx* __restrict__ y,
                         const double* __restrict__ xscale, int xscale_idx, const cplx* __restrict__ diag,
                         cplx shift, cplx* __restrict__ dot_part = nullptr) {
   constexpr int CAPW = 64 * U;
-  __shared__ cplx lds[4][CAPW];
-  const unsigned nb = gridDim.x;
-  const unsigned per = nb >> 3;
-  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  __shared__ cplx lds[STREAM3_WPB][CAPW];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t r0 = ((int64_t)lb * 4 + w) * 64;
+  const int64_t r0 = ((int64_t)blockIdx.x * STREAM3_WPB + w) * 64;
   if (!DOT && r0 >= nrows) return;  // whole wave leaves together
   cplx dsum = cmake(0.0, 0.0);
   if (r0 < nrows) {
@@ -377,11 +386,16 @@ __global__ void __launch_bounds__(256)
   }
   }
   if (DOT) {  // every wave of the block reaches this barrier
-    __shared__ cplx wsum[4];
+    __shared__ cplx wsum[STREAM3_WPB];
     dsum = wave_sum(dsum);
     if (lane == 0) wsum[w] = dsum;
     __syncthreads();
-    if (threadIdx.x == 0) dot_part[blockIdx.x] = cadd(cadd(wsum[0], wsum[1]), cadd(wsum[2], wsum[3]));
+    if (threadIdx.x == 0) {
+      cplx t = wsum[0];
+#pragma unroll
+      for (int q = 1; q < STREAM3_WPB; ++q) t = cadd(t, wsum[q]);
+      dot_part[blockIdx.x] = t;
+    }
   }
 }
 
@@ -436,6 +450,10 @@ static unsigned grid8(int64_t blocks) {
   int64_t g = (blocks + 7) / 8 * 8;
   return (unsigned)(g < 8 ? 8 : g);
 }
+static unsigned stream3_grid(int64_t n) {
+  const int64_t rows = 64 * STREAM3_WPB;
+  return (unsigned)std::max<int64_t>(1, (n + rows - 1) / rows);
+}
 
 // y = s * (H + diag + shift) x, s = xscale[idx] (device) or 1
 int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale, int xscale_idx,
@@ -448,7 +466,7 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
     d_diag = op->diag0;
   }
   if (op->kind == QD_LINOP_CSR && op->lanes_per_row <= 0) {
-    // CSR-stream: 4 waves x 64 rows per block; variant = -lanes_per_row
+    // CSR-stream: 64 rows per wave; variant = -lanes_per_row
     unsigned grid = grid8((n + 255) / 256);
     switch (-op->lanes_per_row) {
 #define QD_STREAM_CASE(V, CAPW, U, NT)                                                               \
@@ -478,8 +496,8 @@ int qd_linop_apply(qd_linop* op, const cplx* x, cplx* y, const double* d_xscale,
 #undef QD_STREAM2_CASE
 #define QD_STREAM3_CASE(V, U)                                                                          \
   case V:                                                                                           \
-    spmv_stream3_kernel<U><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y,      \
-                                                          d_xscale, xscale_idx, d_diag, shift);         \
+    spmv_stream3_kernel<U><<<stream3_grid(n), 64 * STREAM3_WPB, 0, ctx->stream>>>(                    \
+        n, op->outer, op->inner, op->vals, x, y, d_xscale, xscale_idx, d_diag, shift);                 \
     break;
       QD_STREAM3_CASE(13, 8)
       QD_STREAM3_CASE(14, 4)
@@ -748,7 +766,8 @@ extern "C" int qd_expect(qd_ctx* ctx, qd_linop* op, const double* d_psi, double*
 }
 
 // ------------------------------------------------- product fused with a dot
-int64_t qd_linop_dot_blocks(int64_t n) { return grid8((n + 255) / 256); }
+int64_t qd_linop_dot_blocks(int64_t n) { return stream3_grid(n); }
+int qd_linop_dot_rows_per_block() { return 64 * STREAM3_WPB; }
 
 // Stored-matrix operators: y = (H + diag + shift) x in the CSR-stream kernel,
 // whose blocks also write conj(x_r) y_r summed over their rows to
@@ -761,10 +780,10 @@ int qd_linop_apply_dot(qd_linop* op, const cplx* x, cplx* y, cplx* part, bool wr
   const unsigned grid = (unsigned)qd_linop_dot_blocks(n);
   const cplx* diag = op->diag0;
   if (write)
-    spmv_stream3_kernel<8, true, true><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y, nullptr,
+    spmv_stream3_kernel<8, true, true><<<grid, 64 * STREAM3_WPB, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y, nullptr,
                                                                       0, diag, cmake(0.0, 0.0), part);
   else
-    spmv_stream3_kernel<8, true, false><<<grid, 256, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y,
+    spmv_stream3_kernel<8, true, false><<<grid, 64 * STREAM3_WPB, 0, ctx->stream>>>(n, op->outer, op->inner, op->vals, x, y,
                                                                        nullptr, 0, diag, cmake(0.0, 0.0), part);
   QD_CHECK_LAUNCH(ctx);
   return QD_OK;
