@@ -42,7 +42,8 @@ enum {
   QD_E_CAPACITY = 6,    /* a row exceeds the kernel's per-row capacity        */
   QD_E_UNSUPPORTED = 7, /* sector or operator shape not supported             */
   QD_E_BUDGET = 8,      /* request does not fit the HBM budget                */
-  QD_E_STATE = 9        /* call not valid in the handle's current state       */
+  QD_E_STATE = 9,       /* call not valid in the handle's current state       */
+  QD_E_CANCELLED = 10   /* served request stopped by qd_server_cancel         */
 };
 
 /* ------------------------------------------------------------------ handles */
@@ -494,6 +495,18 @@ int qd_server_set_batch(qd_server* s, int slots);
  * reorthogonalisation, 1 = full delayed CGS2 (the scheme of
  * qd_krylov_ground_state), 0 = CGS2.  No reference counterpart. */
 int qd_server_set_gs_method(qd_server* s, int method);
+/* Cancels every request with this tag: a queued one is answered at once
+ * (status QD_E_CANCELLED, worker -1); a running one stops at its next
+ * cancellation point and is answered with QD_E_CANCELLED (iterations = time
+ * steps done).  The cancellation points are the reference's own loop
+ * boundaries: between getGroundState extensions of a ground state with a
+ * tolerance (sparse_algorithm.hpp:564-585, one per chunk), between evolve
+ * calls of a real-time request (:548-562, one per time step), and between
+ * the lockstep steps of a batch; a ground state with tol = 0 is one call and
+ * is reported cancelled when it returns.  QD_E_ARG if no queued or running
+ * request has the tag.  No reference counterpart (the reference runs one
+ * solve per process). */
+int qd_server_cancel(qd_server* s, int64_t tag);
 /* Moves up to cap finished results (in completion order) into out. */
 int qd_server_poll(qd_server* s, qd_result* out, int cap, int* count);
 /* Blocks until every submitted request has finished. */

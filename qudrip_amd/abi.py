@@ -15,7 +15,9 @@ QD_OK = 0
 STATUS = {
     0: "QD_OK", 1: "QD_E_ARG", 2: "QD_E_OOM", 3: "QD_E_HIP", 4: "QD_E_BREAKDOWN",
     5: "QD_E_EIGEN", 6: "QD_E_CAPACITY", 7: "QD_E_UNSUPPORTED", 8: "QD_E_BUDGET", 9: "QD_E_STATE",
+    10: "QD_E_CANCELLED",
 }
+STATUS_CODE = {v: k for k, v in STATUS.items()}
 QD_OP_BIT = 0
 QD_OP_HOP = 1
 QD_OP_MODE = 2
@@ -172,6 +174,7 @@ SIGNATURES = {
     "qd_server_set_batch": (I, [VP, I]),
     "qd_server_set_gs_method": (I, [VP, I]),
     "qd_stream_bandwidth": (I, [VP, ctypes.c_size_t, I, DP, DP]),
+    "qd_server_cancel": (I, [VP, ctypes.c_int64]),
     "qd_server_poll": (I, [VP, ctypes.POINTER(qd_result), I, ctypes.POINTER(I)]),
     "qd_server_drain": (I, [VP]),
     "qd_server_clock": (ctypes.c_double, [VP]),

@@ -17,12 +17,14 @@
 // with the same reduction order whatever else is running, so its result does
 // not depend on the mix.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "mrhs_kron.hpp"
@@ -150,6 +152,11 @@ struct qd_server {
   std::condition_variable cv;
   std::deque<Job> queue;
   std::vector<qd_result> results;
+  // cancellation: running requests per tag, and the tags asked to stop
+  // (ncancel lets a worker skip the lock while nothing is cancelled)
+  std::unordered_map<int64_t, int> active;
+  std::unordered_map<int64_t, int> cancel;
+  std::atomic<int> ncancel{0};
   int64_t submitted = 0, finished = 0, rejected = 0;
   int running = 0, max_running = 0;
   size_t peak_used = 0;
@@ -165,6 +172,25 @@ struct qd_server {
 };
 
 namespace {
+
+// A request with this tag was asked to stop (qd_server_cancel).
+bool stop_requested(qd_server* S, int64_t tag) {
+  if (S->ncancel.load(std::memory_order_acquire) == 0) return false;
+  std::lock_guard<std::mutex> lk(S->mu);
+  return S->cancel.count(tag) != 0;
+}
+
+// Bookkeeping of a request that starts / ends running.  Lock held.
+void mark_running(qd_server* S, int64_t tag) { ++S->active[tag]; }
+void mark_ended(qd_server* S, int64_t tag) {
+  auto it = S->active.find(tag);
+  if (it != S->active.end() && --it->second == 0) {
+    S->active.erase(it);
+    if (S->cancel.erase(tag)) S->ncancel.fetch_sub(1, std::memory_order_release);
+  }
+}
+
+int cancelled(qd_ctx* ctx) { return qd_set_error(ctx, QD_E_CANCELLED, "request cancelled"); }
 
 // Device contexts (one stream each) of servers live as long as the process
 // and pass from a destroyed server to the next one: a process has a fixed
@@ -222,7 +248,7 @@ int diag_expect(qd_ctx* ctx, int64_t n, const cplx* d, const cplx* psi, double* 
   return QD_OK;
 }
 
-int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int method, qd_result* out) {
+int run_ground_state(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request& r, int method, qd_result* out) {
   const int64_t n = M.n;
   cplx *diag = nullptr, *psi = nullptr;
   qd_linop* op = nullptr;
@@ -236,6 +262,7 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int metho
     qd_dev_free(ctx, psi);
     return code;
   };
+  if (stop_requested(S, r.tag)) return done(cancelled(ctx));
   if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&diag) != QD_OK ||
       qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&psi) != QD_OK)
     return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
@@ -255,6 +282,7 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int metho
     for (;;) {
       if ((rc = qd_krylov_gs_step(kr, op, (double*)psi, chunk, &E, false))) return done(rc);
       qd_krylov_get_info(kr, &info);
+      if (stop_requested(S, r.tag)) return done(cancelled(ctx));  // between chunks
       // Ritz-residual bound of the eigenvalue error, r^2 / gap (Kato-Temple
       // with the second Ritz value for the gap), or the move of E0
       const double bound = info.ritz_gap > 0.0 ? info.ritz_resid * info.ritz_resid / info.ritz_gap : 1e300;
@@ -268,6 +296,7 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int metho
   } else {
     if ((rc = qd_krylov_ground_state(kr, op, (double*)psi, -1, &E))) return done(rc);
     qd_krylov_get_info(kr, &info);
+    if (stop_requested(S, r.tag)) return done(cancelled(ctx));  // one call: stopped after it
   }
   double dexp = 0.0, nrm2 = 0.0;
   if ((rc = diag_expect(ctx, n, M.d_diagD, psi, &dexp, &nrm2))) return done(rc);
@@ -279,7 +308,7 @@ int run_ground_state(qd_ctx* ctx, const Model& M, const qd_request& r, int metho
   return done(QD_OK);
 }
 
-int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
+int run_evolve(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
   const int64_t n = M.n;
   cplx *diag = nullptr, *psi = nullptr, *nxt = nullptr;
   qd_linop* op = nullptr;
@@ -294,6 +323,7 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
     qd_dev_free(ctx, nxt);
     return code;
   };
+  if (stop_requested(S, r.tag)) return done(cancelled(ctx));
   if (qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&psi) != QD_OK ||
       qd_dev_alloc(ctx, sizeof(cplx) * n, (void**)&nxt) != QD_OK)
     return done(qd_set_error(ctx, QD_E_OOM, "request vectors"));
@@ -311,6 +341,10 @@ int run_evolve(qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out)
   int64_t msum = 0;
   int m_start = r.chunk > 0 ? r.chunk : 12;
   for (int s = 0; s < r.steps; ++s) {
+    if (stop_requested(S, r.tag)) {  // between time steps
+      out->iterations = s;
+      return done(cancelled(ctx));
+    }
     if (r.tol > 0.0) {
       int mu = 0, me = 0;
       rc = qd_krylov_evolve_tol(kr, op, (const double*)psi, (double*)nxt, r.dt, m_start, r.k, r.tol, &mu, &me);
@@ -400,6 +434,7 @@ struct BatchSlot {
 
 void finish_result(qd_server* S, const qd_result& res) {
   std::lock_guard<std::mutex> lk(S->mu);
+  mark_ended(S, res.tag);
   --S->running;
   S->results.push_back(res);
   ++S->finished;
@@ -504,6 +539,7 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
         if (it->req.kind == QD_REQ_EVOLVE && it->req.model == model && it->req.k == first.req.k &&
             it->req.dt == dt) {
           pending.push_back(*it);
+          mark_running(S, it->req.tag);
           it = S->queue.erase(it);
           ++S->running;
           S->max_running = std::max(S->max_running, S->running);
@@ -530,9 +566,27 @@ void run_evolve_batch(qd_server* S, qd_ctx* ctx, const Job& first, int wi) {
       slots.push_back(sl);
     }
     pending.clear();
-    // requests with nothing (left) to do
+    // requests with nothing (left) to do, or asked to stop
     for (int t = (int)slots.size() - 1; rc == QD_OK && t >= 0; --t) {
       BatchSlot& sl = slots[t];
+      if (stop_requested(S, sl.job.req.tag)) {
+        qd_result res{};
+        res.tag = sl.job.req.tag;
+        res.status = QD_E_CANCELLED;
+        res.t_submit = sl.job.t_submit;
+        res.t_start = sl.t_start;
+        res.t_end = S->now();
+        res.worker = wi;
+        res.iterations = sl.steps_done;
+        finish_result(S, res);
+        const int last = (int)slots.size() - 1;
+        if (t != last) {
+          col_copy_kernel<<<blocks_for(n, 256), 256, 0, st>>>(n, C, last, t, X);
+          slots[t] = slots[last];
+        }
+        slots.pop_back();
+        continue;
+      }
       if (sl.steps_done < sl.job.req.steps) continue;
       qd_result res{};
       res.tag = sl.job.req.tag;
@@ -684,6 +738,7 @@ void worker_main(qd_server* S, int wi) {
       }
       ++S->running;
       S->max_running = std::max(S->max_running, S->running);
+      mark_running(S, job.req.tag);
     }
     if (job.req.kind == QD_REQ_EVOLVE && S->batch_slots > 1) {
       run_evolve_batch(S, ctx, job, wi);
@@ -695,11 +750,12 @@ void worker_main(qd_server* S, int wi) {
     res.t_start = S->now();
     res.worker = wi;
     const Model& M = *job.model;
-    res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(ctx, M, job.req, job.gs_method, &res)
-                                                     : run_evolve(ctx, M, job.req, &res);
+    res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(S, ctx, M, job.req, job.gs_method, &res)
+                                                     : run_evolve(S, ctx, M, job.req, &res);
     res.t_end = S->now();
     {
       std::lock_guard<std::mutex> lk(S->mu);
+      mark_ended(S, job.req.tag);
       S->used -= job.footprint;
       --S->running;
       S->results.push_back(res);
@@ -846,6 +902,38 @@ extern "C" int qd_server_submit(qd_server* S, const qd_request* r) {
   S->queue.push_back(j);
   ++S->submitted;
   S->cv.notify_all();
+  return QD_OK;
+}
+
+extern "C" int qd_server_cancel(qd_server* S, int64_t tag) {
+  if (!S) return QD_E_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  bool found = false;
+  for (auto it = S->queue.begin(); it != S->queue.end();) {
+    if (it->req.tag != tag) {
+      ++it;
+      continue;
+    }
+    qd_result res{};
+    res.tag = tag;
+    res.status = QD_E_CANCELLED;
+    res.t_submit = it->t_submit;
+    res.t_start = res.t_end = S->now();
+    res.worker = -1;
+    S->results.push_back(res);
+    ++S->finished;
+    it = S->queue.erase(it);
+    found = true;
+  }
+  if (S->active.count(tag)) {
+    if (S->cancel.emplace(tag, 1).second) S->ncancel.fetch_add(1, std::memory_order_release);
+    found = true;
+  }
+  if (found) S->cv.notify_all();
+  if (!found) {
+    S->err = "qd_server_cancel: no queued or running request with this tag";
+    return QD_E_ARG;
+  }
   return QD_OK;
 }
 

@@ -55,6 +55,15 @@ class Server:
                            seed=seed, tag=tag)
         self._check(self.lib.qd_server_submit(self.ptr, ctypes.byref(r)), "submit")
 
+    def cancel(self, tag):
+        """Cancels the queued or running requests with this tag (they finish
+        with status QD_E_CANCELLED); False if none is queued or running."""
+        rc = self.lib.qd_server_cancel(self.ptr, tag)
+        if rc == abi.STATUS_CODE["QD_E_ARG"]:
+            return False
+        self._check(rc, "cancel")
+        return True
+
     def poll(self, cap=1024):
         buf = (abi.qd_result * cap)()
         n = ctypes.c_int()
@@ -112,9 +121,13 @@ def hubbard_lattice(Lx=3, Ly=4, nup=6, ndn=6):
 
 
 def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=4, gs_tol=1e-12,
-               rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, gs_method=2, log=None):
+               rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, gs_method=2, log=None,
+               closed_loop=False):
     """Registers the models (timed apart: the structure cache is built once at
-    start-up), then submits every request at its arrival time and waits."""
+    start-up), then submits every request at its arrival time and waits.
+    closed_loop: the requests go one at a time instead (each submitted when
+    the previous one has finished, arrival times ignored): the server's
+    algorithms and cached structure with no overlap between requests."""
     nsite, bonds, nup, ndn = lattice or hubbard_lattice()
     srv = Server(hbm_budget=hbm_budget, workers=workers, batch=batch, gs_method=gs_method)
     try:
@@ -131,8 +144,10 @@ def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs
             log(f"served: {len(models)} models registered in {setup_s:.2f} s")
         c0 = srv.clock()
         for r in sorted(reqs, key=lambda r: r["arrival"]):
+            if closed_loop:
+                srv.drain()
             wait = c0 + r["arrival"] - srv.clock()
-            if wait > 0:
+            if wait > 0 and not closed_loop:
                 time.sleep(wait)
             if r["kind"] == "gs":
                 srv.submit(abi.QD_REQ_GROUND_STATE, models[("gs", r["N"])], r["coeff"], gs_k, chunk=gs_chunk,
@@ -147,7 +162,7 @@ def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs
         srv.close()
     for r in reqs:
         x = res[r["tag"]]
-        x["arrival"] = c0 + r["arrival"]
+        x["arrival"] = x["t_submit"] if closed_loop else c0 + r["arrival"]
         x["latency"] = x["t_end"] - x["arrival"]
     span = max(x["t_end"] for x in res.values()) - min(x["arrival"] for x in res.values())
     return {"results": res, "stats": stats, "setup_s": setup_s, "span_s": span,
@@ -216,5 +231,12 @@ def run_plain(reqs, ctx, lattice=None, gs_k=150, rt_k=30, rt_steps=100, dt=0.05,
 
 
 def latency_percentiles(results):
+    """p50 / p99 of arrival -> result (queueing included)."""
     lat = np.array([x["latency"] for x in results.values()])
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def service_percentiles(results):
+    """p50 / p99 of start -> result on a worker (the solve alone)."""
+    t = np.array([x["t_end"] - x["t_start"] if "t_start" in x else x["time_s"] for x in results.values()])
+    return float(np.percentile(t, 50)), float(np.percentile(t, 99))

@@ -140,3 +140,56 @@ def test_served_config5_sizes_match_plain_and_oracle(ctx):
     csc = oracle.build_csc(basis, ham)
     e_orc = oracle.lanczos_gs(csc, 150, oracle.seeded_state(len(basis), 41))["E0"]
     assert abs(served["results"][0]["energy"] - e_orc) <= 1e-10 * abs(e_orc), (served["results"][0]["energy"], e_orc)
+
+
+def _wait(pred, limit_s=60.0):
+    import time
+    t0 = time.perf_counter()
+    while not pred():
+        assert time.perf_counter() - t0 < limit_s, "server did not reach the expected state"
+        time.sleep(0.002)
+
+
+@pytest.mark.parametrize("batch", [1, 4])
+def test_cancel_queued_and_running(ctx, batch):
+    """qd_server_cancel: a queued request is answered at once, a running one
+    stops at its next time step with QD_E_CANCELLED, an unknown tag is
+    refused, and the server keeps serving afterwards."""
+    idx, A, D = Q.xxz_parts(12, 1.0, pbc=True)
+    sec = Q.sz_sector(idx, 12, 6)
+    srv = SV.Server(hbm_budget=4 << 30, workers=1, batch=batch)
+    done = {}
+
+    def collect():
+        for x in srv.poll():
+            done[x["tag"]] = x
+
+    try:
+        m = srv.add_model(sec, A, D)
+        print("cancel test, batch", batch, flush=True)
+        long_steps = 50000
+        srv.submit(abi.QD_REQ_EVOLVE, m, 0.7, 16, steps=long_steps, dt=0.02, seed=3, tag=1)
+        if batch > 1:  # a short request in the same batch finishes normally
+            srv.submit(abi.QD_REQ_EVOLVE, m, 1.3, 16, steps=20, dt=0.02, seed=4, tag=5)
+        _wait(lambda: srv.stats()["running"] >= 1)
+        srv.submit(abi.QD_REQ_GROUND_STATE, m, 0.5, 60, chunk=10, tol=1e-12, seed=7, tag=2)  # queued behind
+        if batch > 1:
+            _wait(lambda: (collect(), 5 in done)[1])
+            assert done[5]["status"] == 0 and done[5]["iterations"] == 20
+        assert srv.cancel(2)
+        collect()
+        assert done[2]["status"] == abi.STATUS_CODE["QD_E_CANCELLED"] and done[2]["worker"] == -1
+        assert srv.cancel(1)
+        _wait(lambda: (collect(), 1 in done)[1])
+        assert done[1]["status"] == abi.STATUS_CODE["QD_E_CANCELLED"]
+        assert 0 < done[1]["iterations"] < long_steps
+        assert not srv.cancel(1) and not srv.cancel(12345)
+        srv.submit(abi.QD_REQ_GROUND_STATE, m, 0.9, 60, chunk=10, tol=1e-12, seed=7, tag=3)
+        srv.drain()
+        collect()
+        st = srv.stats()
+    finally:
+        srv.close()
+    e = _dense_e0(12, 0.9)
+    assert done[3]["status"] == 0 and abs(done[3]["energy"] - e) <= 1e-10 * abs(e)
+    assert st["running"] == 0 and st["queued"] == 0
