@@ -9,14 +9,21 @@ complex128 values, x read once, y written once):
     B = 20 nnz + 4 (n + 1) + 32 n = 828,647,460 bytes.
 value = B * steps / wall time of the timed steps (GB/s, 1e9 bytes).
 roofline.achieved = B / average kernel duration from HIP events on the
-stream the kernel runs on.  The line also carries the full Lanczos ground
+stream the kernel runs on; roofline.traffic = HBM bytes per launch from the
+committed rocprofv3 counter summary of the same kernel source (null if the
+source changed since).  The line also carries the full Lanczos ground
 state (k = 150, full reorthogonalisation) against the Bethe-ansatz energy,
-and the oracle's single-thread CPU product timed on a bounded sample.
+and the oracle's single-thread CPU product timed on a bounded sample (with
+the host's CPU model and thread count), plus the other CPU rows
+(cpu_baseline_more: XXZ N=24 ground state, Hubbard matrix-free product, on
+bounded samples with labelled projections; --cpu-full times them in full).
 Further sections (each can be skipped): the Hubbard 4x4 quench (ground state
 then real-time steps with the matrix-free Kronecker product), 4096 batched
 quantum-jump trajectories of the dissipative Hubbard chain L=8 against the
-one-at-a-time loop and the oracle, the served request stream (partial and
-full reorthogonalisation) against the plain one-at-a-time path, and extras:
+one-at-a-time loop and the oracle (the reference's loop, operators rebuilt
+every step, and a precomputed-operator variant), the served request stream
+(four workers, one worker, one request at a time with the same rules, full
+reorthogonalisation) against the plain one-at-a-time path, and extras:
 a basis filtered by double occupancy and a qubits-times-mode matrix (build
 and product).  The Lanczos section also runs the reference's two-vector
 solver (Hermitian::getSolver) on the same operator and start, and the quench
@@ -33,12 +40,49 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 E_BETHE_XXX24 = -10.67001451653724  # Bethe ansatz, N=24 PBC (S=1/2 XXX)
-# HBM bytes per launch of the default product kernel on this workload, from
-# two rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from
-# a plain 1 GiB copy); commands and raw counts in
-# profiles/spmv_xxz24_hbm_traffic.md.
-SPMV_XXZ24_TRAFFIC_BYTES = 991.8e6
 SPMV_DEFAULT_MAPPING = -13  # the row mapping the library picks for this matrix (qd_linop_get_lanes)
+TRAFFIC_PROFILE = os.path.join("profiles", "spmv_xxz24_traffic.txt")
+
+
+def spmv_traffic():
+    """HBM bytes per launch of the default product kernel at XXZ N=24 from
+    the committed counter summary (two rocprofv3 --pmc runs of this script:
+    FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a plain 1 GiB copy).
+    The summary's record line names the sha256 of the kernel source it was
+    measured on; the figure is used only while spmv.hip is that source, else
+    the field is null.  Returns (bytes or None, note)."""
+    import hashlib
+    src = os.path.join(ROOT, "qudrip_amd", "csrc", "spmv.hip")
+    try:
+        sha = hashlib.sha256(open(src, "rb").read()).hexdigest()
+        rec = [ln for ln in open(os.path.join(ROOT, TRAFFIC_PROFILE)) if ln.startswith("traffic-record:")]
+    except OSError as e:
+        return None, f"no counter summary ({e.strerror})"
+    if not rec:
+        return None, f"{TRAFFIC_PROFILE} has no traffic-record line"
+    kv = dict(f.split("=", 1) for f in rec[-1].split()[1:])
+    if kv.get("spmv.hip-sha256") != sha:
+        return None, f"{TRAFFIC_PROFILE} was measured on another spmv.hip (sha256 {kv.get('spmv.hip-sha256', '?')[:12]})"
+    return float(kv["bytes_per_launch"]), (f"{TRAFFIC_PROFILE}: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of "
+                                           f"{kv.get('kernel', '?')}, same spmv.hip (sha256 {sha[:12]})")
+
+
+def host_cpu():
+    """CPU model (lscpu's 'Model name', read from /proc/cpuinfo), nproc and
+    the threads this process may use."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.lower().startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_threads": usable}
 
 
 def main():
@@ -59,7 +103,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the Poisson stream")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--traj", type=int, default=4096, help="number of trajectories")
-    ap.add_argument("--traj-cpu", type=int, default=8, help="trajectories in the oracle's CPU sample")
+    ap.add_argument("--traj-cpu", type=int, default=64,
+                    help="trajectories in the oracle's CPU sample (a quarter of them in the reference-loop sample)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--lanes", type=int, default=None, help="SpMV row mapping override (<= 0: CSR-stream variant)")
     ap.add_argument("--serve-profile", action="store_true",
@@ -68,6 +113,12 @@ def main():
                     help="measure the streaming read / copy bandwidth over this many MiB and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--kron-products", type=int, default=0,
+                    help="only apply the Hubbard 4x4 two-block matrix-free product this many times and exit "
+                         "(a run to collect the product's counters)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the CPU restatement on the full-size workloads: the XXZ N=24 Lanczos "
+                         "ground state (k = 150) and one Hubbard 4x4 matrix-free product (minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for {args.gpus}); "
@@ -97,6 +148,10 @@ def main():
         print(json.dumps({"served_requests": len(reqs), "workers": args.serve_workers,
                           "solves_per_s": round(out["solves_per_s"], 3), "span_s": round(out["span_s"], 3),
                           "failed": sum(1 for x in out["results"].values() if x["status"] != 0)}), flush=True)
+        return
+
+    if args.kron_products:
+        kron_products(args.kron_products)
         return
 
     if args.calib_copy_mb:
@@ -168,10 +223,13 @@ def main():
                    "n": n, "nnz": nnz, "bytes_per_spmv": B,
                    "spmv_row_mapping": op.lanes()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": SPMV_XXZ24_TRAFFIC_BYTES if (N == 24 and op.lanes() == SPMV_DEFAULT_MAPPING) else None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "csr_build_ms": round(build_s * 1e3, 2),
     }
+    if N == 24 and op.lanes() == SPMV_DEFAULT_MAPPING:
+        traffic, note = spmv_traffic()
+        result["roofline"]["traffic"] = traffic
+        result["roofline_traffic_source"] = note
 
     if not args.no_lanczos:
         kr = Krylov(ctx, n, args.krylov, herm=True)
@@ -249,8 +307,45 @@ def main():
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(csr, n, nnz, B, x, args.cpu_seconds)
+        result["cpu_baseline_more"] = cpu_more(csr, n, args)
 
     print(json.dumps(result), flush=True)
+
+
+def kron_products(count):
+    """Hubbard 4x4 (5,5) sector, H(U = 2) as the matrix-free two-block
+    product (the quench's real-time operator): `count` timed products after
+    two untimed ones (the first product of a block shape picks the kernel
+    variants).  Prints one JSON line."""
+    import torch
+
+    from qudrip_amd import operators as Q
+    from qudrip_amd.runtime import Basis, Context
+    from qudrip_amd.workloads import make_operator
+    ctx = Context(0)
+    nsite = 16
+    idx, h = Q.hubbard(nsite, Q.hubbard_bonds_square(4, 4), 1.0, 2.0)
+    basis = Basis(ctx, Q.hubbard_sector(idx, nsite, 5, 5))
+    op, keep = make_operator(ctx, basis, h, 2 * nsite, True)
+    n = basis.size
+    x = ctx.seeded_state(n, 3)
+    y = ctx.zeros(n)
+    for _ in range(2):
+        op.apply(x, y)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(count):
+        op.apply(x, y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / count
+    print(json.dumps({"workload": "Hubbard 4x4 (5,5) matrix-free product", "n": n, "products": count,
+                      "us_per_product": round(us, 1), "algorithmic_GBs_32n": round(32 * n / us / 1e3, 1)}),
+          flush=True)
+    for hnd in [op] + keep + [basis]:
+        hnd.close()
 
 
 def extras(ctx):
@@ -345,16 +440,36 @@ def trajectories(ctx, args):
     log(f"trajectories plain: {out['plain_gpu']['traj_steps_per_s']} traj-steps/s; batched "
         f"{out['batched_vs_plain']}x")
     if args.traj_cpu > 0 and not args.no_cpu:
+        import math
+
         import oracle
+        from qudrip_amd import operators as Q
         psi0 = r["psi0"].cpu().numpy()
         t0 = time.perf_counter()
         _, olog = oracle.traj_run(r["ham"], 8, 4, 4, psi0, ch, args.traj_cpu, nsteps, 0.05, 13, 5)
         cpu_s = time.perf_counter() - t0
         out["oracle_prefix_logs_equal"] = bool((olog == r["log"][:args.traj_cpu]).all())
-        out["cpu_baseline"] = {"value": round(args.traj_cpu * nsteps / cpu_s, 2), "unit": "trajectory-steps/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"{args.traj_cpu} trajectories x {nsteps} steps in {cpu_s:.1f} s, oracle "
-                                         "restatement with operators precomputed per sector, 1 thread"}
+        # the loop as the reference runs it: jump operators built by `>>`,
+        # multiplied and evaluated every step, H_eff assembled as a stored
+        # matrix before each Arnoldi step (hubbard_diss.cpp:443,462-463,514-529)
+        g = Q.getHopFermiGate(Q.getQbits(16))
+        lsums = [math.sqrt(rate) * g(m, m) for (_, m, rate) in ch]
+        nf = max(1, args.traj_cpu // 4)
+        t0 = time.perf_counter()
+        _, flog = oracle.traj_run_rebuild(r["ham"], lsums, 8, 4, 4, psi0, ch, nf, nsteps, 0.05, 13, 5)
+        f_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(nf * nsteps / f_s, 2), "unit": "trajectory-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{nf} trajectories x {nsteps} steps in {f_s:.1f} s, oracle restatement "
+                                         "of the reference loop (jump operators and H_eff rebuilt every step), "
+                                         "1 thread",
+                               "logs_equal_batched": bool((flog == r["log"][:nf]).all())}
+        out["cpu_precomputed"] = {"value": round(args.traj_cpu * nsteps / cpu_s, 2), "unit": "trajectory-steps/s",
+                                  "cores": 1, "kind": "port",
+                                  "sample": f"{args.traj_cpu} trajectories x {nsteps} steps in {cpu_s:.1f} s, oracle "
+                                            "restatement with operators precomputed per sector, 1 thread"}
+        out["batched_vs_cpu_reference_loop"] = round(out["traj_steps_per_s"] / out["cpu_baseline"]["value"], 1)
+        out["batched_vs_cpu_precomputed"] = round(out["traj_steps_per_s"] / out["cpu_precomputed"]["value"], 1)
     log(f"trajectories: N(T) = {out['N_T']} vs {out['N_T_closed_form']} (z={out['closed_form_z']})")
     return out
 
@@ -366,12 +481,20 @@ def serving(ctx, args):
     workers, ground states stopped when the Ritz-residual bound r^2/gap or the
     move of E0 over 4 Krylov vectors is <= 1e-12 |E0|, matrix-free real-time
     operator with the Krylov dimension set per step by Saad's estimate).  The
-    ground states use partial reorthogonalisation by default; the stream is
-    served a second time with full reorthogonalisation (the drop-in path's
-    scheme) and reported as served_full_reorth.  Plain: each request alone
-    through the drop-in path (basis + stored matrix per request, Lanczos
-    k = 150 with full reorthogonalisation, evolve with kry_dim_rt = 30).
-    Accuracy: served vs plain energies per request."""
+    ground states use partial reorthogonalisation by default.  Rows, to
+    separate what serving buys from what the per-request algorithm buys:
+      served             -- the stream, args.serve_workers workers;
+      served_1_worker    -- the stream, one worker;
+      one_at_a_time_same_rules -- the server's algorithms and cached
+                            structure, requests one at a time (closed loop);
+      served_full_reorth -- the stream, full reorthogonalisation (the drop-in
+                            path's scheme);
+      plain              -- each request alone through the drop-in path
+                            (basis + stored matrix per request, Lanczos
+                            k = 150 with full reorthogonalisation, evolve with
+                            kry_dim_rt = 30).
+    Latency = arrival -> result (queueing included); service = start on a
+    worker -> result.  Accuracy: served vs plain energies per request."""
     import numpy as np
 
     from qudrip_amd import serving as SV
@@ -379,6 +502,10 @@ def serving(ctx, args):
     reqs = SV.request_stream(args.serve_requests, rate=200.0, seed=4)
     served = SV.run_served(reqs, workers=args.serve_workers, log=log)
     log(f"served: {len(reqs)} requests in {served['span_s']:.2f} s ({served['solves_per_s']:.2f} /s)")
+    one = SV.run_served(reqs, workers=1)
+    log(f"served, one worker: {one['span_s']:.2f} s ({one['solves_per_s']:.2f} /s)")
+    seq = SV.run_served(reqs, workers=1, closed_loop=True)
+    log(f"one at a time, same rules: {seq['span_s']:.2f} s ({seq['solves_per_s']:.2f} /s)")
     full = SV.run_served(reqs, workers=args.serve_workers, gs_method=1)
     log(f"served, full reorthogonalisation: {full['span_s']:.2f} s ({full['solves_per_s']:.2f} /s)")
     plain = SV.run_plain(reqs, ctx)
@@ -396,9 +523,6 @@ def serving(ctx, args):
         else:
             d_rt.append(abs(s["observable"] - p["observable"]))
             drift.append(abs(s["energy"] - s["energy_start"]) / abs(s["energy_start"]))
-    sp50, sp99 = SV.latency_percentiles(served["results"])
-    fp50, fp99 = SV.latency_percentiles(full["results"])
-    pp50, pp99 = SV.latency_percentiles(plain["results"])
     its = [served["results"][r["tag"]]["iterations"] for r in reqs if r["kind"] == "gs"]
     rt_m = [served["results"][r["tag"]]["krylov_mean"] for r in reqs if r["kind"] == "rt"]
     busy = {}
@@ -409,25 +533,40 @@ def serving(ctx, args):
         b[0] += 1
         b[1] += x["t_end"] - x["t_start"]
         b[2] += plain["results"][r["tag"]]["time_s"]
+
+    def row(out, workers):
+        l50, l99 = SV.latency_percentiles(out["results"])
+        s50, s99 = SV.service_percentiles(out["results"])
+        return {"solves_per_s": round(out["solves_per_s"], 3), "span_s": round(out["span_s"], 3),
+                "p50_latency_s": round(l50, 4), "p99_latency_s": round(l99, 4),
+                "p50_service_s": round(s50, 4), "p99_service_s": round(s99, 4), "workers": workers,
+                "failed": sum(1 for x in out["results"].values() if x["status"] != 0)}
+
+    srow = row(served, args.serve_workers)
+    srow.update({"max_running": served["stats"]["max_running"],
+                 "peak_hbm_gb": round(served["stats"]["peak_bytes"] / 1e9, 2),
+                 "model_setup_s": round(served["setup_s"], 3),
+                 "gs_krylov_dim_mean": round(float(np.mean(its)), 1) if its else None,
+                 "rt_krylov_dim_mean": round(float(np.mean(rt_m)), 2) if rt_m else None,
+                 "per_kind_count_served_busy_s_plain_s": {k: [v[0], round(v[1], 3), round(v[2], 3)]
+                                                         for k, v in sorted(busy.items())}})
+    frow = row(full, args.serve_workers)
+    frow.update({"speedup_vs_plain": round(full["solves_per_s"] / plain["solves_per_s"], 3),
+                 "reaches_5x_vs_plain": bool(full["solves_per_s"] >= 5.0 * plain["solves_per_s"]),
+                 "max_rel_dE_gs": max(rel_gs_full) if rel_gs_full else None})
+    p50, p99 = SV.latency_percentiles(plain["results"])
+    q50, q99 = SV.service_percentiles(plain["results"])
     return {"requests": len(reqs), "rate_per_s": 200.0, "gs_fraction": 0.7,
-            "served": {"solves_per_s": round(served["solves_per_s"], 3), "span_s": round(served["span_s"], 3),
-                       "p50_latency_s": round(sp50, 4), "p99_latency_s": round(sp99, 4),
-                       "workers": args.serve_workers, "max_running": served["stats"]["max_running"],
-                       "peak_hbm_gb": round(served["stats"]["peak_bytes"] / 1e9, 2),
-                       "model_setup_s": round(served["setup_s"], 3),
-                       "gs_krylov_dim_mean": round(float(np.mean(its)), 1) if its else None,
-                       "rt_krylov_dim_mean": round(float(np.mean(rt_m)), 2) if rt_m else None,
-                       "per_kind_count_served_busy_s_plain_s": {k: [v[0], round(v[1], 3), round(v[2], 3)]
-                                                               for k, v in sorted(busy.items())},
-                       "failed": sum(1 for x in served["results"].values() if x["status"] != 0)},
-            "served_full_reorth": {"solves_per_s": round(full["solves_per_s"], 3),
-                                   "p50_latency_s": round(fp50, 4), "p99_latency_s": round(fp99, 4),
-                                   "speedup": round(full["solves_per_s"] / plain["solves_per_s"], 3),
-                                   "max_rel_dE_gs": max(rel_gs_full) if rel_gs_full else None,
-                                   "failed": sum(1 for x in full["results"].values() if x["status"] != 0)},
+            "served": srow,
+            "served_1_worker": row(one, 1),
+            "one_at_a_time_same_rules": row(seq, 1),
+            "served_full_reorth": frow,
             "plain": {"solves_per_s": round(plain["solves_per_s"], 3), "busy_s": round(plain["busy_s"], 3),
-                      "p50_latency_s": round(pp50, 4), "p99_latency_s": round(pp99, 4)},
+                      "p50_latency_s": round(p50, 4), "p99_latency_s": round(p99, 4),
+                      "p50_service_s": round(q50, 4), "p99_service_s": round(q99, 4)},
             "speedup": round(served["solves_per_s"] / plain["solves_per_s"], 3),
+            "gain_from_concurrency": round(served["solves_per_s"] / one["solves_per_s"], 3),
+            "gain_from_server_rules_and_cache": round(seq["solves_per_s"] / plain["solves_per_s"], 3),
             "max_rel_dE_gs": max(rel_gs) if rel_gs else None,
             "max_abs_d_docc_rt": max(d_rt) if d_rt else None,
             "max_rel_energy_drift_rt": max(drift) if drift else None}
@@ -437,8 +576,6 @@ def cpu_baseline(csr, n, nnz, B, x, budget_s):
     """Oracle (single-thread C port of the reference's ColMajor scatter SpMV)
     on this host, over the same CSR arrays (bit-identical to the oracle's own
     build), repeated until ~budget_s seconds."""
-    import numpy as np
-
     import oracle
     outer, inner, vals = csr.download()
     xv = x.cpu().numpy()
@@ -449,9 +586,79 @@ def cpu_baseline(csr, n, nnz, B, x, budget_s):
         t_tot += time.perf_counter() - t0
         reps += 1
     gbs = B * reps / t_tot / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} SpMVs of the same XXZ N=24 CSR (n={n}, nnz={nnz}) in {t_tot:.1f} s, "
-                      f"oracle ColMajor scatter (sparse_algorithm.hpp:378), 1 thread, host nproc={os.cpu_count()}"}
+    out = {"value": round(gbs, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+           "sample": f"{reps} SpMVs of the same XXZ N=24 CSR (n={n}, nnz={nnz}) in {t_tot:.1f} s, "
+                     f"oracle ColMajor scatter (sparse_algorithm.hpp:378), 1 thread",
+           "spmv_s": round(t_tot / reps, 4)}
+    out.update(host_cpu())
+    return out
+
+
+def cpu_more(csr, n, args):
+    """The other CPU rows, single thread:
+    - XXZ N=24 ground state: the oracle's getLanczosSolver (two-pass 3-term
+      recurrence, sparse_algorithm.hpp:395-473,564-585) at k = 150 on the same
+      CSR (about 15 s on a current server core);
+    - Hubbard matrix-free product (OperatorSum::map_acc, operator.hpp:215-232)
+      on the 3x4 (6,6) sector (n = 853,776), projected to the 4x4 (5,5) sector
+      by states x terms x log2(states) (the rank lookup is a binary search);
+    - with --cpu-full, the 4x4 product itself (basis scan and one product,
+      a few minutes)."""
+    import math
+
+    import numpy as np
+
+    import oracle
+    from qudrip_amd import operators as Q
+    from qudrip_amd import serving as SV
+    log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+    out = {}
+    outer, inner, vals = csr.download()
+    csc = (outer, inner, vals)
+    psi = oracle.seeded_state(n, 2)
+    k = 150
+    t0 = time.perf_counter()
+    r = oracle.lanczos_gs(csc, k, psi)
+    gs_s = time.perf_counter() - t0
+    out["xxz24_ground_state"] = {
+        "k": k, "solve_s": round(gs_s, 2), "E0": r["E0"],
+        "rel_err": abs(r["E0"] - E_BETHE_XXX24) / abs(E_BETHE_XXX24) if n == 2704156 else None,
+        "method": "oracle getLanczosSolver (two passes of the 3-term recurrence), 1 thread"}
+    log(f"cpu: XXZ24 Lanczos k={k} in {gs_s:.1f} s")
+    nsite, bonds, nup, ndn = SV.hubbard_lattice()
+    idx, H = Q.hubbard(nsite, bonds, 1.0, 4.0)
+    b = oracle.basis_scan(Q.hubbard_sector(idx, nsite, nup, ndn))
+    xs = oracle.seeded_state(len(b), 3)
+    ys = np.zeros(len(b), dtype=np.complex128)
+    t0 = time.perf_counter()
+    oracle.map_acc(b, H, xs, ys)
+    s34 = time.perf_counter() - t0
+    n44, nt34, nt44 = 19079424, len(H.terms), len(Q.hubbard(16, Q.hubbard_bonds_square(4, 4), 1.0, 4.0)[1].terms)
+    proj = s34 * (n44 / len(b)) * (nt44 / nt34) * (math.log2(n44) / math.log2(len(b)))
+    out["hubbard_matfree_product"] = {"hub3x4_n": len(b), "hub3x4_s": round(s34, 2), "hub4x4_n": n44,
+                                      "hub4x4_s": round(proj, 1), "hub4x4_is": "projected",
+                                      "method": "oracle map_acc (operator.hpp:215-232), 1 thread"}
+    log(f"cpu: Hub3x4 map_acc in {s34:.1f} s (4x4 projected {proj:.0f} s)")
+    if args.cpu_full:
+        idx4, H4 = Q.hubbard(16, Q.hubbard_bonds_square(4, 4), 1.0, 2.0)
+        t0 = time.perf_counter()
+        b4 = oracle.basis_scan(Q.hubbard_sector(idx4, 16, 5, 5))
+        scan_s = time.perf_counter() - t0
+        x4 = oracle.seeded_state(len(b4), 3)
+        y4 = np.zeros(len(b4), dtype=np.complex128)
+        t0 = time.perf_counter()
+        oracle.map_acc(b4, H4, x4, y4)
+        s44 = time.perf_counter() - t0
+        out["hubbard_matfree_product"].update({"hub4x4_s": round(s44, 1), "hub4x4_is": "measured",
+                                               "hub4x4_basis_scan_s": round(scan_s, 1)})
+        log(f"cpu: Hub4x4 basis scan {scan_s:.1f} s, map_acc {s44:.1f} s")
+    # one real-time step of the quench (k_rt = 30) makes 29 products; the
+    # projection counts only those (the MGS vector work comes on top)
+    p44 = out["hubbard_matfree_product"]["hub4x4_s"]
+    out["hub4x4_quench_step_s_projected"] = round(29 * p44, 1)
+    out["hub4x4_quench_200_steps_h_projected"] = round(200 * 29 * p44 / 3600, 1)
+    out.update(host_cpu())
+    return out
 
 
 if __name__ == "__main__":

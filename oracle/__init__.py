@@ -74,6 +74,8 @@ def lib():
     L.orc_traj_run.argtypes = [ctypes.c_int, _VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP, _VP,
                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                ctypes.c_uint64, _VP, _VP]
+    L.orc_traj_run_rebuild.restype = ctypes.c_int
+    L.orc_traj_run_rebuild.argtypes = L.orc_traj_run.argtypes + [_VP, _VP, _VP]
     L.orc_traj_uniform.restype = ctypes.c_double
     L.orc_traj_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
     L.orc_arnoldi_evolve.restype = ctypes.c_int
@@ -305,6 +307,48 @@ def traj_run(opsum, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, 
                             ntraj, traj_offset, nsteps, dt, k_rt, seed, _p(dsum), _p(log))
     if rc != 0:
         raise RuntimeError("oracle trajectories failed")
+    return dsum.reshape(nsteps + 1, M), log.reshape(ntraj, max(nsteps, 1))
+
+
+def traj_run_rebuild(opsum, lsums, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k_rt, seed,
+                     traj_offset=0):
+    """traj_run as the reference runs it (hubbard_diss.cpp:443,462-463,514-529):
+    every step the jump operators lsums[c] (e.g. sqrt(g) gate(m, m)) are built
+    by `>>`, multiplied by their adjoints and evaluated, and H_eff is assembled
+    as a stored matrix before the Arnoldi step.  Same outputs as traj_run."""
+    import ctypes
+    terms, nt, ops, no = opsum.program()
+    progs = [ls.program() for ls in lsums]
+    assert len(progs) == len(channels)
+    tot_t = sum(p[1] for p in progs)
+    tot_o = sum(p[3] for p in progs)
+    lt = (type(progs[0][0][0]) * max(1, tot_t))() if progs else None
+    lo = (type(progs[0][2][0]) * max(1, tot_o))() if progs else None
+    off = np.zeros(len(progs) + 1, dtype=np.int32)
+    pt = po = 0
+    for c, (t, ntc, o, noc) in enumerate(progs):
+        for k in range(ntc):
+            ctypes.memmove(ctypes.byref(lt, (pt + k) * ctypes.sizeof(lt[0])), ctypes.byref(t[k]),
+                           ctypes.sizeof(lt[0]))
+            lt[pt + k].op_begin = t[k].op_begin + po
+        for k in range(noc):
+            ctypes.memmove(ctypes.byref(lo, (po + k) * ctypes.sizeof(lo[0])), ctypes.byref(o[k]),
+                           ctypes.sizeof(lo[0]))
+        pt += ntc
+        po += noc
+        off[c + 1] = pt
+    M = 2 * nsite
+    jk = np.array([c[0] for c in channels], dtype=np.int32)
+    jm = np.array([c[1] for c in channels], dtype=np.int32)
+    jr = np.array([c[2] for c in channels], dtype=np.float64)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    dsum = np.zeros((nsteps + 1) * M)
+    log = np.zeros(ntraj * max(nsteps, 1), dtype=np.int32)
+    rc = lib().orc_traj_run_rebuild(nsite, terms, nt, ops, nup0, ndn0, _p(psi0), _p(jk), _p(jm), _p(jr),
+                                    len(channels), ntraj, traj_offset, nsteps, dt, k_rt, seed, _p(dsum), _p(log),
+                                    lt, _p(off), lo)
+    if rc != 0:
+        raise RuntimeError("oracle trajectories (rebuild) failed")
     return dsum.reshape(nsteps + 1, M), log.reshape(ntraj, max(nsteps, 1))
 
 

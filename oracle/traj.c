@@ -41,6 +41,15 @@ int64_t orc_rank(const uint64_t* basis, int64_t n, uint64_t x);
 int64_t orc_build_csc(const uint64_t* basis, int64_t n, const qd_term* terms, int nterms, const qd_op* ops,
                       int32_t* colptr, int32_t* rowidx, double* vals, int64_t cap);
 int orc_expm(int m, const double* A_in, double* E_out);
+int64_t orc_csr_add(int64_t n, const int32_t* ao, const int32_t* ai, const double* av, const int32_t* bo,
+                    const int32_t* bi, const double* bv, int sign, int32_t* co, int32_t* ci, double* cv);
+void orc_csr_scale(int64_t nnz, double c_re, double c_im, int real_scalar, const double* av, double* cv);
+void orc_csr_adjoint(int64_t n, const int32_t* ao, const int32_t* ai, const double* av, int32_t* co, int32_t* ci,
+                     double* cv);
+int64_t orc_csr_multiply(int64_t n, const int32_t* ao, const int32_t* ai, const double* av, const int32_t* bo,
+                         const int32_t* bi, const double* bv, int32_t* co, int32_t* ci, double* cv);
+void orc_spmv_csc(int64_t n, const int32_t* colptr, const int32_t* rowidx, const double* vals, const double* x,
+                  double* y);
 
 static inline uint64_t sm64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -60,6 +69,12 @@ typedef struct {
   int32_t *cp, *ri;
   double* va;
   double* D; /* sum_c g_c n_m per state */
+  /* faithful mode: qj_tot = sum_c L_c^+ L_c, built once per sector like the
+   * reference's qj_tot before its time loop */
+  int qj_built;
+  int64_t qj_nnz;
+  int32_t *qo, *qi;
+  double* qv;
 } sector_t;
 
 static int build_sector(sector_t* S, int nsite, int a, int b, const qd_term* terms, int nterms, const qd_op* ops,
@@ -113,10 +128,44 @@ static double vnorm2(int64_t n, const tc* x) {
   return s;
 }
 
+/* A stored matrix in the reference's ColMajor arrays. */
+typedef struct {
+  int64_t n, nnz;
+  int32_t *o, *i;
+  double* v;
+} csc_t;
+static void csc_free(csc_t* a) {
+  free(a->o);
+  free(a->i);
+  free(a->v);
+  memset(a, 0, sizeof(*a));
+}
+static void csc_alloc(csc_t* a, int64_t n, int64_t nnz) {
+  a->n = n;
+  a->nnz = nnz;
+  a->o = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n + 1));
+  a->i = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+  a->v = (double*)malloc(sizeof(double) * 2 * (size_t)(nnz ? nnz : 1));
+}
+
+/* y = H_eff x for the Arnoldi step: the implicit form (stored H plus the
+ * diagonal -i/2 (D - Gamma)) or an assembled stored matrix. */
+typedef struct {
+  const sector_t* S;
+  double Gamma;
+  const csc_t* A; /* non-NULL: the assembled H_eff */
+} heff_t;
+static void heff_op(const heff_t* op, const tc* x, tc* y) {
+  if (op->A)
+    orc_spmv_csc(op->A->n, op->A->o, op->A->i, op->A->v, (const double*)x, (double*)y);
+  else
+    heff_apply(op->S, op->Gamma, x, y);
+}
+
 /* Arnoldi step of ArnoldiSolver<T,0>::evolve (sparse_algorithm.hpp:548-562):
  * m = k_rt - 1 MGS steps, psi_out = V_m expm(-i h K_m) e_1. */
-static int arnoldi_heff(const sector_t* S, double Gamma, int k_rt, double h, const tc* psi, tc* out) {
-  int64_t n = S->n;
+static int arnoldi_heff(const heff_t* op, int k_rt, double h, const tc* psi, tc* out) {
+  int64_t n = op->S->n;
   int m = k_rt - 1;
   tc** V = (tc**)malloc(sizeof(tc*) * (m + 1));
   for (int j = 0; j <= m; ++j) V[j] = (tc*)malloc(sizeof(tc) * (size_t)n);
@@ -128,7 +177,7 @@ static int arnoldi_heff(const sector_t* S, double Gamma, int k_rt, double h, con
   }
   int idx;
   for (idx = 1; idx <= m; ++idx) {
-    heff_apply(S, Gamma, V[idx - 1], V[idx]);
+    heff_op(op, V[idx - 1], V[idx]);
     for (int i = 0; i < idx; ++i) {
       tc hh = {0, 0};
       for (int64_t r = 0; r < n; ++r) {
@@ -153,7 +202,7 @@ static int arnoldi_heff(const sector_t* S, double Gamma, int k_rt, double h, con
     }
   }
   int act = idx - 1;
-  tc* A = (tc*)malloc(sizeof(tc) * act * act);
+  tc* A = (tc*)calloc((size_t)(act ? act * act : 1), sizeof(tc));
   tc* E = (tc*)malloc(sizeof(tc) * act * act);
   for (int j = 0; j < act; ++j)
     for (int i = 0; i < act; ++i) {
@@ -179,15 +228,82 @@ static int arnoldi_heff(const sector_t* S, double Gamma, int k_rt, double h, con
   return rc;
 }
 
+/* `program >> sector` (operator_abel.hpp:25-34): the ColMajor arrays of one
+ * channel's L_c over the sector's basis. */
+static void build_prog(const sector_t* S, const qd_term* t, int nt, const qd_op* ops, csc_t* out) {
+  int32_t* cp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(S->n + 1));
+  int64_t nnz = orc_build_csc(S->basis, S->n, t, nt, ops, cp, NULL, NULL, 0);
+  free(cp);
+  csc_alloc(out, S->n, nnz < 0 ? 0 : nnz);
+  orc_build_csc(S->basis, S->n, t, nt, ops, out->o, out->i, out->v, out->nnz);
+}
+
+/* SpMatrix(L.adjoint()) * L (hubbard_diss.cpp:527). */
+static void lhl(const csc_t* L, csc_t* out) {
+  csc_t La;
+  csc_alloc(&La, L->n, L->nnz);
+  orc_csr_adjoint(L->n, L->o, L->i, L->v, La.o, La.i, La.v);
+  int64_t nnz = orc_csr_multiply(L->n, La.o, La.i, La.v, L->o, L->i, L->v, NULL, NULL, NULL);
+  csc_alloc(out, L->n, nnz);
+  orc_csr_multiply(L->n, La.o, La.i, La.v, L->o, L->i, L->v, out->o, out->i, out->v);
+  csc_free(&La);
+}
+
+/* C = A + sign B, allocated. */
+static void add_into(const csc_t* A, const csc_t* B, int sign, csc_t* C) {
+  int64_t nnz = orc_csr_add(A->n, A->o, A->i, A->v, B->o, B->i, B->v, sign, NULL, NULL, NULL);
+  csc_alloc(C, A->n, nnz);
+  orc_csr_add(A->n, A->o, A->i, A->v, B->o, B->i, B->v, sign, C->o, C->i, C->v);
+}
+
+/* C = c A, allocated (c real when c_im == 0 and real_scalar). */
+static void scale_into(const csc_t* A, double c_re, double c_im, int real_scalar, csc_t* C) {
+  csc_alloc(C, A->n, A->nnz);
+  memcpy(C->o, A->o, sizeof(int32_t) * (size_t)(A->n + 1));
+  memcpy(C->i, A->i, sizeof(int32_t) * (size_t)(A->nnz ? A->nnz : 1));
+  orc_csr_scale(A->nnz, c_re, c_im, real_scalar, A->v, C->v);
+}
+
+/* <psi|M|psi> (real part) by the stored product, like State::eval. */
+static double expect_re(const csc_t* M, const tc* psi, tc* tmp) {
+  orc_spmv_csc(M->n, M->o, M->i, M->v, (const double*)psi, (double*)tmp);
+  double s = 0.0;
+  for (int64_t r = 0; r < M->n; ++r) s += psi[r].re * tmp[r].re + psi[r].im * tmp[r].im;
+  return s;
+}
+
+/* qj_tot = sum_c L_c^+ L_c over the sector (once per sector). */
+static void build_qj(sector_t* S, int nj, const qd_term* lterms, const int* loff, const qd_op* lops) {
+  csc_t acc;
+  csc_alloc(&acc, S->n, 0);
+  memset(acc.o, 0, sizeof(int32_t) * (size_t)(S->n + 1));
+  for (int c = 0; c < nj; ++c) {
+    csc_t L, M, next;
+    build_prog(S, lterms + loff[c], loff[c + 1] - loff[c], lops, &L);
+    lhl(&L, &M);
+    add_into(&acc, &M, 1, &next);
+    csc_free(&L);
+    csc_free(&M);
+    csc_free(&acc);
+    acc = next;
+  }
+  S->qj_nnz = acc.nnz;
+  S->qo = acc.o;
+  S->qi = acc.i;
+  S->qv = acc.v;
+  S->qj_built = 1;
+}
+
 /*
  * Runs trajectories traj_offset .. traj_offset + ntraj - 1 from psi0 (sector
  * (nup0, ndn0), ascending basis order).  dens_sum[(nsteps+1) * 2 nsite]
  * receives the sum over trajectories of <n_m>(t_i); jump_log (optional,
  * ntraj * nsteps) the channel taken at each step or -1.  Returns 0 or -1.
  */
-int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, int nup0, int ndn0,
-                 const double* psi0, const int* jkind, const int* jmode, const double* jrate, int nj, int ntraj,
-                 int traj_offset, int nsteps, double dt, int k_rt, uint64_t seed, double* dens_sum, int* jump_log) {
+static int traj_impl(int nsite, const qd_term* terms, int nterms, const qd_op* ops, int nup0, int ndn0,
+                     const double* psi0, const int* jkind, const int* jmode, const double* jrate, int nj, int ntraj,
+                     int traj_offset, int nsteps, double dt, int k_rt, uint64_t seed, double* dens_sum,
+                     int* jump_log, const qd_term* lterms, const int* loff, const qd_op* lops) {
   int NS = (nsite + 1) * (nsite + 1);
   sector_t* sec = (sector_t*)calloc(NS, sizeof(sector_t));
   int M = 2 * nsite;
@@ -225,9 +341,25 @@ int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, 
       if (step == nsteps) break;
       /* weights and the jump test */
       double G = 0.0;
-      for (int c = 0; c < nj; ++c) {
-        w[c] = jrate[c] * dens[jmode[c]];
-        G += w[c];
+      if (lterms) {
+        /* lindblad_weight (hubbard_diss.cpp:519-529): every step, each L_c
+         * is built by `>>`, multiplied by its adjoint and evaluated */
+        tc* tmp = (tc*)malloc(sizeof(tc) * (size_t)(S->n ? S->n : 1));
+        for (int c = 0; c < nj; ++c) {
+          csc_t L, M;
+          build_prog(S, lterms + loff[c], loff[c + 1] - loff[c], lops, &L);
+          lhl(&L, &M);
+          w[c] = expect_re(&M, psi, tmp) / n2;
+          G += w[c];
+          csc_free(&L);
+          csc_free(&M);
+        }
+        free(tmp);
+      } else {
+        for (int c = 0; c < nj; ++c) {
+          w[c] = jrate[c] * dens[jmode[c]];
+          G += w[c];
+        }
       }
       int64_t gtraj = traj_offset + tr;
       double r0 = orc_traj_uniform(seed, gtraj, step, 0);
@@ -244,7 +376,15 @@ int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, 
       if (jump_log) jump_log[(size_t)tr * nsteps + step] = ch;
       if (ch >= 0) {
         int m = jmode[ch];
-        if (jkind[ch] == 0) { /* dephasing: n_m psi */
+        if (jkind[ch] == 0 && lterms) { /* jump_op = L >> sector; psi <- L L psi (:462-463) */
+          csc_t L;
+          build_prog(S, lterms + loff[ch], loff[ch + 1] - loff[ch], lops, &L);
+          tc* t1 = (tc*)malloc(sizeof(tc) * (size_t)(S->n ? S->n : 1));
+          orc_spmv_csc(S->n, L.o, L.i, L.v, (const double*)psi, (double*)t1);
+          orc_spmv_csc(S->n, L.o, L.i, L.v, (const double*)t1, (double*)psi);
+          free(t1);
+          csc_free(&L);
+        } else if (jkind[ch] == 0) { /* dephasing: n_m psi */
           for (int64_t s = 0; s < S->n; ++s)
             if (!((S->basis[s] >> m) & 1ull)) psi[s].re = psi[s].im = 0.0;
         } else { /* loss: c_m psi into the neighbouring sector */
@@ -278,7 +418,37 @@ int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, 
         }
       } else {
         tc* out = (tc*)malloc(sizeof(tc) * (size_t)S->n);
-        if (arnoldi_heff(S, G, k_rt, dt, psi, out)) {
+        heff_t op = {S, G, NULL};
+        csc_t Heff = {0, 0, NULL, NULL, NULL};
+        if (lterms) {
+          /* Htot assembled (a linear combination, :443), then QJ_operator
+           * (:514-517): H -= 0.5 i (qj_tot - Gamma Id) */
+          if (!S->qj_built) build_qj(S, nj, lterms, loff, lops);
+          csc_t H = {S->n, S->cp[S->n], S->cp, S->ri, S->va};
+          csc_t Q = {S->n, S->qj_nnz, S->qo, S->qi, S->qv};
+          csc_t Id, GId, A, B, Htot;
+          csc_alloc(&Id, S->n, S->n);
+          for (int64_t r = 0; r <= S->n; ++r) Id.o[r] = (int32_t)r;
+          for (int64_t r = 0; r < S->n; ++r) {
+            Id.i[r] = (int32_t)r;
+            Id.v[2 * r] = 1.0;
+            Id.v[2 * r + 1] = 0.0;
+          }
+          scale_into(&Id, G, 0.0, 1, &GId);
+          add_into(&Q, &GId, -1, &A);
+          scale_into(&A, 0.0, 0.5, 0, &B);
+          scale_into(&H, 1.0, 0.0, 1, &Htot);
+          add_into(&Htot, &B, -1, &Heff);
+          csc_free(&Id);
+          csc_free(&GId);
+          csc_free(&A);
+          csc_free(&B);
+          csc_free(&Htot);
+          op.A = &Heff;
+        }
+        int erc = arnoldi_heff(&op, k_rt, dt, psi, out);
+        if (lterms) csc_free(&Heff);
+        if (erc) {
           free(out);
           rc = -1;
           break;
@@ -300,9 +470,39 @@ int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, 
       free(sec[i].ri);
       free(sec[i].va);
       free(sec[i].D);
+      if (sec[i].qj_built) {
+        free(sec[i].qo);
+        free(sec[i].qi);
+        free(sec[i].qv);
+      }
     }
   free(sec);
   free(dens);
   free(w);
   return rc;
+}
+
+int orc_traj_run(int nsite, const qd_term* terms, int nterms, const qd_op* ops, int nup0, int ndn0,
+                 const double* psi0, const int* jkind, const int* jmode, const double* jrate, int nj, int ntraj,
+                 int traj_offset, int nsteps, double dt, int k_rt, uint64_t seed, double* dens_sum, int* jump_log) {
+  return traj_impl(nsite, terms, nterms, ops, nup0, ndn0, psi0, jkind, jmode, jrate, nj, ntraj, traj_offset, nsteps,
+                   dt, k_rt, seed, dens_sum, jump_log, NULL, NULL, NULL);
+}
+
+/* The same loop as the reference runs it: every step the jump operators
+ * L_c (programs lterms[loff[c] .. loff[c+1]), ops lops, e.g. sqrt(g) n_m) are
+ * built by `>>`, multiplied by their adjoints and evaluated for the weights
+ * (hubbard_diss.cpp:519-529); a dephasing jump applies the built L_c twice
+ * (:462-463); a non-jump step assembles H_eff = Htot - 0.5 i (qj_tot -
+ * Gamma Id) as a stored matrix (:443, :514-517) and runs the Arnoldi step on
+ * it.  qj_tot is built once per sector.  Loss jumps move between sectors by
+ * rank lookups as above.  Same random streams, so the jump logs agree with
+ * orc_traj_run's up to rounding of the weights. */
+int orc_traj_run_rebuild(int nsite, const qd_term* terms, int nterms, const qd_op* ops, int nup0, int ndn0,
+                         const double* psi0, const int* jkind, const int* jmode, const double* jrate, int nj,
+                         int ntraj, int traj_offset, int nsteps, double dt, int k_rt, uint64_t seed, double* dens_sum,
+                         int* jump_log, const qd_term* lterms, const int* loff, const qd_op* lops) {
+  if (!lterms || !loff || !lops) return -1;
+  return traj_impl(nsite, terms, nterms, ops, nup0, ndn0, psi0, jkind, jmode, jrate, nj, ntraj, traj_offset, nsteps,
+                   dt, k_rt, seed, dens_sum, jump_log, lterms, loff, lops);
 }
