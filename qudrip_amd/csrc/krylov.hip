@@ -918,11 +918,11 @@ struct RowsGrid {
 };
 template <int RT>
 static RowsGrid rows_grid(int64_t n, int ncu) {
-  const size_t lds = (size_t)(KT / 64) * RowsGrid::kCmax * 4 * sizeof(double);
+  constexpr size_t lds = (size_t)(KT / 64) * RowsGrid::kCmax * 4 * sizeof(double);
   // queried once per process; the initialisation of a function-local static
   // is thread-safe, so concurrent solves (server workers) never see a
   // half-written value
-  static const int per_cu = [lds]() {
+  static const int per_cu = []() {
     int b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dc_red_rows_kernel<RT>, KT, lds) != hipSuccess || b < 1) b = 1;
     return b;

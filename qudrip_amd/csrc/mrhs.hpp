@@ -63,7 +63,7 @@ __device__ inline int find_sec(int ns, int64_t v, F key) {
 struct Tile {
   int sec, chunk, t, sub, rstep;
 };
-__device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
+__device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns, int lane) {
   Tile r;
   const int64_t b = blockIdx.x;
   r.sec = find_sec(ns, b, [&](int i) { return gs[i].tile0; });
@@ -71,16 +71,17 @@ __device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) {
   const int64_t local = b - S.tile0;
   r.chunk = (int)(local / S.groups);
   if (S.mp < TT) {
-    r.t = (int)threadIdx.x % S.mp;
-    r.sub = (int)threadIdx.x / S.mp;
+    r.t = lane % S.mp;
+    r.sub = lane / S.mp;
     r.rstep = TT / S.mp;
   } else {
-    r.t = (int)(local % S.groups) * TT + threadIdx.x;
+    r.t = (int)(local % S.groups) * TT + lane;
     r.sub = 0;
     r.rstep = 1;
   }
   return r;
 }
+__device__ inline Tile tile_of(const GSec* __restrict__ gs, int ns) { return tile_of(gs, ns, (int)threadIdx.x); }
 
 // sum over the rstep lanes of a tile that hold the same slot (lanes t, t + mp, ...)
 __device__ inline double sub_sum(double v, int mp) {
@@ -274,8 +275,13 @@ __global__ void __launch_bounds__(TT)
       const cplx x = cmake(x0.x * ft, x0.y * ft);
       const cplx dg = cmake(al.x * S.D[s] + be.x, al.y * S.D[s] + be.y);
       const cplx dx = cmul(dg, x);
-      if (Vout) Vout[S.vo + s * m + t] = x;
-      Wout[S.vo + s * m + t] = cmake(sx * ft + dx.x, sy * ft + dx.y);
+      // both outputs stream out non-temporally: the L2 is left to the gathered rows of src
+      if (Vout) {
+        __builtin_nontemporal_store(x.x, &Vout[S.vo + s * m + t].x);
+        __builtin_nontemporal_store(x.y, &Vout[S.vo + s * m + t].y);
+      }
+      __builtin_nontemporal_store(sx * ft + dx.x, &Wout[S.vo + s * m + t].x);
+      __builtin_nontemporal_store(sy * ft + dx.y, &Wout[S.vo + s * m + t].y);
     }
   }
 }
@@ -349,6 +355,109 @@ __global__ void __launch_bounds__(TT)
   }
   pb[(int64_t)(P - 2) * m] = uu;
   pb[(int64_t)(P - 1) * m] = uw;
+}
+
+// g_red_kernel with the basis split over QW waves of one block: all QW waves
+// walk the same rows and slots of a tile, wave w takes the vectors q = w,
+// w + QW, ... (its accumulators only).  Rows go in rounds of QW: wave w
+// loads u and w of row w of the round into LDS (double-buffered, one
+// barrier per round), and every wave reads them from there, so each line of
+// u and w is requested from L2 once per block.  The last wave also sums
+// |u|^2 and u^H w.  Per-lane registers hold ceil(NQ / QW) pairs of
+// accumulators instead of NQ, so more waves stay resident to hide the
+// loads.  Same partials, same layout, rows added in the same order.
+// Every wave of a block runs the same number of rounds (the loop bounds do
+// not depend on the lane), so the barriers are uniform.
+constexpr int QW = 4;
+typedef double mr_dbl2 __attribute__((ext_vector_type(2)));
+__device__ inline cplx nt_load(const cplx* p) {
+  const mr_dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const mr_dbl2*>(p));
+  return cmake(v.x, v.y);
+}
+template <int NQ, int NR>
+__global__ void __launch_bounds__(QW * TT)
+    g_red_split_kernel(const GSec* __restrict__ gs, int ns, const cplx* __restrict__ V, int64_t VS,
+                       const cplx* __restrict__ U, const cplx* __restrict__ W, cplx* __restrict__ part) {
+  constexpr int P = NR * NQ + 2;
+  constexpr int NQW = (NQ + QW - 1) / QW;
+  __shared__ cplx su[2][QW][TT], sw[2][QW][TT];
+  const int wv = threadIdx.x / TT, lane = threadIdx.x % TT;
+  const Tile tl = tile_of(gs, ns, lane);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  const bool act = t < m;
+  const bool sums = wv == QW - 1;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);  // uniform over the block
+  const cplx* __restrict__ Vb = V + S.vo;
+  const cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+  cplx ac[NQW], ad[NR == 2 ? NQW : 1];
+  cplx uu = cmake(0.0, 0.0), uw = cmake(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < NQW; ++i) ac[i] = cmake(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < (NR == 2 ? NQW : 1); ++i) ad[i] = cmake(0.0, 0.0);
+  const int64_t round = (int64_t)QW * tl.rstep;
+  int buf = 0;
+  for (int64_t sb = r0; sb < r1; sb += round, buf ^= 1) {
+    {
+      const int64_t s = sb + tl.sub + (int64_t)wv * tl.rstep;
+      cplx u = cmake(0.0, 0.0), w = cmake(0.0, 0.0);
+      if (act && s < r1) {
+        u = Ub[s * m + t];
+        if (NR == 2) w = Wb[s * m + t];
+      }
+      su[buf][wv][lane] = u;
+      if (NR == 2) sw[buf][wv][lane] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < QW; ++rr) {
+      const int64_t s = sb + tl.sub + (int64_t)rr * tl.rstep;
+      if (!act || s >= r1) continue;
+      const int64_t o = s * m + t;
+      const cplx u = su[buf][rr][lane];
+      const cplx w = (NR == 2) ? sw[buf][rr][lane] : cmake(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < NQW; ++i) {
+        const int q = wv + QW * i;
+        if (q < NQ) {
+          const cplx v = nt_load(Vb + q * VS + o);
+          ac[i] = cadd(ac[i], cdotmul(v, u));
+          if (NR == 2) ad[i] = cadd(ad[i], cdotmul(v, w));
+        }
+      }
+      if (sums) {
+        uu = cadd(uu, cdotmul(u, u));
+        if (NR == 2) uw = cadd(uw, cdotmul(u, w));
+      }
+    }
+  }
+  if (tl.rstep > 1) {
+#pragma unroll
+    for (int i = 0; i < NQW; ++i) {
+      ac[i] = sub_sum(ac[i], S.mp);
+      if (NR == 2) ad[i] = sub_sum(ad[i], S.mp);
+    }
+    uu = sub_sum(uu, S.mp);
+    uw = sub_sum(uw, S.mp);
+  }
+  if (!act || tl.sub != 0) return;
+  cplx* pb = part + P * S.co + (int64_t)tl.chunk * P * m + t;
+#pragma unroll
+  for (int i = 0; i < NQW; ++i) {
+    const int q = wv + QW * i;
+    if (q < NQ) {
+      pb[(int64_t)q * m] = ac[i];
+      if (NR == 2) pb[(int64_t)(NQ + q) * m] = ad[i];
+    }
+  }
+  if (sums) {
+    pb[(int64_t)(P - 2) * m] = uu;
+    pb[(int64_t)(P - 1) * m] = uw;
+  }
 }
 
 // red[p Mtot + slot] = sum over chunks (fixed order) of the partials, P per slot.
@@ -469,6 +578,90 @@ __global__ void __launch_bounds__(TT)
     Vb[(int64_t)j * VS + o] = crmul(inv, csub(u, s1));
     const cplx uf = cmul(u, f);
     Ub[o] = cmake(w.x * inv - s2.x - uf.x, w.y * inv - s2.y - uf.y);
+  }
+}
+
+// g_dc_update_kernel with the basis split over QW waves: wave w sums
+// v_q a_q and v_q b_q over its vectors q = w, w + QW, ... for QW rows at a
+// time and parks the two partial sums in LDS; after a barrier, wave w
+// finishes row w of the round (partials of waves 0..QW-1 added in that
+// order, then the stores of q_j and the next pending vector).  The
+// coefficients of a lane's vectors stay in registers (ceil(NQ / QW) pairs).
+// Every wave of a block runs the same number of rounds (the loop bounds do
+// not depend on the lane), so the barriers are uniform.
+template <int NQ>
+__global__ void __launch_bounds__(QW * TT)
+    g_dc_update_split_kernel(const GSec* __restrict__ gs, int ns, cplx* __restrict__ V, int64_t VS, int j,
+                             cplx* __restrict__ U, const cplx* __restrict__ W, const cplx* __restrict__ coef,
+                             const cplx* __restrict__ sc, int Mtot, int K) {
+  constexpr int NQW = (NQ + QW - 1) / QW;
+  __shared__ cplx sh[QW][2][QW][TT];  // [wave][s1, s2][row of the round][lane]
+  const int wv = threadIdx.x / TT, lane = threadIdx.x % TT;
+  const Tile tl = tile_of(gs, ns, lane);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  const bool act = t < m;
+  const int g = S.so + (act ? t : 0);
+  cplx a[NQW], b[NQW];
+#pragma unroll
+  for (int i = 0; i < NQW; ++i) {
+    const int q = wv + QW * i;
+    a[i] = (act && q < NQ) ? coef[(int64_t)q * Mtot + g] : cmake(0.0, 0.0);
+    b[i] = (act && q < NQ) ? coef[(int64_t)(K + q) * Mtot + g] : cmake(0.0, 0.0);
+  }
+  const double inv = act ? sc[2 * g].x : 0.0;
+  const cplx f = act ? sc[2 * g + 1] : cmake(0.0, 0.0);
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = min<int64_t>(S.n, r0 + S.rpc);  // uniform over the block
+  cplx* __restrict__ Vb = V + S.vo;
+  cplx* __restrict__ Ub = U + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+  const int64_t round = (int64_t)QW * tl.rstep;
+  for (int64_t sb = r0; sb < r1; sb += round) {
+    // this wave's row of the round: u and w requested before the basis loads
+    const int64_t sf = sb + tl.sub + (int64_t)wv * tl.rstep;
+    const bool fin = act && sf < r1;
+    const cplx u_row = fin ? Ub[sf * m + t] : cmake(0.0, 0.0);
+    const cplx w_row = fin ? Wb[sf * m + t] : cmake(0.0, 0.0);
+#pragma unroll
+    for (int rr = 0; rr < QW; ++rr) {
+      const int64_t s = sb + tl.sub + (int64_t)rr * tl.rstep;
+      cplx s1 = cmake(0.0, 0.0), s2 = cmake(0.0, 0.0);
+      if (act && s < r1) {
+        const int64_t o = s * m + t;
+#pragma unroll
+        for (int i = 0; i < NQW; ++i) {
+          const int q = wv + QW * i;
+          if (q < NQ) {
+            const cplx v = nt_load(Vb + q * VS + o);
+            s1 = cadd(s1, cmul(v, a[i]));
+            s2 = cadd(s2, cmul(v, b[i]));
+          }
+        }
+      }
+      sh[wv][0][rr][lane] = s1;
+      sh[wv][1][rr][lane] = s2;
+    }
+    __syncthreads();
+    if (fin) {
+      const int rr = wv;
+      cplx s1 = sh[0][0][rr][lane], s2 = sh[0][1][rr][lane];
+#pragma unroll
+      for (int w2 = 1; w2 < QW; ++w2) {
+        s1 = cadd(s1, sh[w2][0][rr][lane]);
+        s2 = cadd(s2, sh[w2][1][rr][lane]);
+      }
+      {
+        const int64_t o = sf * m + t;
+        const cplx qj = crmul(inv, csub(u_row, s1));
+        __builtin_nontemporal_store(qj.x, &Vb[(int64_t)j * VS + o].x);
+        __builtin_nontemporal_store(qj.y, &Vb[(int64_t)j * VS + o].y);
+        const cplx uf = cmul(u_row, f);
+        Ub[o] = cmake(w_row.x * inv - s2.x - uf.x, w_row.y * inv - s2.y - uf.y);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -611,13 +804,19 @@ __global__ void __launch_bounds__(RW * TT)
   dens_all[(int64_t)slot2traj[S.so + t] * (M + 1) + q] = (q < M) ? v / nn : v;
 }
 
+// basis sizes from which the reduction and update passes split the basis
+// over QW waves (below it the one-wave kernels hold few registers anyway)
+constexpr int QD_SPLIT_MIN = 4;
 template <int NR>
 void launch_red(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, const cplx* V, int64_t VS,
                 const cplx* U, const cplx* W, cplx* part) {
   switch (nq) {
-#define QD_RED_CASE(N)                                                            \
-  case N:                                                                         \
-    g_red_kernel<N, NR><<<tiles, TT, 0, st>>>(gs, ns, V, VS, U, W, part);         \
+#define QD_RED_CASE(N)                                                                  \
+  case N:                                                                               \
+    if (N >= QD_SPLIT_MIN)                                                              \
+      g_red_split_kernel<N, NR><<<tiles, QW * TT, 0, st>>>(gs, ns, V, VS, U, W, part);  \
+    else                                                                                \
+      g_red_kernel<N, NR><<<tiles, TT, 0, st>>>(gs, ns, V, VS, U, W, part);             \
     break;
     QD_RED_CASE(1) QD_RED_CASE(2) QD_RED_CASE(3) QD_RED_CASE(4) QD_RED_CASE(5) QD_RED_CASE(6)
     QD_RED_CASE(7) QD_RED_CASE(8) QD_RED_CASE(9) QD_RED_CASE(10) QD_RED_CASE(11) QD_RED_CASE(12)
@@ -634,9 +833,12 @@ void launch_red(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, 
 void launch_dc_update(int nq, unsigned tiles, hipStream_t st, const GSec* gs, int ns, cplx* V, int64_t VS, int j,
                       cplx* U, const cplx* W, const cplx* coef, const cplx* sc, int Mtot, int K) {
   switch (nq) {
-#define QD_UPD_CASE(N)                                                                         \
-  case N:                                                                                      \
-    g_dc_update_kernel<N><<<tiles, TT, 0, st>>>(gs, ns, V, VS, j, U, W, coef, sc, Mtot, K);    \
+#define QD_UPD_CASE(N)                                                                                   \
+  case N:                                                                                                \
+    if (N >= QD_SPLIT_MIN)                                                                               \
+      g_dc_update_split_kernel<N><<<tiles, QW * TT, 0, st>>>(gs, ns, V, VS, j, U, W, coef, sc, Mtot, K);  \
+    else                                                                                                 \
+      g_dc_update_kernel<N><<<tiles, TT, 0, st>>>(gs, ns, V, VS, j, U, W, coef, sc, Mtot, K);            \
     break;
     QD_UPD_CASE(1) QD_UPD_CASE(2) QD_UPD_CASE(3) QD_UPD_CASE(4) QD_UPD_CASE(5) QD_UPD_CASE(6)
     QD_UPD_CASE(7) QD_UPD_CASE(8) QD_UPD_CASE(9) QD_UPD_CASE(10) QD_UPD_CASE(11) QD_UPD_CASE(12)

@@ -27,6 +27,8 @@
 // choice; a different choice moves sums by rounding only.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "mrhs.hpp"
@@ -420,8 +422,9 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEventCreate(&ev0);
   hipEventCreate(&ev1);
+  const bool tune_log = std::getenv("QD_TUNE_LOG") != nullptr;
   auto tune_tiles = [&]() {
-    const int cand[5] = {512, 1024, 2048, 4096, 8192};
+    const int cand[7] = {512, 1024, 2048, 4096, 8192, 16384, 32768};
     float best = 1e30f;
     int best_c = occupancy_target;
     for (int c : cand) {
@@ -435,6 +438,9 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       hipEventSynchronize(ev1);
       float ms = 0.f;
       hipEventElapsedTime(&ms, ev0, ev1);
+      if (tune_log)
+        std::fprintf(stderr, "[qd] trajectory tiles: target %d -> %lld tiles over %d sectors, reduction pass %.1f us\n", c,
+                     (long long)tiles, ns, 1000.0 * ms / 2);
       if (ms < best) {
         best = ms;
         best_c = c;
