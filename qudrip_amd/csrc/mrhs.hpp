@@ -620,10 +620,8 @@ __global__ void __launch_bounds__(QW * TT)
   const int64_t round = (int64_t)QW * tl.rstep;
   for (int64_t sb = r0; sb < r1; sb += round) {
     // this wave's row of the round: u and w requested before the basis loads
-    const int64_t sf = sb + tl.sub + (int64_t)wv * tl.rstep;
+    const int64_t sf = sb + tl.sub + (int64_t)wv * tl.rstep;  // this wave's row of the round
     const bool fin = act && sf < r1;
-    const cplx u_row = fin ? Ub[sf * m + t] : cmake(0.0, 0.0);
-    const cplx w_row = fin ? Wb[sf * m + t] : cmake(0.0, 0.0);
 #pragma unroll
     for (int rr = 0; rr < QW; ++rr) {
       const int64_t s = sb + tl.sub + (int64_t)rr * tl.rstep;
@@ -654,6 +652,8 @@ __global__ void __launch_bounds__(QW * TT)
       }
       {
         const int64_t o = sf * m + t;
+        const cplx u_row = Ub[o];
+        const cplx w_row = Wb[o];
         const cplx qj = crmul(inv, csub(u_row, s1));
         __builtin_nontemporal_store(qj.x, &Vb[(int64_t)j * VS + o].x);
         __builtin_nontemporal_store(qj.y, &Vb[(int64_t)j * VS + o].y);
