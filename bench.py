@@ -272,7 +272,15 @@ def main():
               f"({hs * 1e3 / steps:.3f} ms/step)", file=sys.stderr, flush=True)
         hl.close()
 
+    # sections start from a clean device heap: cached blocks of the earlier
+    # sections are returned first (a server that starts after heavy
+    # allocation churn in its process serves measurably slower)
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        result["serving"] = serving(ctx, args)
+
     if not args.no_quench:
+        torch.cuda.empty_cache()
         from qudrip_amd import workloads as WL
         log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
         q = WL.hubbard_quench(ctx, 4, 4, 5, 5, U0=4.0, U1=2.0, k_gs=120, k_rt=30, dt=0.05,
@@ -297,10 +305,8 @@ def main():
         del q
 
     if not args.no_traj:
+        torch.cuda.empty_cache()
         result["hubbard_l8_trajectories"] = trajectories(ctx, args)
-
-    if not args.no_serve:
-        result["serving"] = serving(ctx, args)
 
     if not args.no_extras:
         result["extras"] = extras(ctx)
