@@ -14,7 +14,13 @@
 //   - then acc = acc + c_k * v_k, a new entry starting from (+0, +0)
 //     (operator_abel.hpp:28-31 through Eigen's sparse sum).
 // Pass 1 counts the distinct targets per column; an exclusive scan gives the
-// column pointers; pass 2 writes rows ascending and values.
+// column pointers; pass 2 writes rows ascending and values.  When every
+// column has at most 32 distinct targets (the common case) one expansion is
+// enough: the fill pass writes each column's sorted list to a staging area
+// at a fixed stride of 32 together with its count, the scan gives the column
+// pointers, and a compaction copies the lists into place.  Same per-column
+// arithmetic, so the same bits; a longer column makes the staged pass flag an
+// overflow and the two-pass build runs instead.
 #include "qd_internal.hpp"
 
 // sorted per-thread list: find or insert key; returns position or -1 when full
@@ -51,7 +57,7 @@ struct ModeArgs {
   const cplx* mats;
 };
 
-template <int CAP, bool FILL, bool MODES>
+template <int CAP, bool FILL, bool MODES, bool STAGE = false>
 __global__ void __launch_bounds__(128)
     csr_rows_kernel(SectorDesc sec, const uint64_t* __restrict__ states, int64_t n,
                     const DevTerm* __restrict__ terms, int nterms, const DevOp* __restrict__ ops,
@@ -125,18 +131,34 @@ __global__ void __launch_bounds__(128)
   }
   if (overflow) {
     atomicOr(flag, 1);
-    if (!FILL) counts[i] = 0;
+    if (!FILL || STAGE) counts[i] = 0;
     return;
   }
   if (!FILL) {
     counts[i] = ne;
   } else {
-    int64_t base = offs[i];
+    if (STAGE) counts[i] = ne;
+    const int64_t base = STAGE ? i * CAP : offs[i];  // staging: fixed stride CAP per column
     for (int e = 0; e < ne; ++e) {
       inner[base + e] = key[e];
       vals[base + e] = acc[e];
     }
   }
+}
+
+// staged lists -> their final places (one thread per staging slot; slots
+// past a column's count are skipped)
+template <int CAP>
+__global__ void stage_compact_kernel(int64_t n, const int32_t* __restrict__ counts, const int64_t* __restrict__ offs,
+                                     const int32_t* __restrict__ s_inner, const cplx* __restrict__ s_vals,
+                                     int32_t* __restrict__ inner, cplx* __restrict__ vals) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * CAP) return;
+  const int64_t col = e / CAP;
+  const int k = (int)(e % CAP);
+  if (k >= counts[col]) return;
+  inner[offs[col] + k] = s_inner[e];
+  vals[offs[col] + k] = s_vals[e];
 }
 
 __global__ void max_count_kernel(const int32_t* __restrict__ c, int64_t n, int* __restrict__ out) {
@@ -217,6 +239,24 @@ static int pass_dispatch(int cap, qd_ctx* ctx, qd_basis* basis, int64_t n, const
   return pass_cap<false>(cap, ctx, basis, n, dterms, nterms, dops, fill, counts, offs, inner, vals, dflag, ma);
 }
 
+constexpr int STAGE_CAP = 32;
+// staging is used while it stays under this many bytes (the two-pass build
+// needs no staging)
+constexpr size_t STAGE_MAX_BYTES = (size_t)16 << 30;
+
+static int run_stage(qd_ctx* ctx, qd_basis* basis, int64_t n, const DevTerm* dterms, int nterms, const DevOp* dops,
+                     int32_t* counts, int32_t* s_inner, cplx* s_vals, int* dflag, ModeArgs ma) {
+  const int B = 128;
+  if (ma.mdim > 1)
+    csr_rows_kernel<STAGE_CAP, true, true, true><<<qd_grid(n, B), B, 0, ctx->stream>>>(
+        basis->sec, basis->d_states, n, dterms, nterms, dops, counts, nullptr, s_inner, s_vals, dflag, ma);
+  else
+    csr_rows_kernel<STAGE_CAP, true, false, true><<<qd_grid(n, B), B, 0, ctx->stream>>>(
+        basis->sec, basis->d_states, n, dterms, nterms, dops, counts, nullptr, s_inner, s_vals, dflag, ma);
+  QD_CHECK_LAUNCH(ctx);
+  return QD_OK;
+}
+
 int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoint, qd_csr** out) {
   *out = nullptr;
   int64_t n = (int64_t)basis->sec.total;
@@ -247,12 +287,16 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
   int32_t* counts = nullptr;
   int64_t* offs = nullptr;
   int* dflag = nullptr;
+  int32_t* s_inner = nullptr;
+  cplx* s_vals = nullptr;
   int rc = QD_OK;
   int hflag[2] = {0, 0};
   int64_t nnz = 0;
   int cap = 32;
   auto fail = [&](int code, const char* what) {
     hipStreamSynchronize(ctx->stream);
+    hipFree(s_inner);
+    hipFree(s_vals);
     hipFree(counts);
     hipFree(offs);
     hipFree(dflag);
@@ -267,6 +311,47 @@ int qd_csr_build_impl(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog, int adjoi
       hipMalloc(&offs, sizeof(int64_t) * (n + 1)) != hipSuccess || hipMalloc(&dflag, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_outer, sizeof(int32_t) * (n + 1)) != hipSuccess)
     return fail(QD_E_OOM, "counts");
+  // one expansion with staged lists, when the staging fits
+  const size_t stage_bytes = (size_t)n * STAGE_CAP * (sizeof(int32_t) + sizeof(cplx));
+  if (n > 0 && stage_bytes <= STAGE_MAX_BYTES && hipMalloc(&s_inner, sizeof(int32_t) * (size_t)n * STAGE_CAP) == hipSuccess &&
+      hipMalloc(&s_vals, sizeof(cplx) * (size_t)n * STAGE_CAP) == hipSuccess) {
+    hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream);
+    rc = run_stage(ctx, basis, n, dterms, nterms, dops, counts, s_inner, s_vals, dflag, ma);
+    if (rc) return fail(rc, "staged fill");
+    hipMemcpyAsync(hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(qd_set_error(ctx, QD_E_HIP, "staged fill"), "");
+    if (!hflag[0]) {
+      rc = qd_scan_counts(ctx, counts, n, offs);
+      if (rc) return fail(rc, "scan");
+      hipMemcpyAsync(&nnz, offs + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(qd_set_error(ctx, QD_E_HIP, "staged scan"), "");
+      if (nnz >= (int64_t)INT32_MAX)
+        return fail(qd_set_error(ctx, QD_E_UNSUPPORTED, "nnz %lld exceeds int32 storage indices", (long long)nnz), "");
+      c->nnz = nnz;
+      offs_to_outer<<<qd_grid(n + 1, 256), 256, 0, ctx->stream>>>(offs, n, c->d_outer);
+      if (hipMalloc(&c->d_inner, sizeof(int32_t) * (nnz ? nnz : 1)) != hipSuccess ||
+          hipMalloc(&c->d_vals, sizeof(cplx) * (nnz ? nnz : 1)) != hipSuccess)
+        return fail(QD_E_OOM, "values");
+      stage_compact_kernel<STAGE_CAP><<<qd_grid(n * STAGE_CAP, 256), 256, 0, ctx->stream>>>(
+          n, counts, offs, s_inner, s_vals, c->d_inner, c->d_vals);
+      QD_CHECK_LAUNCH(ctx);
+      hipStreamSynchronize(ctx->stream);
+      hipFree(s_inner);
+      hipFree(s_vals);
+      hipFree(counts);
+      hipFree(offs);
+      hipFree(dflag);
+      *out = c;
+      return QD_OK;
+    }
+    cap = 2 * STAGE_CAP;  // some column holds more than 32 targets: two passes from 64
+  } else if (n > 0 && stage_bytes <= STAGE_MAX_BYTES) {
+    (void)hipGetLastError();  // a failed staging allocation is not an error: two passes instead
+  }
+  hipFree(s_inner);
+  hipFree(s_vals);
+  s_inner = nullptr;
+  s_vals = nullptr;
   // pass 1 with growing capacity
   for (;;) {
     hipMemsetAsync(dflag, 0, 2 * sizeof(int), ctx->stream);

@@ -121,7 +121,24 @@ def _close(*hs):
         h.close()
 
 
+def _xy_all_to_all(N, seed):
+    """sum_{i<j} J_ij (S+_i S-_j + h.c.) with non-dyadic J_ij: at half filling a
+    column has (N/2)^2 off-diagonal targets, above the single-expansion build's
+    32 (its count-then-fill fallback runs)."""
+    rng = np.random.default_rng(seed)
+    idx = Q.getQbits(N)
+    gc, ga = Q.getBoseGate(idx), Q.getBoseGate(idx)
+    gc << 0.5 * (Q.pauli_x + Q.II * Q.pauli_y)
+    ga << 0.5 * (Q.pauli_x - Q.II * Q.pauli_y)
+    ham = Q.getEmptyOperator()
+    for i in range(N):
+        for j in range(i + 1, N):
+            ham = ham + float(rng.uniform(0.3, 1.7)) * (gc(j) * ga(i) + gc(i) * ga(j))
+    return ham, Q.Constrain(idx, Q.getNset(idx) == N // 2)
+
+
 CSR_CASES = {
+    "xy12_all_to_all": lambda: _xy_all_to_all(12, 3),
     "xy16_obc": lambda: (lambda i, h: (h, Q.Constrain(i, Q.getNset(i) == 8)))(*Q.xy_chain(16, pbc=False)),
     "xxz16_pbc": lambda: xxz(16)[1:],
     "xxz20_pbc": lambda: xxz(20)[1:],
