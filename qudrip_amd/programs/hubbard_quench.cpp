@@ -30,7 +30,7 @@ using cplx = std::complex<double>;
 
 template <typename TG, typename TI>
 SpMatrix lindblad(int i, int Nsite, TG& hopgate, TI& index) {  // hubbard_diss.cpp:15-18
-  return (hopgate(i, i) + hopgate(i + Nsite, i + Nsite) >> index);
+  return ((hopgate(i, i) + hopgate(i + Nsite, i + Nsite)) >> index);
 }
 
 void QJ_operator(SpMatrix& H, const SpMatrix& qj_tot, const SpMatrix& Id, double wt) {  // :514-517
@@ -121,7 +121,7 @@ int main(int argc, char** argv) {
     h0 += (nupg + ndog) >> hubbard;
     auto ni = lindblad(site, Nsite, gate, hubbard);
     qj_tot += gamma * SpMatrix(ni.adjoint()) * ni;
-    Pc = Pc * (Id - (nupg + ndog >> hubbard) + 2. * (nupg * ndog >> hubbard));
+    Pc = Pc * (Id - ((nupg + ndog) >> hubbard) + 2. * ((nupg * ndog) >> hubbard));
   }
   dump_matrix(out + ".qj", qj_tot);
   dump_matrix(out + ".pc", Pc);
