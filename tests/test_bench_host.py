@@ -1,0 +1,54 @@
+"""Host logic of bench.py that needs no GPU: the traffic figure is taken from
+a counter record only while the kernel source is the one the record names,
+and the CPU description of cpu_baseline."""
+import hashlib
+import os
+
+import bench
+
+
+def _tree(tmp_path, src, record_sha):
+    os.makedirs(tmp_path / "qudrip_amd" / "csrc")
+    os.makedirs(tmp_path / "profiles")
+    (tmp_path / "qudrip_amd" / "csrc" / "spmv.hip").write_text(src)
+    (tmp_path / "profiles" / "spmv_xxz24_traffic.txt").write_text(
+        "header line\n"
+        f"traffic-record: kernel=k<8> bytes_per_launch=123456789 spmv.hip-sha256={record_sha}\n")
+
+
+def test_traffic_taken_from_a_matching_record(tmp_path, monkeypatch):
+    src = "// kernel source\n"
+    _tree(tmp_path, src, hashlib.sha256(src.encode()).hexdigest())
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    traffic, note = bench.spmv_traffic()
+    assert traffic == 123456789.0
+    assert "same spmv.hip" in note and "k<8>" in note
+
+
+def test_traffic_null_when_the_source_changed(tmp_path, monkeypatch):
+    _tree(tmp_path, "// edited source\n", hashlib.sha256(b"// the profiled source\n").hexdigest())
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    traffic, note = bench.spmv_traffic()
+    assert traffic is None and "another spmv.hip" in note
+
+
+def test_traffic_null_without_a_record(tmp_path, monkeypatch):
+    os.makedirs(tmp_path / "qudrip_amd" / "csrc")
+    (tmp_path / "qudrip_amd" / "csrc" / "spmv.hip").write_text("x")
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    traffic, note = bench.spmv_traffic()
+    assert traffic is None and "no counter summary" in note
+
+
+def test_committed_record_parses():
+    """The committed record line carries the three fields bench.py reads."""
+    line = [ln for ln in open(os.path.join(bench.ROOT, bench.TRAFFIC_PROFILE)) if ln.startswith("traffic-record:")]
+    assert len(line) == 1
+    kv = dict(f.split("=", 1) for f in line[0].split()[1:])
+    assert float(kv["bytes_per_launch"]) > 0 and len(kv["spmv.hip-sha256"]) == 64 and kv["kernel"]
+
+
+def test_host_cpu_fields():
+    h = bench.host_cpu()
+    assert set(h) == {"cpu_model", "nproc", "usable_threads"}
+    assert h["nproc"] >= 1 and 1 <= h["usable_threads"] <= h["nproc"]
