@@ -42,6 +42,25 @@ struct KronDesc {
   // at most 2 * KRON_WP entries): entry = col << 3 | value id (id 0 pads),
   // two entries per word, word p of row a at ell16[p * nA + a]
   const uint32_t* ell16;
+  // T_A for the branch-free rows kernel (nA < 8191, same limits as ell16):
+  // entry = (col + 1) << 3 | low, low = the value id, or in the sign form
+  // (asign: every table value is +acoef or -acoef) bit 2 set for -acoef;
+  // entry 0 pads (slot 0 of the kernel's row buffer holds 0); two entries
+  // per word, word p of row a at ell16s[p * nA + a]
+  const uint32_t* ell16s;
+  int asign;
+  cplx acoef;
+  // T_B for the wave-per-row slab kernel (nA nB 16 < 2^32): row b at
+  // ellB4[b * ellB4_stride]: the entry count, then per entry the byte offset
+  // in x of its source row (col nA 16) | a code in the low 4 bits: in the
+  // sign form (bsign: every value is +bcoef or -bcoef) 1 for +bcoef, 2 for
+  // -bcoef; otherwise an id into vtabB4; code 0 (offset 0) pads the row to
+  // ellB4_stride - 1 entries, a multiple of 8
+  const uint32_t* ellB4;
+  int ellB4_stride;
+  int bsign;
+  cplx bcoef;
+  const cplx* vtabB4;
   // per product: an extra diagonal (n values, state b nA + a), or nullptr
   const cplx* ext_diag;
 };

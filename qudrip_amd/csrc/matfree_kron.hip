@@ -55,10 +55,13 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 struct qd_kron {
   uint32_t* d_ellB = nullptr;
   cplx* d_vtabB = nullptr;
+  uint32_t* d_ellB4 = nullptr;
+  cplx* d_vtabB4 = nullptr;
   uint64_t* d_nmask = nullptr;
   cplx* d_ncoef = nullptr;
   uint32_t* d_ell = nullptr;
   uint32_t* d_ell16 = nullptr;
+  uint32_t* d_ell16s = nullptr;
   cplx* d_vtab = nullptr;
   qd_basis* basisA = nullptr;
   qd_basis* basisB = nullptr;
@@ -501,6 +504,230 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// x entry of a row buffer at byte offset `off` (LDS)
+__device__ __forceinline__ kdbl2 lds_x(const cplx* buf, uint32_t off) {
+  return *reinterpret_cast<const kdbl2*>(reinterpret_cast<const char*>(buf) + off);
+}
+// d with its sign bit flipped where bit 31 of s is set
+__device__ __forceinline__ double flip_sign(double d, uint32_t s) {
+  return __longlong_as_double(__double_as_longlong(d) ^ ((long long)s << 32));
+}
+
+// Branch-free persistent rows kernel.  Same row walk as kron_rows3_kernel
+// (one block per CU over a contiguous range of rows, two LDS row buffers,
+// the T_A words of a thread's S outputs a = t + 1024 i in registers), with
+// these changes:
+//  - every entry slot is evaluated: a pad word (0) points at slot 0 of the
+//    row buffer, which holds 0 (the row itself sits at slots 1..nA), so no
+//    entry needs a branch.  A pad adds +0: the result differs from skipping
+//    it at most in the sign of a zero;
+//  - the S * 2 WP entry slots of a thread's row are one unrolled sequence
+//    taken in groups of G: the gathers of group g + 1 are issued before the
+//    sums of group g (PIPE), so an LDS wait covers G gathers and overlaps
+//    G sums;
+//  - SIGN: when every value of T_A is +c or -c for one c (hopping with one
+//    amplitude), the entry carries the sign in bit 2 instead of a table id,
+//    the row sums sum_e (+/-) x[col_e] and multiplies by conj(c) once per
+//    output -- one LDS read and two additions per entry instead of two reads
+//    and four multiply-adds;
+//  - DMA: the next row goes from HBM straight into the free LDS buffer
+//    (global_load_lds_dwordx4, 64 entries per wave and instruction) while
+//    the current row is computed, instead of through registers.
+// The number-product diagonal uses 32-bit sets (at most 32 products).
+// Entries are summed in order with fused multiply-adds, so the rounding
+// differs from the other rows kernels by a few units in the last place.
+template <int WP, int S, bool SIGN, bool GENDIAG, int G, bool PIPE, bool DMA>
+__global__ void __launch_bounds__(1024)
+    kron_rows4_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift, int rows_per_block) {
+  extern __shared__ cplx lds[];
+  cplx* vt = lds;      // 8 values
+  cplx* nc = lds + 8;  // 32 number-product coefficients
+  uint64_t* nmB = reinterpret_cast<uint64_t*>(lds + 40);  // their block-B masks
+  const int nA = (int)k.nA;
+  cplx* buf0 = lds + 56;  // two row buffers of 1 + nA: slot 0 = 0, then the row
+  cplx* buf1 = buf0 + nA + 1;
+  const int64_t b0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t b1 = min<int64_t>(k.nB, b0 + rows_per_block);
+  if (b0 >= b1) return;  // block-uniform
+  const int t = threadIdx.x;
+  const int nnum = GENDIAG ? 0 : k.nnum;  // <= 32 when !GENDIAG
+  const uint64_t amask = k.loB >= 64 ? ~0ull : ((1ull << k.loB) - 1ull);
+  if (t < 8) vt[t] = t < k.nvtab ? k.vtab[t] : cmake(0.0, 0.0);
+  if (t < nnum) {
+    nc[t] = k.ncoef[t];
+    nmB[t] = k.nmask[t] >> k.loB;
+  }
+  if (t == 0) {
+    buf0[0] = cmake(0.0, 0.0);
+    buf1[0] = cmake(0.0, 0.0);
+  }
+  // row r of x into slots 1..nA of buf: 64 entries per wave and DMA
+  // instruction (lane l writes 16 bytes at the wave's LDS base + 16 l)
+  auto dma_row = [&](int64_t r, cplx* buf) {
+    const cplx* __restrict__ xr = x + r * nA;
+    const int lane = t & 63;
+    for (int q = t >> 6; q * 64 < nA; q += 16) {
+      const int c = q * 64 + lane;
+      if (c < nA)
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(xr + c),
+                                         (__attribute__((address_space(3))) void*)(buf + 1 + q * 64), 16, 0, 0);
+    }
+  };
+  if (DMA) {
+    dma_row(b0, buf0);
+  } else {
+    const cplx* __restrict__ xb = x + b0 * nA;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int c = t + 1024 * i;
+      if (c < nA) buf0[1 + c] = xb[c];
+    }
+  }
+  uint32_t e[S][WP];
+  uint32_t sa[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int a = t + 1024 * i;
+#pragma unroll
+    for (int p = 0; p < WP; ++p) e[i][p] = a < nA ? k.ell16s[p * nA + a] : 0u;
+    const uint64_t st = k.statesA[a < nA ? a : nA - 1];
+    if (GENDIAG) {
+      sa[i] = (uint32_t)(a < nA ? a : nA - 1);  // the state is read again when needed
+    } else {
+      uint32_t f = 0;
+      for (int q = 0; q < nnum; ++q) {
+        const uint64_t m = k.nmask[q] & amask;
+        if ((st & m) == m) f |= 1u << q;
+      }
+      sa[i] = f;
+    }
+  }
+  const cplx ac = k.acoef;
+  if (DMA) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's row copies have landed
+  __syncthreads();
+  kdbl2 pre[DMA ? 1 : S];
+  if (!DMA) {
+    const kdbl2* __restrict__ xn = reinterpret_cast<const kdbl2*>(x + min<int64_t>(b0 + 1, b1 - 1) * nA);
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int c = t + 1024 * i;
+      pre[i] = xn[c < nA ? c : nA - 1];
+    }
+  }
+  constexpr int NSL = 2 * WP;  // entry slots per output
+  constexpr int TOT = S * NSL;
+  constexpr int NG = (TOT + G - 1) / G;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int par = (int)((b - b0) & 1);
+    const cplx* cur = par ? buf1 : buf0;
+    cplx* nxt = par ? buf0 : buf1;
+    const uint64_t sbb = k.statesB[b];
+    if (DMA) {
+      if (b + 1 < b1) dma_row(b + 1, nxt);  // nxt's last reader, row b - 1, ended before the barrier
+    } else {
+      if (b + 1 < b1) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const int c = t + 1024 * i;
+          if (c < nA) nxt[1 + c] = cmake(pre[i].x, pre[i].y);
+        }
+      }
+      const kdbl2* __restrict__ xn = reinterpret_cast<const kdbl2*>(x + min<int64_t>(b + 2, b1 - 1) * nA);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int c = t + 1024 * i;
+        pre[i] = xn[c < nA ? c : nA - 1];
+      }
+    }
+    uint32_t g = 0;  // number products whose B bits row b has
+    if (!GENDIAG) {
+#pragma unroll 8
+      for (int q = 0; q < nnum; ++q)
+        if ((sbb & nmB[q]) == nmB[q]) g |= 1u << q;
+    }
+    const uint64_t sB = sbb << k.loB;
+    // decoded afresh for every row: offsets hoisted out of the row loop
+    // would hold 2 S WP registers more
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int p = 0; p < WP; ++p) asm volatile("" : "+v"(e[i][p]));
+    kdbl2 xr[2][G];
+    double sx = 0.0, sy = 0.0;
+    // gathers of slot group gg into register set gg & 1
+#define QD_ROWS4_ISSUE(gg)                                                   \
+  _Pragma("unroll") for (int j = 0; j < G; ++j) {                            \
+    const int sl = (gg) * G + j;                                             \
+    if (sl < TOT) {                                                          \
+      const uint32_t w = e[sl / NSL][(sl % NSL) >> 1];                       \
+      const uint32_t hw = (sl & 1) ? (w >> 16) : (w & 0xFFFFu);              \
+      xr[(gg) & 1][j] = lds_x(cur, (hw >> 3) << 4);                          \
+    }                                                                        \
+  }
+    QD_ROWS4_ISSUE(0)
+#pragma unroll
+    for (int gi = 0; gi < NG; ++gi) {
+      if (PIPE && gi + 1 < NG) {
+        QD_ROWS4_ISSUE(gi + 1)
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int sl = gi * G + j;
+        if (sl < TOT) {
+          const int i = sl / NSL;
+          const uint32_t w = e[i][(sl % NSL) >> 1];
+          const uint32_t hw = (sl & 1) ? (w >> 16) : (w & 0xFFFFu);
+          const kdbl2 xv = xr[gi & 1][j];
+          if (SIGN) {
+            const uint32_t s = (hw << 29) & 0x80000000u;  // bit 2: the entry is -c
+            sx += flip_sign(xv.x, s);
+            sy += flip_sign(xv.y, s);
+          } else {
+            const cplx v = vt[hw & 7u];
+            sx = __builtin_fma(v.x, xv.x, sx);  // conj(v) * x
+            sx = __builtin_fma(v.y, xv.y, sx);
+            sy = __builtin_fma(v.x, xv.y, sy);
+            sy = __builtin_fma(-v.y, xv.x, sy);
+          }
+          if (sl % NSL == NSL - 1) {  // output i complete
+            const int a = t + 1024 * i;
+            asm volatile("" : "+v"(sx), "+v"(sy));
+            if (SIGN) {
+              const double rx = ac.x * sx + ac.y * sy;  // conj(c) * sum
+              const double ry = ac.x * sy - ac.y * sx;
+              sx = rx;
+              sy = ry;
+            }
+            if (a < nA) {
+              cplx d;
+              if (GENDIAG) {
+                d = diag_value(k, sB | k.statesA[sa[i]]);
+              } else {
+                d = cmake(0.0, 0.0);
+                for (uint32_t m = sa[i] & g; m; m &= m - 1) d = cadd(d, nc[__builtin_ctz(m)]);
+              }
+              d = cadd(d, shift);
+              if (k.ext_diag) d = cadd(d, k.ext_diag[b * nA + a]);
+              const cplx dx = cmul(d, cur[1 + a]);
+              y_st(y + b * nA + a, sx + dx.x, sy + dx.y);
+            }
+            sx = 0.0;
+            sy = 0.0;
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (!PIPE && gi + 1 < NG) {
+        QD_ROWS4_ISSUE(gi + 1)
+      }
+    }
+#undef QD_ROWS4_ISSUE
+    if (DMA) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the next row has landed
+    __syncthreads();
+  }
+}
+
 // Slab kernel with the block's rows of packed T_B staged in LDS (one
 // coalesced read) and U gathers of X row segments in flight per step; same
 // summation order as kron_slab_packed_kernel.
@@ -618,6 +845,75 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Slab pass with one wave per (row b, 64-column slab): the entries of T_B's
+// row b are the same for every lane, so they sit in scalar registers (one
+// scalar load of the whole row) and every entry is one coalesced 1 KB load
+// of the x row segment, addressed as the entry's byte offset (scalar) plus
+// the lane's column.  Entries go in groups of U, a group at a time (a
+// uniform branch per group, none per entry; pad entries of a group point at
+// row 0 with coefficient 0).  SIGN: every T_B value is +c or -c, the entry
+// code (+1, -1, 0) is a scalar multiplier and conj(c) scales the row sum;
+// otherwise the code is a value id (0 pads).  y of the row is requested
+// before its gathers.
+template <bool SIGN, int U, int NE>
+__global__ void __launch_bounds__(256)
+    kron_slab4_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, int chunks,
+                      int rows_per_chunk) {
+  const int xcd = blockIdx.x & 7;
+  const int kk = blockIdx.x >> 3;
+  const int64_t slab = (int64_t)xcd + 8 * (kk / chunks);
+  const int chunk = kk % chunks;
+  const int64_t a0 = slab * 64;
+  const int64_t rb = (int64_t)chunk * rows_per_chunk;
+  const int64_t re = min<int64_t>(k.nB, rb + rows_per_chunk);
+  if (a0 >= k.nA || rb >= re) return;  // uniform over the block
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t a = min<int64_t>(a0 + lane, k.nA - 1);  // clamp: duplicate lanes recompute the last column
+  const bool col_ok = a0 + lane < k.nA;
+  const char* __restrict__ xa = reinterpret_cast<const char*>(x + a);
+  const int stride = k.ellB4_stride;  // NE + 1
+  const cplx bc = k.bcoef;
+  for (int64_t b = rb + wv; b < re; b += 4) {
+    const uint32_t* __restrict__ er = k.ellB4 + b * stride;
+    uint32_t ent[NE + 1];
+#pragma unroll
+    for (int j = 0; j <= NE; ++j) ent[j] = er[j];
+    const int ne = (int)ent[0];
+    const cplx o = y_ld(y + b * k.nA + a);
+    double px = 0.0, py = 0.0;
+#pragma unroll
+    for (int g = 0; g < NE; g += U) {
+      if (g < ne) {
+        kdbl2 xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = *reinterpret_cast<const kdbl2*>(xa + (ent[1 + g + u] & ~15u));
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t code = ent[1 + g + u] & 15u;
+          if (SIGN) {
+            const double sg = code == 1u ? 1.0 : (code == 2u ? -1.0 : 0.0);
+            px = __builtin_fma(sg, xv[u].x, px);
+            py = __builtin_fma(sg, xv[u].y, py);
+          } else {
+            const cplx v = k.vtabB4[code];
+            px = __builtin_fma(v.x, xv[u].x, px);  // conj(v) * x
+            px = __builtin_fma(v.y, xv[u].y, px);
+            py = __builtin_fma(v.x, xv[u].y, py);
+            py = __builtin_fma(-v.y, xv[u].x, py);
+          }
+        }
+      }
+    }
+    double rx = px, ry = py;
+    if (SIGN) {
+      rx = bc.x * px + bc.y * py;  // conj(c) * sum
+      ry = bc.x * py - bc.y * px;
+    }
+    if (col_ok) y_st(y + b * k.nA + a, o.x + rx, o.y + ry);
+  }
+}
+
 // ------------------------------------------------------------------ host
 // pack a pull-form block matrix: entries (col << 16 | value id); row-major
 // when rowmajor, else entry-major (ELL column j contiguous over rows)
@@ -661,6 +957,67 @@ static bool pack_block(qd_ctx* ctx, const qd_csr* c, int64_t n, bool rowmajor, i
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
   *width = w;
   *ntab = (int)tab.size();
+  return true;
+}
+
+// pack a pull-form block matrix for kron_slab4_kernel (rows of uniform
+// entries): per row the entry count, then the byte offsets of the source
+// rows of x (row length `rowbytes`) | a code: +1 / -1 (1, 2) in the sign
+// form, a value id (1..15) otherwise; pads (code 0, offset 0) fill the row to
+// a multiple of 8 entries
+static bool pack_block_rows(qd_ctx* ctx, const qd_csr* c, int64_t n, uint64_t rowbytes, KronDesc* d,
+                            uint32_t** d_ell, cplx** d_tab) {
+  if (n >= 65536 || rowbytes % 16 != 0 || rowbytes * (uint64_t)n >= (1ull << 32)) return false;
+  int64_t nnz = 0;
+  qd_csr_info(c, nullptr, &nnz, nullptr);
+  std::vector<int32_t> ao(n + 1), ai(nnz);
+  std::vector<cplx> av(nnz);
+  if (qd_csr_download(const_cast<qd_csr*>(c), ao.data(), ai.data(), (double*)av.data()) != QD_OK) return false;
+  int w = 0;
+  for (int64_t r = 0; r < n; ++r) w = std::max(w, ao[r + 1] - ao[r]);
+  if (w > 32) return false;
+  bool sign = nnz > 0;
+  const cplx c0 = nnz > 0 ? av[0] : cmake(0.0, 0.0);
+  for (int64_t e = 0; e < nnz && sign; ++e)
+    sign = (av[e].x == c0.x && av[e].y == c0.y) || (av[e].x == -c0.x && av[e].y == -c0.y);
+  std::vector<cplx> tab(1, cmake(0.0, 0.0));  // id 0 pads
+  std::vector<uint32_t> code(nnz, 0u);
+  for (int64_t e = 0; e < nnz; ++e) {
+    if (sign) {
+      code[e] = (av[e].x == c0.x && av[e].y == c0.y) ? 1u : 2u;
+      continue;
+    }
+    size_t q = 1;
+    for (; q < tab.size(); ++q)
+      if (std::memcmp(&tab[q], &av[e], sizeof(cplx)) == 0) break;
+    if (q == tab.size()) {
+      if (tab.size() >= 16) return false;
+      tab.push_back(av[e]);
+    }
+    code[e] = (uint32_t)q;
+  }
+  const int stride = 1 + std::max(8, (w + 7) / 8 * 8);
+  std::vector<uint32_t> ell((size_t)stride * n, 0u);
+  for (int64_t r = 0; r < n; ++r) {
+    uint32_t* row = ell.data() + (size_t)r * stride;
+    row[0] = (uint32_t)(ao[r + 1] - ao[r]);
+    for (int32_t e = ao[r]; e < ao[r + 1]; ++e)
+      row[1 + (e - ao[r])] = (uint32_t)((uint64_t)ai[e] * rowbytes) | code[e];
+  }
+  if (hipMalloc(d_ell, sizeof(uint32_t) * ell.size()) != hipSuccess) return false;
+  if (hipMalloc(d_tab, sizeof(cplx) * tab.size()) != hipSuccess) {
+    hipFree(*d_ell);
+    *d_ell = nullptr;
+    return false;
+  }
+  hipMemcpyAsync(*d_ell, ell.data(), sizeof(uint32_t) * ell.size(), hipMemcpyHostToDevice, ctx->stream);
+  hipMemcpyAsync(*d_tab, tab.data(), sizeof(cplx) * tab.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
+  d->ellB4 = *d_ell;
+  d->ellB4_stride = stride;
+  d->bsign = sign ? 1 : 0;
+  d->bcoef = c0;
+  d->vtabB4 = *d_tab;
   return true;
 }
 
@@ -909,6 +1266,41 @@ qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
             if (hipStreamSynchronize(ctx->stream) == hipSuccess) d.ell16 = k->d_ell16;
           }
         }
+        // the branch-free form: columns shifted by one (0 pads), and the sign
+        // form when every nonzero table value is +c or -c
+        d.ell16s = nullptr;
+        d.asign = 0;
+        d.acoef = cmake(0.0, 0.0);
+        if (d.ell16 && d.nA + 1 < 8192) {
+          bool sign = tab.size() >= 2;
+          const cplx c = sign ? tab[1] : cmake(0.0, 0.0);
+          std::vector<uint32_t> low(tab.size(), 0u);
+          for (size_t q = 1; q < tab.size() && sign; ++q) {
+            if (tab[q].x == c.x && tab[q].y == c.y)
+              low[q] = 0u;
+            else if (tab[q].x == -c.x && tab[q].y == -c.y)
+              low[q] = 4u;
+            else
+              sign = false;
+          }
+          std::vector<uint32_t> e16s((size_t)KRON_WP * d.nA, 0u);
+          for (int64_t a = 0; a < d.nA; ++a)
+            for (int j = 0; j < w; ++j) {
+              const int32_t e = ao[a] + j;
+              if (e >= ao[a + 1]) break;
+              const uint32_t v = (((uint32_t)ai[e] + 1u) << 3) | (sign ? low[vid[e]] : vid[e]);
+              e16s[(size_t)(j / 2) * d.nA + a] |= v << (16 * (j & 1));
+            }
+          if (hipMalloc(&k->d_ell16s, sizeof(uint32_t) * e16s.size()) == hipSuccess) {
+            hipMemcpyAsync(k->d_ell16s, e16s.data(), sizeof(uint32_t) * e16s.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+            if (hipStreamSynchronize(ctx->stream) == hipSuccess) {
+              d.ell16s = k->d_ell16s;
+              d.asign = sign ? 1 : 0;
+              d.acoef = c;
+            }
+          }
+        }
       }
     }
   }
@@ -917,6 +1309,8 @@ qd_kron* qd_kron_try_build(qd_ctx* ctx, qd_basis* basis, qd_termprog* prog) {
     d.ellB = k->d_ellB;
     d.vtabB = k->d_vtabB;
   }
+  d.ellB4 = nullptr;
+  pack_block_rows(ctx, useB, d.nB, (uint64_t)d.nA * sizeof(cplx), &d, &k->d_ellB4, &k->d_vtabB4);
   return k;
 }
 
@@ -927,11 +1321,14 @@ void qd_kron_destroy(qd_kron* k) {
   if (!k) return;
   hipFree(k->d_ell);
   hipFree(k->d_ell16);
+  hipFree(k->d_ell16s);
   hipFree(k->d_vtab);
   hipFree(k->d_nmask);
   hipFree(k->d_ncoef);
   hipFree(k->d_ellB);
   hipFree(k->d_vtabB);
+  hipFree(k->d_ellB4);
+  hipFree(k->d_vtabB4);
   qd_csr_destroy(k->csrA);
   qd_csr_destroy(k->csrB);
   qd_termprog_destroy(k->progA);
@@ -1038,13 +1435,67 @@ bool launch_rows3_w(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* 
   return launch_rows3_d<12, S>(d, optin, ncu, x, y, shift, st);
 }
 
+// branch-free persistent rows kernel (kron_rows4_kernel), G entries per
+// LDS wait (0: compiler order)
+template <int WP, int S, bool SG, bool GD, int G, bool PI, bool DM>
+bool launch_rows4(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const size_t lds = sizeof(cplx) * (56 + 2 * (size_t)(d.nA + 1));
+  if (lds > (size_t)optin) return false;
+  const void* fn = (const void*)kron_rows4_kernel<WP, S, SG, GD, G, PI, DM>;
+  static std::once_flag once;
+  std::call_once(once, [fn, optin]() { hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, optin); });
+  const int per_xcd = xcd_resident(fn, lds, ncu, 1024);
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(d.nB, (int64_t)per_xcd * 8));
+  const int rpb = (int)((d.nB + grid - 1) / grid);
+  const unsigned g = (unsigned)((d.nB + rpb - 1) / rpb);
+  kron_rows4_kernel<WP, S, SG, GD, G, PI, DM><<<g, 1024, lds, st>>>(d, x, y, shift, rpb);
+  return true;
+}
+
+template <int WP, int S, int G, bool PI, bool DM>
+bool launch_rows4_d(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const bool gd = d.ndterms > 0 || d.nnum > 32;
+  if (d.asign) return gd ? launch_rows4<WP, S, true, true, G, PI, DM>(d, optin, ncu, x, y, shift, st)
+                         : launch_rows4<WP, S, true, false, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+  return gd ? launch_rows4<WP, S, false, true, G, PI, DM>(d, optin, ncu, x, y, shift, st)
+            : launch_rows4<WP, S, false, false, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+}
+
+template <int S, int G, bool PI, bool DM>
+bool launch_rows4_w(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const int wp = (d.ell_w + 1) / 2;
+  if (wp <= 8) return launch_rows4_d<8, S, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+  if (wp <= 10) return launch_rows4_d<10, S, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+  return launch_rows4_d<12, S, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+}
+
+template <int G, bool PI, bool DM>
+bool launch_rows4_s(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  if (!d.ell16s) return false;
+  switch ((d.nA + 1023) / 1024) {
+    case 1: return launch_rows4_w<1, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+    case 2: return launch_rows4_w<2, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+    case 3: return launch_rows4_w<3, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+    case 4: return launch_rows4_w<4, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+    case 5: return launch_rows4_w<5, G, PI, DM>(d, optin, ncu, x, y, shift, st);
+    default: return false;
+  }
+}
+
 // rows pass (writes y): variant 0 = kron_rows_packed<2>, 1 = rows2<2,8>,
 // 2 = rows2<1,8>, 3 = rows2<4,4>, 4 = the persistent rows3 (T_A in
-// registers; needs the compressed table); a variant whose LDS does not fit
-// returns false.  Without the packed table: kron_rows_kernel.
-constexpr int KRON_ROWS_VARIANTS = 5;
+// registers; needs the compressed table), 5-8 = the branch-free rows4 (needs
+// its table): 5 groups of 4 entries, rows staged through registers; 6 and 7
+// groups of 4 and 6, pipelined, rows copied straight to LDS; 8 groups of 8,
+// rows copied to LDS; a variant whose LDS does not fit returns false.
+// Without the packed table: kron_rows_kernel.
+constexpr int KRON_ROWS_VARIANTS = 9;
 bool launch_rows(const KronDesc& d, int v, int optin, int ncu, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
   if (v < 0 || v >= KRON_ROWS_VARIANTS) return false;
+  if (v == 5) return launch_rows4_s<4, false, false>(d, optin, ncu, x, y, shift, st);
+  if (v == 6) return launch_rows4_s<4, true, true>(d, optin, ncu, x, y, shift, st);
+  if (v == 7) return launch_rows4_s<6, true, true>(d, optin, ncu, x, y, shift, st);
+  if (v == 8) return launch_rows4_s<8, false, true>(d, optin, ncu, x, y, shift, st);
   if (v == 4) {
     if (!d.ell16) return false;
     switch ((d.nA + 1023) / 1024) {  // row loads per thread when staging
@@ -1085,13 +1536,41 @@ bool launch_rows(const KronDesc& d, int v, int optin, int ncu, const cplx* x, cp
   return true;
 }
 
+// wave-per-row slab kernel: 64-column slabs, one slab per XCD at a time
+template <int U, int NE>
+bool launch_slab4_n(const KronDesc& d, int ncu, const cplx* x, cplx* y, hipStream_t st) {
+  const void* fn =
+      d.bsign ? (const void*)kron_slab4_kernel<true, U, NE> : (const void*)kron_slab4_kernel<false, U, NE>;
+  const SlabShape sh = slab_shape(d, 64, xcd_resident(fn, 0, ncu));
+  if (d.bsign)
+    kron_slab4_kernel<true, U, NE><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk);
+  else
+    kron_slab4_kernel<false, U, NE><<<sh.grid, 256, 0, st>>>(d, x, y, sh.chunks, sh.rows_per_chunk);
+  return true;
+}
+
+template <int U>
+bool launch_slab4(const KronDesc& d, int ncu, const cplx* x, cplx* y, hipStream_t st) {
+  if (!d.ellB4) return false;
+  const int ne = d.ellB4_stride - 1;  // entry slots per row (a multiple of 8)
+  if (ne <= 8) return launch_slab4_n<U, 8>(d, ncu, x, y, st);
+  if (ne <= 16) return launch_slab4_n<U, 16>(d, ncu, x, y, st);
+  if (ne <= 24) return launch_slab4_n<U, 24>(d, ncu, x, y, st);
+  if (ne <= 32) return launch_slab4_n<U, 32>(d, ncu, x, y, st);
+  return false;
+}
+
 // slab pass (adds into y): variant 0 = kron_slab_packed<32,4>, 1 =
 // slab2<32,8>, 2 = slab2<32,16>, each at 128 chunks per slab; 3 = slab2<32,8>,
 // 4 = slab2<32,16>, 5 = slab3<32,8> and 6 = slab3<32,4> with one slab per
-// XCD at a time.  Without the packed table: kron_slab_kernel.
-constexpr int KRON_SLAB_VARIANTS = 7;
+// XCD at a time; 7 = slab4<8> and 8 = slab4<4> (one wave per row of a
+// 64-column slab; needs its table).  Without the packed table:
+// kron_slab_kernel.
+constexpr int KRON_SLAB_VARIANTS = 9;
 
 bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipStream_t st) {
+  if (v == 7) return launch_slab4<8>(d, ncu, x, y, st);
+  if (v == 8) return launch_slab4<4>(d, ncu, x, y, st);
   if (!d.ellB) {
     if (v != 0) return false;
     const SlabShape sh = slab_shape(d, KRON_SLAB_W);

@@ -478,15 +478,37 @@ def test_matfree_structured_matches_oracle(ctx, case):
         _close(mf, p, b)
 
 
-@pytest.mark.parametrize("nup,ndn", [(7, 1), (1, 7)])
-def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
+def _chain_two_amplitudes(nsite, nup, ndn, t1, t2, U):
+    """Hubbard ring whose bonds alternate between hopping amplitudes t1 and
+    t2: the block operators hold more than one value up to sign."""
+    idx = Q.getQbits(2 * nsite)
+    gate = Q.getHopFermiGate(idx)
+    ham = Q.getEmptyOperator()
+    for j, (a, b) in enumerate(Q.hubbard_bonds_chain(nsite, True)):
+        t = t1 if j % 2 == 0 else t2
+        for off in (0, nsite):
+            ham = ham + (-t) * gate(a + off, b + off) + (-t) * gate(b + off, a + off)
+    for i in range(nsite):
+        ham = ham + U * (gate(i, i) * gate(i + nsite, i + nsite))
+    return ham, Q.hubbard_sector(idx, nsite, nup, ndn)
+
+
+@pytest.mark.parametrize("nup,ndn,amps", [(7, 1, 1), (1, 7, 1), (7, 1, 2)])
+def test_matfree_structured_every_variant(ctx, nup, ndn, amps, monkeypatch):
     """Every rows-pass and slab-pass variant of the two-block product, fixed
-    through QD_KRON_VARIANTS, against the oracle (rows variant 4 splits the
-    3432 columns into four groups of blocks): a 14-site chain with one
-    block of 3432 states (more than 128 chunks of 8 rows, so the slab
+    through QD_KRON_VARIANTS, against the oracle (rows variants 4 to 8
+    split the 3432 columns into four groups of blocks): a 14-site ring with
+    one block of 3432 states (more than 128 chunks of 8 rows, so the slab
     variants that size chunks to one XCD's resident blocks differ from the
-    128-chunk ones) and one of 14."""
-    ham, sec = hub(14, Q.hubbard_bonds_chain(14, True), nup, ndn, U=3.0)[1:]
+    128-chunk ones; 54 slabs of 64 columns for slab variants 7 and 8) and
+    one of 14 (one partial 64-column slab).  One hopping amplitude
+    (amps = 1) puts rows variants 5-8 and slab variants 7-8 in their sign
+    form (every block value +c or -c); two alternating amplitudes
+    (amps = 2) in their value-table form."""
+    if amps == 1:
+        ham, sec = hub(14, Q.hubbard_bonds_chain(14, True), nup, ndn, U=3.0)[1:]
+    else:
+        ham, sec = _chain_two_amplitudes(14, nup, ndn, 1.0, 0.55, 3.0)
     basis = oracle.basis_scan(sec)
     ref = oracle.build_csc(basis, ham)
     x = oracle.seeded_state(len(basis), 23)
@@ -496,8 +518,8 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
     xd = to_dev(ctx, x)
     ran = []
     try:
-        for rv in range(5):
-            for sv in range(7):
+        for rv in range(9):
+            for sv in range(9):
                 monkeypatch.setenv("QD_KRON_VARIANTS", f"{rv},{sv}")
                 mf = LinOp.matfree(ctx, b, p)
                 try:
@@ -513,8 +535,8 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, monkeypatch):
                 ran.append((rv, sv))
     finally:
         _close(p, b)
-    assert {s for r, s in ran} == set(range(7)) and (0, 0) in ran
-    assert 4 in {r for r, s in ran}  # the persistent rows kernel (T_A in registers)
+    assert {s for r, s in ran} == set(range(9)) and (0, 0) in ran
+    assert {4, 5, 6, 7, 8} <= {r for r, s in ran}  # the persistent rows kernels (T_A in registers)
 
 
 @pytest.mark.parametrize("method", [0, 1])
