@@ -728,6 +728,136 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// Fused two-block product for small blocks (nA <= 1024, x within the
+// MALL): one 256-thread block per row b writes
+//   y[b][a] = conj(cA) sum_{T_A entries of a} (+/-) x[b][a']
+//           + conj(cB) sum_{T_B entries of b} (+/-) x[b'][a] + (D + shift + ext) x[b][a]
+// in one pass.  The row x[b] is staged in LDS for the T_A gathers (the
+// branch-free sign form of kron_rows4_kernel); row b's T_B entries are the
+// same for the whole block (scalar loads of ellB4: byte offsets of the
+// source rows and sign codes), so the T_B part is coalesced loads of the
+// source rows, issued before the LDS staging completes.  y is written once,
+// and there is one launch per product instead of two.  Needs T_A and T_B in
+// sign form and the diagonal from at most 64 number products.
+template <int WP, int S>
+__global__ void __launch_bounds__(256)
+    kron_fused_kernel(KronDesc k, const cplx* __restrict__ x, cplx* __restrict__ y, cplx shift) {
+  extern __shared__ cplx lds[];
+  cplx* nc = lds;                                           // 64 number-product coefficients
+  uint64_t* nmB = reinterpret_cast<uint64_t*>(lds + 64);    // their block-B masks (32 cplx)
+  cplx* row = lds + 96;                                     // slot 0 = 0, then x[b]
+  const int nA = (int)k.nA;
+  const int64_t b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int nnum = k.nnum;
+  const uint64_t amask = k.loB >= 64 ? ~0ull : ((1ull << k.loB) - 1ull);
+  const cplx* __restrict__ xb = x + b * nA;
+  if (t < nnum) {
+    nc[t] = k.ncoef[t];
+    nmB[t] = k.nmask[t] >> k.loB;
+  }
+  if (t == 0) row[0] = cmake(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int c = t + 256 * i;
+    if (c < nA) row[1 + c] = xb[c];
+  }
+  // T_B part: row b's entries, uniform over the block
+  const uint32_t* __restrict__ er = k.ellB4 + b * k.ellB4_stride;
+  const int ne = (int)er[0];
+  const char* __restrict__ xc = reinterpret_cast<const char*>(x);
+  double bx[S], by[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) bx[i] = by[i] = 0.0;
+  for (int j = 0; j < ne; j += 4) {
+    kdbl2 v[4][S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t ent = (j + u < ne) ? er[1 + j + u] : 0u;  // padding: row 0, weight 0
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int c = min(t + 256 * i, nA - 1);
+        v[u][i] = *reinterpret_cast<const kdbl2*>(xc + (ent & ~15u) + (size_t)c * sizeof(cplx));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t ent = (j + u < ne) ? er[1 + j + u] : 0u;
+      const uint32_t code = ent & 15u;
+      const double sg = code == 1u ? 1.0 : (code == 2u ? -1.0 : 0.0);  // scalar
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        bx[i] = __builtin_fma(sg, v[u][i].x, bx[i]);
+        by[i] = __builtin_fma(sg, v[u][i].y, by[i]);
+      }
+    }
+  }
+  // T_A words and the outputs' number-product sets
+  uint32_t e[S][WP];
+  uint64_t sa[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int a = t + 256 * i;
+#pragma unroll
+    for (int p = 0; p < WP; ++p) e[i][p] = a < nA ? k.ell16s[p * nA + a] : 0u;
+    const uint64_t st = k.statesA[a < nA ? a : nA - 1];
+    uint64_t f = 0;
+    for (int q = 0; q < nnum; ++q) {
+      const uint64_t m = k.nmask[q] & amask;
+      if ((st & m) == m) f |= 1ull << q;
+    }
+    sa[i] = f;
+  }
+  __syncthreads();
+  const uint64_t sbb = k.statesB[b];
+  uint64_t g = 0;  // number products whose B bits row b has
+  for (int q = 0; q < nnum; ++q)
+    if ((sbb & nmB[q]) == nmB[q]) g |= 1ull << q;
+  const cplx ca = k.acoef, cb = k.bcoef;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int a = t + 256 * i;
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int g0 = 0; g0 < 2 * WP; g0 += 4) {
+      kdbl2 xv[4];
+      uint32_t hv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int sl = g0 + u;
+        if (sl < 2 * WP) {
+          const uint32_t w = e[i][sl >> 1];
+          hv[u] = (sl & 1) ? (w >> 16) : (w & 0xFFFFu);
+          xv[u] = lds_x(row, (hv[u] >> 3) << 4);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (g0 + u < 2 * WP) {
+          const uint32_t sgn = (hv[u] << 29) & 0x80000000u;  // bit 2: the entry is -cA
+          sx += flip_sign(xv[u].x, sgn);
+          sy += flip_sign(xv[u].y, sgn);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    asm volatile("" : "+v"(sx), "+v"(sy));
+    if (a < nA) {
+      double rx = ca.x * sx + ca.y * sy;  // conj(cA) * sum
+      double ry = ca.x * sy - ca.y * sx;
+      rx += cb.x * bx[i] + cb.y * by[i];  // conj(cB) * sum
+      ry += cb.x * by[i] - cb.y * bx[i];
+      cplx d = cmake(0.0, 0.0);
+      for (uint64_t m = sa[i] & g; m; m &= m - 1) d = cadd(d, nc[__builtin_ctzll(m)]);
+      d = cadd(d, shift);
+      if (k.ext_diag) d = cadd(d, k.ext_diag[b * nA + a]);
+      const cplx dx = cmul(d, row[1 + a]);
+      y_st(y + b * nA + a, rx + dx.x, ry + dx.y);
+    }
+  }
+}
+
 // Slab kernel with the block's rows of packed T_B staged in LDS (one
 // coalesced read) and U gathers of X row segments in flight per step; same
 // summation order as kron_slab_packed_kernel.
@@ -1482,6 +1612,35 @@ bool launch_rows4_s(const KronDesc& d, int optin, int ncu, const cplx* x, cplx* 
   }
 }
 
+// the fused one-pass product (kron_fused_kernel), or false when the form
+// does not qualify (block sizes, sign forms, diagonal terms)
+template <int WP, int S>
+bool launch_fused_k(const KronDesc& d, int optin, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  const size_t lds = sizeof(cplx) * (96 + 1 + (size_t)d.nA);
+  if (lds > (size_t)optin) return false;
+  kron_fused_kernel<WP, S><<<(unsigned)d.nB, 256, lds, st>>>(d, x, y, shift);
+  return true;
+}
+
+bool launch_fused(const KronDesc& d, int optin, const cplx* x, cplx* y, cplx shift, hipStream_t st) {
+  if (!d.ell16s || !d.asign || !d.ellB4 || !d.bsign || d.ndterms > 0 || d.nnum > 64 || d.nA > 1024) return false;
+  if ((double)d.nA * (double)d.nB * sizeof(cplx) > 128e6) return false;  // x well within the MALL
+  const int wp = (d.ell_w + 1) / 2;
+#define QD_FUSED_S(S)                                                            \
+  if ((d.nA + 255) / 256 == S) {                                                 \
+    if (wp <= 8) return launch_fused_k<8, S>(d, optin, x, y, shift, st);         \
+    if (wp <= 10) return launch_fused_k<10, S>(d, optin, x, y, shift, st);       \
+    return launch_fused_k<12, S>(d, optin, x, y, shift, st);                     \
+  }
+  QD_FUSED_S(1)
+  QD_FUSED_S(2)
+  QD_FUSED_S(3)
+  QD_FUSED_S(4)
+#undef QD_FUSED_S
+  return false;
+}
+constexpr int KRON_FUSED = 15;  // variant word 15 * 16 + 15: the fused product
+
 // rows pass (writes y): variant 0 = kron_rows_packed<2>, 1 = rows2<2,8>,
 // 2 = rows2<1,8>, 3 = rows2<4,4>, 4 = the persistent rows3 (T_A in
 // registers; needs the compressed table), 5-8 = the branch-free rows4 (needs
@@ -1638,8 +1797,8 @@ float time_launches(hipStream_t st, F&& f, int reps) {
 int kron_tune_locked(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
   const KronDesc& d = k->desc;
   // QD_KRON_VARIANTS="rows,slab" fixes both variants instead of timing them
-  // (tests run every variant through it); an unfit variant then makes the
-  // product return QD_E_UNSUPPORTED.
+  // (tests run every variant through it; "15,15" is the fused product); an
+  // unfit variant then makes the product return QD_E_UNSUPPORTED.
   if (const char* f = std::getenv("QD_KRON_VARIANTS")) {
     int r = -1, s = -1;
     if (std::sscanf(f, "%d,%d", &r, &s) == 2 && r >= 0 && s >= 0 && r < 16 && s < 16) {
@@ -1678,6 +1837,16 @@ int kron_tune_locked(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
       best_s = v;
     }
   }
+  {  // the fused product, against the best pair of passes
+    bool ok = true;
+    const float ms = time_launches(st, [&] { ok = ok && launch_fused(d, optin, x, y, cmake(0.0, 0.0), st); }, 3);
+    if (ok && std::getenv("QD_TUNE_LOG")) std::fprintf(stderr, "[qd]   fused: %.1f us\n", 1000.0 * ms / 3);
+    if (ok && ms < tr + ts) {
+      best_r = best_s = KRON_FUSED;
+      tr = ms;
+      ts = 0.0f;
+    }
+  }
   QD_CHECK_LAUNCH(ctx);
   if (std::getenv("QD_TUNE_LOG"))
     std::fprintf(stderr, "[qd] Kronecker nA=%lld nB=%lld w=%d wB=%d: rows variant %d (%.1f us), slab variant %d (%.1f us)\n",
@@ -1710,6 +1879,12 @@ int qd_kron_apply(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y, cplx shift, c
   KronDesc d = k->desc;
   d.ext_diag = ext_diag;
   const int optin = kron_optin(ctx->device);
+  if (rows_variant == KRON_FUSED && slab_variant == KRON_FUSED) {
+    if (!launch_fused(d, optin, x, y, shift, ctx->stream))
+      return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker fused product");
+    QD_CHECK_LAUNCH(ctx);
+    return QD_OK;
+  }
   if (!launch_rows(d, rows_variant, optin, ctx->num_cus, x, y, shift, ctx->stream))
     return qd_set_error(ctx, QD_E_UNSUPPORTED, "Kronecker rows variant %d", rows_variant);
   QD_CHECK_LAUNCH(ctx);

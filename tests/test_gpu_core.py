@@ -539,6 +539,47 @@ def test_matfree_structured_every_variant(ctx, nup, ndn, amps, monkeypatch):
     assert {4, 5, 6, 7, 8} <= {r for r, s in ran}  # the persistent rows kernels (T_A in registers)
 
 
+@pytest.mark.parametrize("nup,ndn,amps", [(2, 3, 1), (3, 2, 1), (2, 3, 2)])
+def test_matfree_fused_matches_oracle(ctx, nup, ndn, amps, monkeypatch):
+    """The fused one-pass two-block product (QD_KRON_VARIANTS=15,15: T_A
+    from the LDS-staged row, T_B from coalesced loads of the source rows, y
+    written once) against the oracle, alone and under a complex diagonal
+    view, on a 14-site ring with blocks of 91 and 364 states.  With two
+    hopping amplitudes (amps = 2) the blocks are not in sign form and the
+    fused product is refused with QD_E_UNSUPPORTED."""
+    if amps == 1:
+        ham, sec = hub(14, Q.hubbard_bonds_chain(14, True), nup, ndn, U=3.0)[1:]
+    else:
+        ham, sec = _chain_two_amplitudes(14, nup, ndn, 1.0, 0.55, 3.0)
+    basis = oracle.basis_scan(sec)
+    ref = oracle.build_csc(basis, ham)
+    x = oracle.seeded_state(len(basis), 29)
+    yr = oracle.spmv(ref, x)
+    rng = np.random.default_rng(7)
+    dv = rng.standard_normal(len(basis)) + 1j * rng.standard_normal(len(basis))
+    b = Basis(ctx, sec)
+    p = TermProgram(ctx, ham, sec.nbits)
+    xd, dd = to_dev(ctx, x), to_dev(ctx, dv)
+    monkeypatch.setenv("QD_KRON_VARIANTS", "15,15")
+    mf = LinOp.matfree(ctx, b, p)
+    try:
+        assert mf.set_structured(True)
+        if amps == 2:
+            with pytest.raises(abi.QDError) as ei:
+                mf.apply(xd)
+            assert abi.STATUS[ei.value.code] == "QD_E_UNSUPPORTED"
+            return
+        y = mf.apply(xd).cpu().numpy()
+        assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14
+        view = LinOp.with_diagonal(ctx, mf, dd)
+        yv = view.apply(xd).cpu().numpy()
+        view.close()
+        yd = yr + dv * x
+        assert np.linalg.norm(yv - yd) / np.linalg.norm(yd) < 1e-14
+    finally:
+        _close(mf, p, b)
+
+
 @pytest.mark.parametrize("method", [0, 1])
 def test_reorth_methods_agree(ctx, method):
     """CGS2 (4 passes) and delayed CGS2 (2 passes): same E0, same evolve, orthonormal basis."""
