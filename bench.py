@@ -101,7 +101,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the Nd-filtered basis and qubits-times-mode measurements")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the Poisson stream")
-    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-workers", type=int, default=6)
     ap.add_argument("--traj", type=int, default=4096, help="number of trajectories")
     ap.add_argument("--traj-cpu", type=int, default=64,
                     help="trajectories in the oracle's CPU sample (a quarter of them in the reference-loop sample)")

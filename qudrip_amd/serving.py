@@ -23,7 +23,7 @@ from . import operators as Q
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget=200 << 30, workers=4, batch=1, gs_method=2):
+    def __init__(self, device=0, hbm_budget=200 << 30, workers=6, batch=1, gs_method=2):
         self.lib = abi.load()
         p = ctypes.c_void_p()
         rc = self.lib.qd_server_create(device, hbm_budget, workers, ctypes.byref(p))
@@ -120,7 +120,7 @@ def hubbard_lattice(Lx=3, Ly=4, nup=6, ndn=6):
     return nsite, Q.hubbard_bonds_square(Lx, Ly), nup, ndn
 
 
-def run_served(reqs, lattice=None, workers=4, hbm_budget=200 << 30, gs_k=150, gs_chunk=4, gs_tol=1e-12,
+def run_served(reqs, lattice=None, workers=6, hbm_budget=200 << 30, gs_k=150, gs_chunk=4, gs_tol=1e-12,
                rt_k=30, rt_steps=100, dt=0.05, rt_tol=1e-13, rt_mmin=16, batch=1, gs_method=2, log=None,
                closed_loop=False):
     """Registers the models (timed apart: the structure cache is built once at
