@@ -1765,13 +1765,20 @@ bool launch_slab(const KronDesc& d, int v, int ncu, const cplx* x, cplx* y, hipS
 // Variant choice per block shape, shared by every product of the process
 // (the first product of a shape times each variant: 3 launches after one
 // warm-up, best average).
+// The key holds everything that decides which variants can run on a form
+// (the packed tables built, their sign forms, the diagonal terms), so a
+// cached pair is always one the form supports.
 struct KronKey {
   int64_t nA, nB;
-  int w, wB, nvtab;
+  int w, wB, nvtab, nvtabB, feat;
   bool operator<(const KronKey& o) const {
-    return std::tie(nA, nB, w, wB, nvtab) < std::tie(o.nA, o.nB, o.w, o.wB, o.nvtab);
+    return std::tie(nA, nB, w, wB, nvtab, nvtabB, feat) < std::tie(o.nA, o.nB, o.w, o.wB, o.nvtab, o.nvtabB, o.feat);
   }
 };
+int kron_features(const KronDesc& d) {
+  return (d.ell16 ? 1 : 0) | (d.ell16s ? 2 : 0) | (d.asign ? 4 : 0) | (d.ellB4 ? 8 : 0) | (d.bsign ? 16 : 0) |
+         (d.ndterms > 0 ? 32 : 0) | (d.nnum > 32 ? 64 : 0) | (d.nnum > 64 ? 128 : 0);
+}
 std::mutex g_tune_mu;
 std::map<KronKey, std::pair<int, int>> g_tuned;
 
@@ -1806,7 +1813,8 @@ int kron_tune_locked(qd_ctx* ctx, qd_kron* k, const cplx* x, cplx* y) {
       return QD_OK;
     }
   }
-  const KronKey key{d.nA, d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, d.nvtab};
+  const KronKey key{d.nA, d.nB, d.ell ? d.ell_w : -1, d.ellB ? d.ellB_w : -1, d.nvtab, d.ellB ? d.nvtabB : -1,
+                    kron_features(d)};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);

@@ -580,6 +580,46 @@ def test_matfree_fused_matches_oracle(ctx, nup, ndn, amps, monkeypatch):
         _close(mf, p, b)
 
 
+def test_kron_tuned_variants_follow_the_form(ctx):
+    """The variant pair chosen for a block shape is cached for the process;
+    two forms of one shape (14-site ring, 2 up and 3 down) whose down-spin
+    blocks differ -- one hopping amplitude (sign form: the fused product and
+    the sign-form kernels can run) and two alternating amplitudes on the down
+    spins (they cannot) -- must each get a pair they support: both products
+    against the oracle, in either order."""
+    idx = Q.getQbits(28)
+    gate = Q.getHopFermiGate(idx)
+    bonds = Q.hubbard_bonds_chain(14, True)
+
+    def model(t_dn2):
+        ham = Q.getEmptyOperator()
+        for j, (a, b) in enumerate(bonds):
+            for off, t in ((0, 1.0), (14, 1.0 if j % 2 == 0 else t_dn2)):
+                ham = ham + (-t) * gate(a + off, b + off) + (-t) * gate(b + off, a + off)
+        for i in range(14):
+            ham = ham + 3.0 * (gate(i, i) * gate(i + 14, i + 14))
+        return ham
+
+    sec = Q.hubbard_sector(idx, 14, 2, 3)
+    basis = oracle.basis_scan(sec)
+    x = oracle.seeded_state(len(basis), 31)
+    b = Basis(ctx, sec)
+    xd = to_dev(ctx, x)
+    for order in ((1.0, 0.55), (0.4, 1.0)):
+        for t_dn2 in order:
+            ham = model(t_dn2)
+            yr = oracle.spmv(oracle.build_csc(basis, ham), x)
+            p = TermProgram(ctx, ham, sec.nbits)
+            mf = LinOp.matfree(ctx, b, p)
+            try:
+                assert mf.set_structured(True)
+                y = mf.apply(xd).cpu().numpy()
+                assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14, t_dn2
+            finally:
+                _close(mf, p)
+    _close(b)
+
+
 @pytest.mark.parametrize("method", [0, 1])
 def test_reorth_methods_agree(ctx, method):
     """CGS2 (4 passes) and delayed CGS2 (2 passes): same E0, same evolve, orthonormal basis."""
