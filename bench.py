@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--kron-products", type=int, default=0,
                     help="only apply the Hubbard 4x4 two-block matrix-free product this many times and exit "
                          "(a run to collect the product's counters)")
+    ap.add_argument("--trace-summary", default="",
+                    help="summarise a rocprofv3 --kernel-trace output directory (dispatches, span, GPU busy as "
+                         "the union of the kernel intervals), print one JSON line and exit (no GPU)")
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time the CPU restatement on the full-size workloads: the XXZ N=24 Lanczos "
                          "ground state (k = 150) and one Hubbard 4x4 matrix-free product (minutes)")
@@ -148,6 +151,10 @@ def main():
         print(json.dumps({"served_requests": len(reqs), "workers": args.serve_workers,
                           "solves_per_s": round(out["solves_per_s"], 3), "span_s": round(out["span_s"], 3),
                           "failed": sum(1 for x in out["results"].values() if x["status"] != 0)}), flush=True)
+        return
+
+    if args.trace_summary:
+        print(json.dumps(trace_summary(args.trace_summary)), flush=True)
         return
 
     if args.kron_products:
@@ -316,6 +323,30 @@ def main():
         result["cpu_baseline_more"] = cpu_more(csr, n, args)
 
     print(json.dumps(result), flush=True)
+
+
+def trace_summary(outdir):
+    """Dispatch count, span and GPU busy time of a rocprofv3 kernel trace
+    (`run_kernel_trace.csv` under `outdir`): busy is the length of the union
+    of the kernel intervals, so kernels that overlap on several streams count
+    once."""
+    import csv
+    path = os.path.join(outdir, "run_kernel_trace.csv")
+    with open(path, newline="") as fh:
+        iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in csv.DictReader(fh))
+    if not iv:
+        return {"dispatches": 0, "span_s": 0.0, "busy_s": 0.0, "busy_frac": 0.0}
+    busy, (cs, ce) = 0, iv[0]
+    for s, e in iv[1:]:
+        if s > ce:
+            busy += ce - cs
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    busy += ce - cs
+    span = iv[-1][1] - iv[0][0]
+    return {"dispatches": len(iv), "span_s": span / 1e9, "busy_s": busy / 1e9,
+            "busy_frac": busy / span if span else 0.0}
 
 
 def kron_products(count):

@@ -1,6 +1,7 @@
 """Host logic of bench.py that needs no GPU: the traffic figure is taken from
 a counter record only while the kernel source is the one the record names,
-and the CPU description of cpu_baseline."""
+the CPU description of cpu_baseline, and the kernel-trace summary (GPU busy
+as the union of overlapping kernel intervals)."""
 import hashlib
 import os
 
@@ -52,3 +53,16 @@ def test_host_cpu_fields():
     h = bench.host_cpu()
     assert set(h) == {"cpu_model", "nproc", "usable_threads"}
     assert h["nproc"] >= 1 and 1 <= h["usable_threads"] <= h["nproc"]
+
+
+def test_trace_summary_merges_overlapping_kernels(tmp_path):
+    rows = [(100, 200), (150, 260), (300, 350), (340, 400), (500, 510)]  # ns; two overlapping pairs
+    with open(tmp_path / "run_kernel_trace.csv", "w") as fh:
+        fh.write("Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for s, e in reversed(rows):
+            fh.write(f"k,{s},{e}\n")
+    out = bench.trace_summary(str(tmp_path))
+    assert out["dispatches"] == 5
+    assert abs(out["span_s"] - 410e-9) < 1e-15
+    assert abs(out["busy_s"] - (160 + 100 + 10) * 1e-9) < 1e-15
+    assert abs(out["busy_frac"] - 270 / 410) < 1e-12

@@ -76,7 +76,7 @@ int main() {
 
 
 def test_driver_programs_built():
-    for p in ("spin", "dropin_dump", "mc_dump", "hubbard_quench"):
+    for p in ("spin", "dropin_dump", "mc_dump", "hubbard_quench", "serve_stress"):
         assert os.access(os.path.join(BIN, p), os.X_OK), p
 
 
@@ -85,6 +85,17 @@ def _run(args, **kw):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     return r.stdout
+
+
+@pytest.mark.gpu
+def test_serve_stress_driver_six_workers():
+    """The server from C++ alone (programs/serve_stress.cpp): a mixed stream
+    submitted at once to six workers, twice; every request succeeds and
+    identical requests agree to 1e-12 whatever ran beside them."""
+    out = _run([os.path.join(BIN, "serve_stress"), "6", "48", "2", "20"])
+    m = re.search(r"serve_stress: (\d+) results, (\d+) distinct requests, (\d+) failures, max running (\d+)", out)
+    assert m and m.group(1) == "96" and m.group(3) == "0", out
+    assert int(m.group(2)) > 1 and 1 <= int(m.group(4)) <= 6
 
 
 @pytest.mark.gpu
