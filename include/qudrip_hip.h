@@ -403,7 +403,11 @@ int qd_hlanczos_coefficients(qd_hlanczos* lz, double* h_alpha, double* h_beta, i
  * c_mode, fermionic) channels; a loss moves a trajectory to the sector with
  * one particle less.  All trajectories of one sector advance together
  * (multi-RHS product, per-trajectory Arnoldi with kry_dim_rt, device-side
- * jump sampling).  Random numbers: key = splitmix64(seed*0x9E3779B97F4A7C15 +
+ * jump sampling).  When every up mode carries the same total jump rate and
+ * every down mode likewise, H_eff is H plus a number on each sector and the
+ * Arnoldi step runs as the equivalent Lanczos recurrence (same Krylov space
+ * and projected exponential; QD_TRAJ_KRYLOV=arnoldi in the environment keeps
+ * the Arnoldi form).  Random numbers: key = splitmix64(seed*0x9E3779B97F4A7C15 +
  * traj), r_k(step) = (splitmix64(key + 4 step + k) >> 11) 2^-53. */
 enum { QD_JUMP_DEPHASE = 0, QD_JUMP_LOSS = 1 };
 typedef struct {
@@ -415,6 +419,9 @@ typedef struct {
   int64_t jumps;
   int64_t losses;
   int32_t sectors;
+  int32_t lanczos; /* 1: the steps ran in the Lanczos form (every sector's
+                    * sum_c L_c^+ L_c constant, H_eff = H + c_t there);
+                    * 0: the Arnoldi form */
 } qd_traj_stats;
 /* H: term program over 2*nsite bits conserving each spin's particle number.
  * d_psi0: start state in sector (nup0, ndn0) (device, ascending basis order).

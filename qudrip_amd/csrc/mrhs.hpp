@@ -940,6 +940,182 @@ inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t til
   mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, f, alpha, beta, b, SpmmProduct{d_gsec, ns, blocks});
 }
 
+// ---------------------------------------------------------- Lanczos form
+// When H is Hermitian and alpha_t D + beta_t is one number c_t on every
+// sector (each mode carries the same total jump rate r, so sum_c L_c^+ L_c =
+// r N is constant on a sector), H_eff = H + c_t there.  Its Arnoldi process
+// then reduces to the three-term recurrence: in exact arithmetic the
+// Hessenberg is tridiagonal, h_jj = alpha_j (Paige's form: v_j^H (H_eff v_j
+// - beta_j v_{j-1}), which includes c_t), h_{j+1,j} = h_{j,j+1} = beta_{j+1},
+// and the Krylov vectors are the same.  Per Krylov vector: the product, one
+// pass over (u_j, W) for alpha_j, one update pass; no pass over the basis.
+// The vectors stay unnormalised in the pool (u_0 = v_0, u_j = beta_j v_j),
+// so nothing is rewritten: the product takes 1/beta_j as its source factor,
+// the update and the final combination fold it into their coefficients.
+// Finite-precision Lanczos vectors lose orthogonality as Ritz values
+// converge, but the projected exponential stays accurate (Druskin,
+// Greenbaum & Knizhnerman, SIAM J. Sci. Comput. 19 (1998) 38-54).
+struct LzState {
+  double* beta;  // [j Mtot + slot] beta_j = |u_j|
+  double* inv;   // 1 / beta_j (inv_0 = 1; 0 once the Krylov space has ended)
+  cplx* ip;      // <v_j, v_{j-1}>
+  cplx* ca;      // update coefficients of the current step: alpha_j / beta_j
+  double* cb;    //                                          beta_j / beta_{j-1}
+};
+
+// u_{j+1} = W - ca_t u_j - cb_t u_{j-1} (no u_{j-1} term for j = 0) into pool
+// slot j + 1, with part[2 co + (chunk 2 + p) m + t]: p = 0: |u_{j+1}|^2,
+// p = 1: conj(u_j) u_{j+1}.  Tiles as in g_pass_kernel.
+__global__ void __launch_bounds__(TT)
+    g_lz_update_kernel(const GSec* __restrict__ gs, int ns, cplx* __restrict__ V, int64_t VS, int j,
+                       const cplx* __restrict__ W, const cplx* __restrict__ ca, const double* __restrict__ cb,
+                       cplx* __restrict__ part) {
+  const Tile tl = tile_of(gs, ns);
+  const GSec& S = gs[tl.sec];
+  const int t = tl.t;
+  const int m = S.m;
+  const bool act = t < m;
+  if (tl.rstep == 1 && !act) return;
+  const int64_t r0 = (int64_t)tl.chunk * S.rpc;
+  const int64_t r1 = act ? min<int64_t>(S.n, r0 + S.rpc) : r0;
+  const cplx* __restrict__ Uj = V + (int64_t)j * VS + S.vo;
+  const cplx* __restrict__ Um = V + (int64_t)(j > 0 ? j - 1 : 0) * VS + S.vo;
+  cplx* __restrict__ Un = V + (int64_t)(j + 1) * VS + S.vo;
+  const cplx* __restrict__ Wb = W + S.vo;
+  const cplx a = act ? ca[S.so + t] : cmake(0.0, 0.0);
+  const double b = (act && j > 0) ? cb[S.so + t] : 0.0;
+  double n2 = 0.0;
+  cplx c = cmake(0.0, 0.0);
+#pragma unroll 2
+  for (int64_t s = r0 + tl.sub; s < r1; s += tl.rstep) {
+    const int64_t o = s * m + t;
+    const cplx w = Wb[o];
+    const cplx u = Uj[o];
+    const cplx um = j > 0 ? Um[o] : cmake(0.0, 0.0);
+    const cplx un = cmake(w.x - (a.x * u.x - a.y * u.y) - b * um.x, w.y - (a.x * u.y + a.y * u.x) - b * um.y);
+    Un[o] = un;
+    n2 += un.x * un.x + un.y * un.y;
+    c = cadd(c, cdotmul(u, un));
+  }
+  if (tl.rstep > 1) {
+    n2 = sub_sum(n2, S.mp);
+    c = sub_sum(c, S.mp);
+  }
+  if (!act || tl.sub != 0) return;
+  cplx* pb = part + 2 * S.co;
+  pb[((int64_t)tl.chunk * 2 + 0) * m + t] = cmake(n2, 0.0);
+  pb[((int64_t)tl.chunk * 2 + 1) * m + t] = c;
+}
+
+// sum over the chunks of value q (of NP per chunk) for slot t of sector S, by
+// the RW waves of the block (same order as g_reduce_kernel); valid in wave 0
+__device__ inline cplx lz_chunk_sum(const GSec& S, const cplx* __restrict__ part, int NP, int q, int t, int wv,
+                                    int lane, cplx (*sh)[TT]) {
+  cplx s = cmake(0.0, 0.0);
+  if (t < S.m) {
+    const cplx* pb = part + (int64_t)NP * S.co;
+#pragma unroll 4
+    for (int c = wv; c < S.nch; c += RW) s = cadd(s, pb[((int64_t)c * NP + q) * S.m + t]);
+  }
+  __syncthreads();
+  sh[wv][lane] = s;
+  __syncthreads();
+  s = sh[0][lane];
+  for (int w = 1; w < RW; ++w) s = cadd(s, sh[w][lane]);
+  return s;
+}
+
+// Lanczos step j after the pass over (u_j, W): alpha_j = inv_j <u_j, W> -
+// beta_j <v_j, v_{j-1}> -> Hess[j][j]; ca = alpha_j inv_j, cb = beta_j
+// inv_{j-1} (0 for j = 0 and once the space has ended).  One block per
+// 64-slot group.
+__global__ void __launch_bounds__(RW * TT)
+    g_lz_alpha_kernel(const GSec* __restrict__ gs, int ns, int Mtot, const cplx* __restrict__ part, int j, int K,
+                      cplx* __restrict__ hess, LzState lz) {
+  __shared__ cplx sh[RW][TT];
+  const int64_t gi = blockIdx.x;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  const cplx s = lz_chunk_sum(S, part, 1, 0, t, wv, lane, sh);
+  if (wv != 0 || t >= S.m) return;
+  const int g = S.so + t;
+  double iv = 1.0, cbv = 0.0;
+  cplx al = s;
+  if (j == 0) {
+    lz.inv[g] = 1.0;
+  } else {
+    iv = lz.inv[(int64_t)j * Mtot + g];
+    const double bj = lz.beta[(int64_t)j * Mtot + g];
+    const cplx ip = lz.ip[(int64_t)j * Mtot + g];
+    al = cmake(s.x * iv - bj * ip.x, s.y * iv - bj * ip.y);
+    cbv = iv > 0.0 ? bj * lz.inv[(int64_t)(j - 1) * Mtot + g] : 0.0;
+  }
+  hess[((int64_t)(j * (K + 1) + j)) * Mtot + g] = al;
+  lz.ca[g] = crmul(iv, al);
+  lz.cb[g] = cbv;
+}
+
+// after the update of step j: beta_{j+1} = |u_{j+1}|, inv_{j+1} (0 below
+// 1e-10 or after an earlier end, as the Arnoldi form), <v_{j+1}, v_j>;
+// Hess[j+1][j] = beta_{j+1}, Hess[j][j+1] = beta_{j+1} (0 if the space ended).
+__global__ void __launch_bounds__(RW * TT)
+    g_lz_beta_kernel(const GSec* __restrict__ gs, int ns, int Mtot, const cplx* __restrict__ part, int j, int K,
+                     cplx* __restrict__ hess, LzState lz) {
+  __shared__ cplx sh[RW][TT];
+  const int64_t gi = blockIdx.x;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int t = (int)(gi - S.grp0) * TT + lane;
+  const cplx n2 = lz_chunk_sum(S, part, 2, 0, t, wv, lane, sh);
+  const cplx c = lz_chunk_sum(S, part, 2, 1, t, wv, lane, sh);
+  if (wv != 0 || t >= S.m) return;
+  const int g = S.so + t;
+  const double b = sqrt(n2.x);
+  const double ivj = lz.inv[(int64_t)j * Mtot + g];
+  const double iv = (b >= 1e-10 && ivj > 0.0) ? 1.0 / b : 0.0;
+  lz.beta[(int64_t)(j + 1) * Mtot + g] = b;
+  lz.inv[(int64_t)(j + 1) * Mtot + g] = iv;
+  lz.ip[(int64_t)(j + 1) * Mtot + g] = cmake(c.x * iv * ivj, -c.y * iv * ivj);
+  hess[((int64_t)(j * (K + 1) + j + 1)) * Mtot + g] = cmake(b, 0.0);
+  hess[((int64_t)((j + 1) * (K + 1) + j)) * Mtot + g] = cmake(iv > 0.0 ? b : 0.0, 0.0);
+}
+
+// coef[j Mtot + slot] *= inv_j: coefficients of the unnormalised pool vectors
+__global__ void g_lz_scale_kernel(int Mtot, int K, const double* __restrict__ inv, cplx* __restrict__ coef) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)K * Mtot) return;
+  coef[e] = crmul(inv[e], coef[e]);
+}
+
+// K Lanczos vectors for every slot (the Lanczos form above): V_0 = f X, then
+// y = expm(-i dt T_t) e_1 scaled for the unnormalised vectors, so that
+// mrhs_combine writes V y into X as after mrhs_arnoldi.  Hess must hold zeros
+// off the three diagonals (only those are written here).
+template <class Prod>
+inline void mrhs_lanczos_with(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t ngroups,
+                              int64_t VS, int Mtot, int K, double dt, const double* f, const cplx* alpha,
+                              const cplx* beta, const MrhsBufs& b, const LzState& lz, const Prod& prod) {
+  const unsigned tl = (unsigned)tiles;
+  cplx* W = b.W;
+  for (int j = 0; j < K; ++j) {
+    if (j == 0)
+      prod(st, 1, nullptr, f, alpha, beta, b.V, W);
+    else
+      prod(st, 0, b.V + (int64_t)j * VS, lz.inv + (int64_t)j * Mtot, alpha, beta, nullptr, W);
+    g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, b.V + (int64_t)j * VS, VS, W, nullptr, Mtot, b.part);
+    g_lz_alpha_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, b.part, j, K, b.hess, lz);
+    if (j + 1 < K) {
+      g_lz_update_kernel<<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, j, W, lz.ca, lz.cb, b.part);
+      g_lz_beta_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, b.part, j, K, b.hess, lz);
+    }
+  }
+  small_expm_kernel<<<(unsigned)Mtot, 64, 0, st>>>(Mtot, K, K, b.hess, dt, b.coef);
+  g_lz_scale_kernel<<<g1((int64_t)K * Mtot, 256), 256, 0, st>>>(Mtot, K, lz.inv, b.coef);
+}
+
 inline void mrhs_combine(hipStream_t st, const GSec* d_gsec, int ns, int64_t VS, int Mtot, int K, const MrhsBufs& b) {
   g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, b.V, VS, K, b.coef, Mtot);
 }

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "mrhs.hpp"
@@ -225,6 +226,21 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       return qd_set_error(ctx, QD_E_ARG, "jump channel %d invalid", c);
     }
   }
+  // Lanczos form (mrhs.hpp) when sum_c L_c^+ L_c is constant on every sector:
+  // D = sum_m r_m n_m with r_m the total rate of mode m, constant on a sector
+  // (fixed up and down counts) when r_m is one value over the up modes and one
+  // over the down modes.  QD_TRAJ_KRYLOV=arnoldi keeps the Arnoldi form.
+  bool use_lz = true;
+  {
+    std::vector<double> rm(M, 0.0);
+    for (int c = 0; c < njumps; ++c) rm[jm[c]] += jrt[c];
+    for (int half = 0; half < 2; ++half) {
+      const auto mm = std::minmax_element(rm.begin() + half * nsite, rm.begin() + (half + 1) * nsite);
+      if (*mm.second - *mm.first > 1e-13 * std::fabs(*mm.second)) use_lz = false;
+    }
+    const char* e = std::getenv("QD_TRAJ_KRYLOV");
+    if (e && std::string(e) == "arnoldi") use_lz = false;
+  }
   std::vector<Sector> sec((nsite + 1) * (nsite + 1));
   auto sid = [&](int a, int b) { return a * (nsite + 1) + b; };
   // largest sector reachable by losses from (nup0, ndn0): C(n, a) is not monotone in a
@@ -244,6 +260,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   cplx *d_alpha = nullptr, *d_beta = nullptr, *d_red = nullptr, *d_coef2 = nullptr, *d_sc = nullptr;
   cplx *d_V = nullptr, *d_W = nullptr, *d_W2 = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr;
   cplx *d_coef = nullptr, *d_staging = nullptr;
+  LzState lz{};
   GSec* d_gsec = nullptr;
   JumpRec* d_jr = nullptr;
   int2 *d_pairs = nullptr, *d_moves = nullptr;
@@ -303,10 +320,20 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&d_hess, sizeof(cplx) * (size_t)K * (K + 1) * ntraj) ||
       !dalloc((void**)&d_coef, sizeof(cplx) * (size_t)K * ntraj) ||
       !dalloc((void**)&d_gsec, sizeof(GSec) * NSMAX) || !dalloc((void**)&d_jr, sizeof(JumpRec) * ntraj) ||
-      !dalloc((void**)&d_pairs, sizeof(int2) * ntraj) || !dalloc((void**)&d_moves, sizeof(int2) * ntraj)) {
+      !dalloc((void**)&d_pairs, sizeof(int2) * ntraj) || !dalloc((void**)&d_moves, sizeof(int2) * ntraj) ||
+      !dalloc((void**)&lz.beta, sizeof(double) * (size_t)K * ntraj) ||
+      !dalloc((void**)&lz.inv, sizeof(double) * (size_t)K * ntraj) ||
+      !dalloc((void**)&lz.ip, sizeof(cplx) * (size_t)K * ntraj) || !dalloc((void**)&lz.ca, sizeof(cplx) * ntraj) ||
+      !dalloc((void**)&lz.cb, sizeof(cplx) * ntraj)) {
     cleanup();
     return qd_set_error(ctx, QD_E_OOM, "trajectory workspace (%d trajectories, dim %lld, k %d)", ntraj,
                         (long long)nmax, k_rt);
+  }
+  if (use_lz) {
+    // the Lanczos form writes only the three diagonals of each Hessenberg
+    zero_kernel<<<g1((int64_t)K * (K + 1) * ntraj, 256), 256, 0, st>>>((int64_t)K * (K + 1) * ntraj, d_hess);
+    zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, lz.ca);
+    zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, (cplx*)lz.cb);  // allocated as ntraj cplx
   }
   if (njumps) {
     hipMemcpyAsync(d_jmode, jm.data(), sizeof(int) * njumps, hipMemcpyHostToDevice, st);
@@ -408,7 +435,11 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     const int Mtot = ntraj;
     slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
                                                        d_scale);
-    mrhs_arnoldi(st, d_gsec, ns, tiles, blocks, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb);
+    if (use_lz)
+      mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz,
+                        SpmmProduct{d_gsec, ns, blocks});
+    else
+      mrhs_arnoldi(st, d_gsec, ns, tiles, blocks, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb);
     mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
     QD_CHECK_LAUNCH(ctx);
     return QD_OK;
@@ -417,7 +448,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   // Tile autotuner for the uneven sectors: the occupancy target of the
   // reduction tiles (rows per chunk of every sector) is timed on the widest
   // reduction pass over the current sector membership (a read-only pass,
-  // results discarded).  It runs at the start and again once the loss jumps
+  // results discarded; in the Lanczos form the update pass, whose output
+  // slot is rewritten before it is read).  It runs at the start and again once the loss jumps
   // have spread the trajectories over smaller sectors.
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEventCreate(&ev0);
@@ -431,9 +463,18 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       occupancy_target = c;
       build_groups();
       const int nq = std::max(1, K - 1);
-      launch_red<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, d_W2, d_part);
+      auto pass = [&]() {
+        if (!use_lz)
+          launch_red<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, d_W2, d_part);
+        else if (K >= 2)
+          g_lz_update_kernel<<<(unsigned)tiles, TT, 0, st>>>(d_gsec, ns, d_V, VS, K >= 3 ? 1 : 0, d_W, lz.ca, lz.cb,
+                                                             d_part);
+        else
+          g_pass_kernel<1, 0><<<(unsigned)tiles, TT, 0, st>>>(d_gsec, ns, d_V, VS, d_W, nullptr, ntraj, d_part);
+      };
+      pass();
       hipEventRecord(ev0, st);
-      for (int r = 0; r < 2; ++r) launch_red<2>(nq, (unsigned)tiles, st, d_gsec, ns, d_V, VS, d_W, d_W2, d_part);
+      for (int r = 0; r < 2; ++r) pass();
       hipEventRecord(ev1, st);
       hipEventSynchronize(ev1);
       float ms = 0.f;
@@ -607,6 +648,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       int used = 0;
       for (auto& S : sec) used += S.basis ? 1 : 0;
       stats->sectors = used;
+      stats->lanczos = use_lz ? 1 : 0;
     }
   }
   hipEventDestroy(ev0);

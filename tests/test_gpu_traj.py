@@ -84,3 +84,69 @@ def test_traj_batched_matches_plain_one_at_a_time(ctx):
     dsum, log, _ = plain_trajectories(ctx, ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 8)
     assert (g["log"] == log).all()
     np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
+
+
+def _with_krylov_env(value, fn):
+    import os
+    old = os.environ.get("QD_TRAJ_KRYLOV")
+    if value is None:
+        os.environ.pop("QD_TRAJ_KRYLOV", None)
+    else:
+        os.environ["QD_TRAJ_KRYLOV"] = value
+    try:
+        return fn()
+    finally:
+        if old is None:
+            os.environ.pop("QD_TRAJ_KRYLOV", None)
+        else:
+            os.environ["QD_TRAJ_KRYLOV"] = old
+
+
+def test_traj_lanczos_form_matches_arnoldi_form(ctx):
+    """Uniform channels: the steps run in the Lanczos form; forcing the Arnoldi
+    form on the same input draws the same jumps and gives the same densities
+    (same Krylov space and projected exponential, rounding apart), at L = 6
+    with sectors from 400 states down to one and k_rt = 13."""
+    L, nup, ndn = 6, 3, 3
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(400, 21)
+    ch = jump_channels(L, 0.3, 0.2)
+    args = (96, 80, 0.05, 13, 9)
+    lz = _with_krylov_env(None, lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True))
+    ar = _with_krylov_env("arnoldi", lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True))
+    assert lz["lanczos"] == 1 and ar["lanczos"] == 0
+    assert lz["losses"] > 50 and lz["sectors"] > 6
+    assert (lz["log"] == ar["log"]).all()
+    np.testing.assert_allclose(lz["mean"], ar["mean"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(lz["var"], ar["var"], rtol=0, atol=1e-12)
+
+
+def test_traj_spin_dependent_rates_lanczos_against_oracle(ctx):
+    """Up modes and down modes with different rates: sum L^+L = r_up N_up +
+    r_dn N_dn is still one number per sector, so the Lanczos form runs; the
+    oracle (Arnoldi) draws the same jumps and the same densities."""
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 23)
+    ch = [(0, m, 0.5 if m < L else 0.2) for m in range(2 * L)] + [(1, m, 0.1 if m < L else 0.4) for m in range(2 * L)]
+    ntraj, nsteps, dt, k = 120, 50, 0.05, 10
+    g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 6, want_log=True)
+    dsum, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 6)
+    assert g["lanczos"] == 1
+    assert (g["log"] == log).all()
+    np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
+
+
+def test_traj_site_dependent_rates_arnoldi_against_oracle(ctx):
+    """Site-dependent rates make sum L^+L vary inside a sector: H_eff is no
+    longer H plus a number, the Arnoldi form runs, and it matches the oracle."""
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 25)
+    ch = [(0, m, 0.2 + 0.1 * (m % L)) for m in range(2 * L)] + [(1, m, 0.3) for m in range(2 * L)]
+    ntraj, nsteps, dt, k = 120, 50, 0.05, 10
+    g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 7, want_log=True)
+    dsum, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 7)
+    assert g["lanczos"] == 0
+    assert (g["log"] == log).all()
+    np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
