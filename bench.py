@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--kron-products", type=int, default=0,
                     help="only apply the Hubbard 4x4 two-block matrix-free product this many times and exit "
                          "(a run to collect the product's counters)")
+    ap.add_argument("--traj-steps-only", type=int, default=0,
+                    help="only run the batched dissipative trajectory workload for this many time steps (T = steps "
+                         "x 0.05) and exit (a run to collect the trajectory engine's counters)")
     ap.add_argument("--trace-summary", default="",
                     help="summarise a rocprofv3 --kernel-trace output directory (dispatches, span, GPU busy as "
                          "the union of the kernel intervals), print one JSON line and exit (no GPU)")
@@ -159,6 +162,16 @@ def main():
 
     if args.kron_products:
         kron_products(args.kron_products)
+        return
+
+    if args.traj_steps_only:
+        from qudrip_amd import workloads as WL
+        from qudrip_amd.runtime import Context
+        log = lambda m: print("[bench] " + m, file=sys.stderr, flush=True)
+        r = WL.hubbard_trajectories(Context(0), ntraj=args.traj, T=args.traj_steps_only * 0.05, log=log)
+        print(json.dumps({"trajectories": args.traj, "steps": r["nsteps"], "run_s": round(r["run_s"], 3),
+                          "jumps": r["jumps"], "losses": r["losses"],
+                          "krylov_form": "lanczos" if r["lanczos"] else "arnoldi"}))
         return
 
     if args.calib_copy_mb:

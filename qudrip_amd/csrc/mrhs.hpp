@@ -286,6 +286,229 @@ __global__ void __launch_bounds__(TT)
   }
 }
 
+// p[0] through a global-space pointer: flat loads also count in lgkmcnt, so
+// a wait for an LDS permute would wait for every gather in flight too
+typedef double qd_d2 __attribute__((ext_vector_type(2)));
+__device__ inline cplx load_global(const cplx* p) {
+  const qd_d2 v = *(const __attribute__((address_space(1))) qd_d2*)p;
+  return cmake(v.x, v.y);
+}
+
+// p[0] = v, one 16-byte non-temporal global store
+__device__ inline void store_nt(cplx* p, cplx v) {
+  qd_d2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, (__attribute__((address_space(1))) qd_d2*)p);
+}
+
+// g_spmm_kernel with the entries of a wave's RB rows taken as one stream:
+// they are contiguous in the CSR arrays, so the batches of 8 gathers run
+// across row boundaries and only the last batch of the wave is padded (rows
+// of about 9 entries otherwise pad 7 of every 16 gathers).  A row's sum is
+// flushed to its outputs when the stream passes its last entry (a uniform
+// compare).  The kernel is bound by the texture path's instruction rate, so
+// the per-row loads and stores are few and wide: the rows' diagonal values
+// come in one load, the outputs go out as 16-byte stores, and the own-row
+// values of the diagonal terms are loaded up front (PRE) or at the flush.
+// Items are RB rows of one 64-slot group (blk0 / nblocks count ceil(n / RB)
+// * groups per sector).  Same per-row entry order and sums as g_spmm_kernel
+// (pads added none).
+template <int RB, bool PRE>
+__global__ void __launch_bounds__(TT)
+    g_spmm_flat_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x,
+                       const cplx* __restrict__ Wsrc, const double* __restrict__ f, const cplx* __restrict__ alpha,
+                       const cplx* __restrict__ beta, cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
+  static_assert(RB < TT, "row pointers of an item fit in one wave");
+  const int64_t per_xcd = (int64_t)gridDim.x / 8;
+  const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (b >= nblocks) return;
+  const int si = find_sec(ns, b, [&](int i) { return gs[i].blk0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x;
+  const int64_t nrb = (S.n + RB - 1) / RB;
+  const int64_t local = b - S.blk0;
+  const int64_t s0 = (local % nrb) * RB;
+  const int nr = (int)min<int64_t>(RB, S.n - s0);
+  const int t = (int)(local / nrb) * TT + lane;
+  const int m = S.m;
+  const bool active = t < m;
+  const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
+  const int64_t lds = from_x ? S.ldx : m;
+  const int g = S.so + (active ? t : 0);
+  const int tl = active ? t : 0;  // inactive lanes read slot 0 (in range) and drop it
+  const double ft = active ? (f ? f[g] : 1.0) : 0.0;
+  const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
+  const cplx be = active ? beta[g] : cmake(0.0, 0.0);
+  const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
+  const double dv = (lane < nr) ? S.D[s0 + lane] : 0.0;  // lane r: D of row r
+  cplx xo[PRE ? RB : 1];
+  if (PRE) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r) xo[r] = load_global(src + min<int64_t>(s0 + r, S.n - 1) * lds + tl);
+  }
+  const int E0 = __builtin_amdgcn_readlane(optr, 0);
+  const int E1 = __builtin_amdgcn_readlane(optr, nr);
+  int r = 0;
+  int rend = __builtin_amdgcn_readlane(optr, 1);
+  double sx = 0.0, sy = 0.0;
+  // outputs of row rr (uniform, rr < nr)
+  auto flush = [&](int rr) {
+    const int64_t s = s0 + rr;
+    const double d = readlane_d(dv, rr);
+    if (active) {
+      cplx x0;
+      if (PRE) {
+        x0 = xo[0];
+#pragma unroll
+        for (int q = 1; q < (PRE ? RB : 1); ++q)
+          if (rr == q) x0 = xo[q];
+      } else {
+        x0 = load_global(src + s * lds + t);
+      }
+      const cplx x = cmake(x0.x * ft, x0.y * ft);
+      const cplx dg = cmake(al.x * d + be.x, al.y * d + be.y);
+      const cplx dx = cmul(dg, x);
+      if (Vout) store_nt(Vout + S.vo + s * m + t, x);
+      store_nt(Wout + S.vo + s * m + t, cmake(sx * ft + dx.x, sy * ft + dx.y));
+    }
+    sx = 0.0;
+    sy = 0.0;
+  };
+  for (int base = E0; base < E1; base += TT) {
+    const int cnt = min(TT, E1 - base);
+    const int ci = (lane < cnt) ? S.inner[base + lane] : 0;
+    const cplx cv = (lane < cnt) ? S.vals[base + lane] : cmake(0.0, 0.0);
+    for (int k0 = 0; k0 < cnt; k0 += 8) {
+      cplx x[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int col = __builtin_amdgcn_readlane(ci, min(k0 + i, cnt - 1));
+        x[i] = load_global(src + (int64_t)col * lds + tl);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = k0 + i;
+        if (k < cnt) {
+          while (base + k >= rend) {  // rows that end here (empty rows included)
+            flush(r);
+            ++r;
+            rend = __builtin_amdgcn_readlane(optr, r + 1);
+          }
+          const double vx = readlane_d(cv.x, k);
+          const double vy = readlane_d(cv.y, k);
+          sx += vx * x[i].x + vy * x[i].y;
+          sy += vx * x[i].y - vy * x[i].x;
+        }
+      }
+    }
+  }
+  for (; r < nr; ++r) flush(r);
+}
+
+// g_spmm_kernel over SW-slot segments (SW = 32 or 16): a wave covers R = 64 /
+// SW rows at once, lane = sub SW + tt holding slot tt of row sub, so a
+// work item is (sector, SW-slot group, SPMM_RB rows) and blk0 / nblocks
+// count ceil(n / SPMM_RB) * ceil(m / SW) items per sector.  The source block
+// of one slot group (n rows x SW slots) is 64 / SW times smaller than with
+// 64 slots, so an XCD working through one group keeps it in its L2 (n = 4900
+// rows: 2.5 MB at 32 slots, 4.9 MB at 64, against 4 MB).  The lanes of a
+// segment load its row's entries (one per lane) and every lane pulls entry k
+// of its own row with ds_bpermute (__shfl: column and value, five reads); a
+// segment whose row has fewer entries than the longest of the R rows masks
+// its loads off for the rest.  The entries of a row are summed in the same
+// order as in g_spmm_kernel (the padding adds zeros).
+template <int SW>
+__global__ void __launch_bounds__(TT)
+    g_spmm_sw_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
+                     const double* __restrict__ f, const cplx* __restrict__ alpha, const cplx* __restrict__ beta,
+                     cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
+  constexpr int R = TT / SW;
+  static_assert(SW % 8 == 0 && SPMM_RB % R == 0, "segments of whole entry batches, whole row groups");
+  const int64_t per_xcd = (int64_t)gridDim.x / 8;
+  const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (b >= nblocks) return;
+  const int si = find_sec(ns, b, [&](int i) { return gs[i].blk0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x;
+  const int sub = lane / SW, tt = lane % SW;
+  const int64_t nrb = (S.n + SPMM_RB - 1) / SPMM_RB;
+  const int64_t local = b - S.blk0;
+  const int64_t s0 = (local % nrb) * SPMM_RB;
+  const int nr = (int)min<int64_t>(SPMM_RB, S.n - s0);
+  const int t = (int)(local / nrb) * SW + tt;
+  const int m = S.m;
+  const bool active = t < m;
+  const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
+  const int64_t lds = from_x ? S.ldx : m;
+  const int g = S.so + (active ? t : 0);
+  const int tl = active ? t : 0;  // inactive lanes read slot 0 (in range) and drop it
+  const double ft = active ? (f ? f[g] : 1.0) : 0.0;
+  const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
+  const cplx be = active ? beta[g] : cmake(0.0, 0.0);
+  const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
+  for (int r0 = 0; r0 < nr; r0 += R) {
+    // entry ranges of the R rows (uniform), this lane's row, the longest count
+    int e0 = 0, e1 = 0, cmax = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int a0 = __builtin_amdgcn_readlane(optr, min(r0 + q, nr));
+      const int a1 = __builtin_amdgcn_readlane(optr, min(r0 + q + 1, nr));
+      cmax = max(cmax, a1 - a0);
+      if (sub == q) {
+        e0 = a0;
+        e1 = a1;
+      }
+    }
+    const int r = r0 + sub;
+    const int cnt = e1 - e0;  // 0 past the last row (both pointers clamp to nr)
+    double sx = 0.0, sy = 0.0;
+    // pad entries gather the row's own line (read below anyway) with weight 0
+    const int own = (int)min<int64_t>(s0 + r, S.n - 1);
+    for (int base = 0; base < cmax; base += SW) {
+      const int here = cnt - base;  // entries of this lane's row in the window
+      const int ci = (tt < here) ? S.inner[e0 + base + tt] : own;
+      const cplx cv = (tt < here) ? S.vals[e0 + base + tt] : cmake(0.0, 0.0);
+      const int wcnt = min(SW, cmax - base);
+      for (int k0 = 0; k0 < wcnt; k0 += 8) {
+        cplx x[8];
+        // the eight gathers first, then each entry's value as it is summed
+        // (values pulled early would hold 32 more registers across the loads)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = k0 + i;  // < SW: k0 <= SW - 8 (both multiples of 8)
+          const int col = __shfl(ci, sub * SW + k);
+          x[i] = load_global(src + (int64_t)col * lds + tl);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = k0 + i;
+          const bool real = active && k < here;
+          const double px = __shfl(cv.x, sub * SW + k);  // every lane takes part in the permute
+          const double py = __shfl(cv.y, sub * SW + k);
+          const double vx = real ? px : 0.0;
+          const double vy = real ? py : 0.0;
+          sx += vx * x[i].x + vy * x[i].y;
+          sy += vx * x[i].y - vy * x[i].x;
+        }
+      }
+    }
+    if (active && r < nr) {
+      const int64_t s = s0 + r;
+      const cplx x0 = load_global(src + s * lds + t);
+      const cplx x = cmake(x0.x * ft, x0.y * ft);
+      const cplx dg = cmake(al.x * S.D[s] + be.x, al.y * S.D[s] + be.y);
+      const cplx dx = cmul(dg, x);
+      if (Vout) {
+        __builtin_nontemporal_store(x.x, &Vout[S.vo + s * m + t].x);
+        __builtin_nontemporal_store(x.y, &Vout[S.vo + s * m + t].y);
+      }
+      __builtin_nontemporal_store(sx * ft + dx.x, &Wout[S.vo + s * m + t].x);
+      __builtin_nontemporal_store(sy * ft + dx.y, &Wout[S.vo + s * m + t].y);
+    }
+  }
+}
+
 // ---------------------------------------------------------- delayed CGS2
 // Batched form of the delayed classical Gram-Schmidt with one
 // reorthogonalisation (the single-vector form and its derivation are in
@@ -899,6 +1122,38 @@ struct SpmmProduct {
                   const cplx* beta, cplx* Vout, cplx* Wout) const {
     const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
     g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+  }
+};
+
+// The same product by one of four kernels, chosen by timing: 0
+// g_spmm_kernel, 1 g_spmm_flat_kernel<4, true>, 2 g_spmm_sw_kernel<32>, 3
+// g_spmm_flat_kernel<8, false>; blocks counts ceil(n / rows) * ceil(m /
+// width) items per sector, as the sectors' blk0 do.  (16-slot segments and
+// 16-row streams were slower than these on the L = 8 trajectory sectors.)
+constexpr int SPMM_VARIANTS = 4;
+inline int spmm_width(int var) { return var == 2 ? 32 : TT; }
+inline int spmm_rows(int var) { return var == 3 ? 8 : SPMM_RB; }
+struct SpmmVarProduct {
+  const GSec* gs;
+  int ns;
+  int64_t blocks;
+  int var;
+  void operator()(hipStream_t st, int from_x, const cplx* Wsrc, const double* f, const cplx* alpha,
+                  const cplx* beta, cplx* Vout, cplx* Wout) const {
+    const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
+    switch (var) {
+      case 1:
+        g_spmm_flat_kernel<4, true><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        break;
+      case 2:
+        g_spmm_sw_kernel<32><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        break;
+      case 3:
+        g_spmm_flat_kernel<8, false><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        break;
+      default:
+        g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+    }
   }
 };
 

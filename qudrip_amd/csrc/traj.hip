@@ -307,7 +307,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&d_action, sizeof(int) * ntraj) || !dalloc((void**)&d_slot2traj, sizeof(int) * ntraj) ||
       !dalloc((void**)&d_dens_all, sizeof(double) * ntraj * (M + 1)) ||
       !dalloc((void**)&d_gtraj, sizeof(double) * ntraj) || !dalloc((void**)&d_dsum, sizeof(double) * M) ||
-      !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_scale, sizeof(double) * ntraj) ||
+      !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_scale, sizeof(double) * (ntraj + 1)) ||
       !dalloc((void**)&d_alpha, sizeof(cplx) * ntraj) || !dalloc((void**)&d_beta, sizeof(cplx) * ntraj) ||
       !dalloc((void**)&d_red, sizeof(cplx) * (size_t)(2 * K + 2) * ntraj) ||
       !dalloc((void**)&d_coef2, sizeof(cplx) * (size_t)(2 * K) * ntraj) ||
@@ -329,6 +329,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     return qd_set_error(ctx, QD_E_OOM, "trajectory workspace (%d trajectories, dim %lld, k %d)", ntraj,
                         (long long)nmax, k_rt);
   }
+  // per-slot operands of the product, defined before the first tuning run reads them
+  zero_kernel<<<g1((ntraj + 1) / 2, 256), 256, 0, st>>>((ntraj + 1) / 2, (cplx*)d_scale);  // ntraj + 1 doubles
+  zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, d_alpha);
+  zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, d_beta);
   if (use_lz) {
     // the Lanczos form writes only the three diagonals of each Hessenberg
     zero_kernel<<<g1((int64_t)K * (K + 1) * ntraj, 256), 256, 0, st>>>((int64_t)K * (K + 1) * ntraj, d_hess);
@@ -368,6 +372,14 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   int ns = 0;
   int64_t VS = 0, tiles = 0, blocks = 0, ngroups = 0;
   int occupancy_target = 2048;  // waves per sector for the reduction tiles (tuned below)
+  int spmm_var = 0;              // product kernel (SpmmVarProduct; tuned below)
+  const char* var_env = std::getenv("QD_TRAJ_SPMM");
+  const int var_forced = var_env ? std::atoi(var_env) : -1;
+  if (var_env && (var_forced < 0 || var_forced >= SPMM_VARIANTS)) {
+    cleanup();
+    return qd_set_error(ctx, QD_E_ARG, "QD_TRAJ_SPMM must be 0..%d", SPMM_VARIANTS - 1);
+  }
+  if (var_forced >= 0) spmm_var = var_forced;
 
   // grouped-launch tables from the current sector membership
   auto build_groups = [&]() {
@@ -402,7 +414,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       vo += S.n * S.m;
       co += (int64_t)g.nch * S.m;
       tiles += (int64_t)g.nch * g.groups;
-      blocks += (S.n + SPMM_RB - 1) / SPMM_RB * g.groups;
+      blocks += (S.n + spmm_rows(spmm_var) - 1) / spmm_rows(spmm_var) *
+                ((S.m + spmm_width(spmm_var) - 1) / spmm_width(spmm_var));
       ngroups += g.groups;
     }
     ns = (int)h_gsec.size();
@@ -435,11 +448,11 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     const int Mtot = ntraj;
     slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
                                                        d_scale);
+    const SpmmVarProduct prod{d_gsec, ns, blocks, spmm_var};
     if (use_lz)
-      mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz,
-                        SpmmProduct{d_gsec, ns, blocks});
+      mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz, prod);
     else
-      mrhs_arnoldi(st, d_gsec, ns, tiles, blocks, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb);
+      mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod);
     mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
     QD_CHECK_LAUNCH(ctx);
     return QD_OK;
@@ -490,9 +503,41 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     occupancy_target = best_c;
   };
   tune_tiles();
+  // Product kernel, timed the same way on the product from X into the pools
+  // (X is only read; the pools are rewritten by the next step).
+  auto tune_spmm = [&]() {
+    if (var_forced >= 0) return;
+    float best = 1e30f;
+    int best_var = spmm_var;
+    for (int var = 0; var < SPMM_VARIANTS; ++var) {
+      spmm_var = var;
+      build_groups();
+      const SpmmVarProduct prod{d_gsec, ns, blocks, var};
+      prod(st, 1, nullptr, d_scale, d_alpha, d_beta, d_V, d_W);
+      hipEventRecord(ev0, st);
+      for (int r = 0; r < 2; ++r) prod(st, 1, nullptr, d_scale, d_alpha, d_beta, d_V, d_W);
+      hipEventRecord(ev1, st);
+      hipEventSynchronize(ev1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev0, ev1);
+      if (tune_log)
+        std::fprintf(stderr, "[qd] trajectory product: kernel %d (%d rows x %d slots), %lld items over %d sectors, %.1f us\n",
+                     var, spmm_rows(var), spmm_width(var), (long long)blocks, ns, 1000.0 * ms / 2);
+      if (ms < best) {
+        best = ms;
+        best_var = var;
+      }
+    }
+    spmm_var = best_var;
+    build_groups();
+  };
+  tune_spmm();
 
   for (int step = 0; step <= nsteps; ++step) {
-    if (step > 0 && step == nsteps / 4) tune_tiles();
+    if (step > 0 && step == nsteps / 4) {
+      tune_tiles();
+      tune_spmm();
+    }
     build_groups();
     rc = densities(step);
     if (rc) break;

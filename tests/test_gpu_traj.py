@@ -150,3 +150,34 @@ def test_traj_site_dependent_rates_arnoldi_against_oracle(ctx):
     assert g["lanczos"] == 0
     assert (g["log"] == log).all()
     np.testing.assert_allclose(g["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("var", ["1", "2", "3"])
+def test_traj_product_kernels_agree(ctx, var):
+    """The product kernels (QD_TRAJ_SPMM: 1 and 3 one entry stream per wave
+    over 4 and 8 rows, 2 over 32-slot row segments) sum the same entries in
+    the same order as kernel 0: same jumps, densities equal to rounding, over
+    sectors of every size (L = 6, 130 trajectories: partial slot groups and
+    row blocks in every sector)."""
+    import os
+    L, nup, ndn = 6, 3, 3
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(400, 27)
+    ch = jump_channels(L, 0.3, 0.2)
+    args = (130, 60, 0.05, 13, 11)
+
+    def run(v):
+        old = os.environ.get("QD_TRAJ_SPMM")
+        os.environ["QD_TRAJ_SPMM"] = v
+        try:
+            return traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True)
+        finally:
+            if old is None:
+                os.environ.pop("QD_TRAJ_SPMM", None)
+            else:
+                os.environ["QD_TRAJ_SPMM"] = old
+    ref = run("0")
+    got = run(var)
+    assert got["losses"] > 50
+    assert (got["log"] == ref["log"]).all()
+    np.testing.assert_allclose(got["mean"], ref["mean"], rtol=0, atol=1e-13)
