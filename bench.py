@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--trace-summary", default="",
                     help="summarise a rocprofv3 --kernel-trace output directory (dispatches, span, GPU busy as "
                          "the union of the kernel intervals), print one JSON line and exit (no GPU)")
+    ap.add_argument("--trace-window", default="",
+                    help="with --trace-summary: only from the first to the last dispatch of a kernel whose name "
+                         "starts with this prefix, with the kernel time per kernel in that window")
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time the CPU restatement on the full-size workloads: the XXZ N=24 Lanczos "
                          "ground state (k = 150) and one Hubbard 4x4 matrix-free product (minutes)")
@@ -157,7 +160,7 @@ def main():
         return
 
     if args.trace_summary:
-        print(json.dumps(trace_summary(args.trace_summary)), flush=True)
+        print(json.dumps(trace_summary(args.trace_summary, args.trace_window)), flush=True)
         return
 
     if args.kron_products:
@@ -338,28 +341,60 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def trace_summary(outdir):
+def kernel_short_name(name):
+    """`void (anonymous namespace)::k<4, 8>(args)` -> `k<4, 8>`."""
+    n = name.replace("void ", "").replace("(anonymous namespace)::", "")
+    depth = 0
+    for i, c in enumerate(n):
+        if c == "<":
+            depth += 1
+        elif c == ">":
+            depth -= 1
+        elif c == "(" and depth == 0:
+            return n[:i]
+    return n
+
+
+def trace_summary(outdir, window=""):
     """Dispatch count, span and GPU busy time of a rocprofv3 kernel trace
-    (`run_kernel_trace.csv` under `outdir`): busy is the length of the union
-    of the kernel intervals, so kernels that overlap on several streams count
-    once."""
+    (`run_kernel_trace.csv`, or its gzip, under `outdir`): busy is the length
+    of the union of the kernel intervals, so kernels that overlap on several
+    streams count once.  window: only the dispatches from the first to the
+    last one whose kernel name starts with this prefix (one engine's run
+    inside a longer program), with the kernel time per kernel there."""
     import csv
+    import gzip
     path = os.path.join(outdir, "run_kernel_trace.csv")
-    with open(path, newline="") as fh:
-        iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in csv.DictReader(fh))
-    if not iv:
+    fh = open(path, newline="") if os.path.exists(path) else gzip.open(path + ".gz", "rt", newline="")
+    with fh:
+        rows = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kernel_short_name(r["Kernel_Name"]))
+                      for r in csv.DictReader(fh))
+    if window:
+        hit = [i for i, r in enumerate(rows) if r[2].startswith(window)]
+        rows = rows[hit[0]:hit[-1] + 1] if hit else []
+    if not rows:
         return {"dispatches": 0, "span_s": 0.0, "busy_s": 0.0, "busy_frac": 0.0}
-    busy, (cs, ce) = 0, iv[0]
-    for s, e in iv[1:]:
+    busy, cs, ce = 0, rows[0][0], rows[0][1]
+    for s, e, _ in rows[1:]:
         if s > ce:
             busy += ce - cs
             cs, ce = s, e
         else:
             ce = max(ce, e)
     busy += ce - cs
-    span = iv[-1][1] - iv[0][0]
-    return {"dispatches": len(iv), "span_s": span / 1e9, "busy_s": busy / 1e9,
-            "busy_frac": busy / span if span else 0.0}
+    span = max(e for _, e, _ in rows) - rows[0][0]
+    out = {"dispatches": len(rows), "span_s": span / 1e9, "busy_s": busy / 1e9,
+           "busy_frac": busy / span if span else 0.0}
+    if window:
+        per = {}
+        for s, e, k in rows:
+            c = per.setdefault(k, [0, 0])
+            c[0] += 1
+            c[1] += e - s
+        out["window"] = window
+        out["kernels"] = [{"kernel": k, "calls": c, "ms": round(t / 1e6, 3)}
+                          for k, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1])]
+    return out
 
 
 def kron_products(count):

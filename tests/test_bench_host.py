@@ -66,3 +66,23 @@ def test_trace_summary_merges_overlapping_kernels(tmp_path):
     assert abs(out["span_s"] - 410e-9) < 1e-15
     assert abs(out["busy_s"] - (160 + 100 + 10) * 1e-9) < 1e-15
     assert abs(out["busy_frac"] - 270 / 410) < 1e-12
+
+
+def test_trace_summary_window_and_gzip(tmp_path):
+    import gzip
+    us = 1000  # ns per microsecond
+    rows = [("void (anonymous namespace)::other(int)", 0, 50 * us),
+            ("void (anonymous namespace)::g_a<4, true>(int*)", 100 * us, 200 * us),
+            ("void (anonymous namespace)::other(int)", 210 * us, 230 * us),
+            ("void (anonymous namespace)::g_b(int)", 240 * us, 300 * us),
+            ("void (anonymous namespace)::other(int)", 400 * us, 450 * us)]
+    with gzip.open(tmp_path / "run_kernel_trace.csv.gz", "wt") as fh:
+        fh.write("Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for k, s, e in rows:
+            fh.write(f'"{k}",{s},{e}\n')
+    out = bench.trace_summary(str(tmp_path), "g_")
+    assert out["dispatches"] == 3
+    assert abs(out["span_s"] - 200e-6) < 1e-12 and abs(out["busy_s"] - 180e-6) < 1e-12
+    names = {k["kernel"]: (k["calls"], k["ms"]) for k in out["kernels"]}
+    assert names == {"g_a<4, true>": (1, 0.1), "g_b": (1, 0.06), "other": (1, 0.02)}
+    assert bench.trace_summary(str(tmp_path))["dispatches"] == 5
