@@ -516,6 +516,25 @@ def trajectories(ctx, args):
            "closed_form_z": round(abs(N.mean() - expect) / sigma, 3),
            "density_site0_up_T": round(float(r["mean"][-1][0]), 6),
            "density_site0_up_T_sigma": round(float(np.sqrt(max(r["var"][-1][0], 0.0) / args.traj)), 6)}
+    # the same run with the Krylov step fully reorthogonalised (the Arnoldi
+    # form, delayed CGS2 over the whole basis): its time, and how far the
+    # Lanczos form's results are from it
+    if r["lanczos"]:
+        old_form = os.environ.get("QD_TRAJ_KRYLOV")
+        os.environ["QD_TRAJ_KRYLOV"] = "arnoldi"
+        try:
+            ra = WL.hubbard_trajectories(ctx, ntraj=args.traj, want_log=True)
+        finally:
+            if old_form is None:
+                os.environ.pop("QD_TRAJ_KRYLOV", None)
+            else:
+                os.environ["QD_TRAJ_KRYLOV"] = old_form
+        out["arnoldi_form"] = {"run_s": round(ra["run_s"], 3),
+                               "traj_steps_per_s": round(args.traj * nsteps / ra["run_s"], 1),
+                               "logs_equal": bool((ra["log"] == r["log"]).all()),
+                               "max_abs_density_diff": float(np.abs(ra["mean"] - r["mean"]).max())}
+        log(f"trajectories, Arnoldi form (full reorthogonalisation): {ra['run_s']:.3f} s, logs equal "
+            f"{out['arnoldi_form']['logs_equal']}, max |d density| {out['arnoldi_form']['max_abs_density_diff']:.1e}")
     # plain: one trajectory at a time through the single-vector evolve (same
     # random streams, so the jump sequences must agree), on a sample
     ns_plain = 8
