@@ -196,7 +196,9 @@ __global__ void __launch_bounds__(RW * TT)
 
 // Multi-RHS product with the normalisation of the source folded in, all sectors:
 //   x = f_t src[s,t] (f = NULL: 1),  Vout[s,t] = x (Vout may be NULL),
-//   Wout[s,t] = (H x)[s] + (alpha_t D[s] + beta_t) x[s]
+//   Wout[s,t] = (H x)[s] + (alpha_t D[s] + beta_t) x[s],
+//   apart[item 64 + lane] = sum over the item's rows s of conj(x[s,t]) Wout[s,t]
+//   (apart may be NULL; item = the block's work item, slot t = its lane)
 // H from the ColMajor arrays of a Hermitian operator (pull form: row s of H is
 // column s conjugated); alpha, beta per slot.
 // src is the sector's X (from_x) or the W pool.  One wave per (SPMM_RB rows,
@@ -219,7 +221,7 @@ __device__ inline double readlane_d(double v, int k) {
 __global__ void __launch_bounds__(TT)
     g_spmm_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
                   const double* __restrict__ f, const cplx* __restrict__ alpha, const cplx* __restrict__ beta,
-                  cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
+                  cplx* __restrict__ Vout, cplx* __restrict__ Wout, cplx* __restrict__ apart) {
   const int64_t per_xcd = (int64_t)gridDim.x / 8;
   const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (b >= nblocks) return;
@@ -240,6 +242,7 @@ __global__ void __launch_bounds__(TT)
   const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
   const cplx be = active ? beta[g] : cmake(0.0, 0.0);
   const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
+  cplx ap = cmake(0.0, 0.0);  // sum over the item's rows of conj(x) W (apart)
   for (int r = 0; r < nr; ++r) {
     const int64_t s = s0 + r;
     const int e0 = __builtin_amdgcn_readlane(optr, r);
@@ -280,14 +283,17 @@ __global__ void __launch_bounds__(TT)
         __builtin_nontemporal_store(x.x, &Vout[S.vo + s * m + t].x);
         __builtin_nontemporal_store(x.y, &Vout[S.vo + s * m + t].y);
       }
-      __builtin_nontemporal_store(sx * ft + dx.x, &Wout[S.vo + s * m + t].x);
-      __builtin_nontemporal_store(sy * ft + dx.y, &Wout[S.vo + s * m + t].y);
+      const cplx wv = cmake(sx * ft + dx.x, sy * ft + dx.y);
+      __builtin_nontemporal_store(wv.x, &Wout[S.vo + s * m + t].x);
+      __builtin_nontemporal_store(wv.y, &Wout[S.vo + s * m + t].y);
+      ap = cadd(ap, cdotmul(x, wv));
     }
   }
+  if (apart) apart[b * TT + lane] = ap;
 }
 
-// p[0] through a global-space pointer: flat loads also count in lgkmcnt, so
-// a wait for an LDS permute would wait for every gather in flight too
+// p[0] through a global-space pointer (a global load counts in vmcnt only;
+// a flat load also counts in lgkmcnt)
 typedef double qd_d2 __attribute__((ext_vector_type(2)));
 __device__ inline cplx load_global(const cplx* p) {
   const qd_d2 v = *(const __attribute__((address_space(1))) qd_d2*)p;
@@ -313,12 +319,13 @@ __device__ inline void store_nt(cplx* p, cplx v) {
 // values of the diagonal terms are loaded up front (PRE) or at the flush.
 // Items are RB rows of one 64-slot group (blk0 / nblocks count ceil(n / RB)
 // * groups per sector).  Same per-row entry order and sums as g_spmm_kernel
-// (pads added none).
+// (pads added none), and the same apart.
 template <int RB, bool PRE>
 __global__ void __launch_bounds__(TT)
     g_spmm_flat_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x,
                        const cplx* __restrict__ Wsrc, const double* __restrict__ f, const cplx* __restrict__ alpha,
-                       const cplx* __restrict__ beta, cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
+                       const cplx* __restrict__ beta, cplx* __restrict__ Vout, cplx* __restrict__ Wout,
+                       cplx* __restrict__ apart) {
   static_assert(RB < TT, "row pointers of an item fit in one wave");
   const int64_t per_xcd = (int64_t)gridDim.x / 8;
   const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -352,6 +359,7 @@ __global__ void __launch_bounds__(TT)
   int r = 0;
   int rend = __builtin_amdgcn_readlane(optr, 1);
   double sx = 0.0, sy = 0.0;
+  cplx ap = cmake(0.0, 0.0);  // sum over the item's rows of conj(x) W (apart)
   // outputs of row rr (uniform, rr < nr)
   auto flush = [&](int rr) {
     const int64_t s = s0 + rr;
@@ -370,7 +378,9 @@ __global__ void __launch_bounds__(TT)
       const cplx dg = cmake(al.x * d + be.x, al.y * d + be.y);
       const cplx dx = cmul(dg, x);
       if (Vout) store_nt(Vout + S.vo + s * m + t, x);
-      store_nt(Wout + S.vo + s * m + t, cmake(sx * ft + dx.x, sy * ft + dx.y));
+      const cplx wv = cmake(sx * ft + dx.x, sy * ft + dx.y);
+      store_nt(Wout + S.vo + s * m + t, wv);
+      ap = cadd(ap, cdotmul(x, wv));
     }
     sx = 0.0;
     sy = 0.0;
@@ -404,109 +414,7 @@ __global__ void __launch_bounds__(TT)
     }
   }
   for (; r < nr; ++r) flush(r);
-}
-
-// g_spmm_kernel over SW-slot segments (SW = 32 or 16): a wave covers R = 64 /
-// SW rows at once, lane = sub SW + tt holding slot tt of row sub, so a
-// work item is (sector, SW-slot group, SPMM_RB rows) and blk0 / nblocks
-// count ceil(n / SPMM_RB) * ceil(m / SW) items per sector.  The source block
-// of one slot group (n rows x SW slots) is 64 / SW times smaller than with
-// 64 slots, so an XCD working through one group keeps it in its L2 (n = 4900
-// rows: 2.5 MB at 32 slots, 4.9 MB at 64, against 4 MB).  The lanes of a
-// segment load its row's entries (one per lane) and every lane pulls entry k
-// of its own row with ds_bpermute (__shfl: column and value, five reads); a
-// segment whose row has fewer entries than the longest of the R rows masks
-// its loads off for the rest.  The entries of a row are summed in the same
-// order as in g_spmm_kernel (the padding adds zeros).
-template <int SW>
-__global__ void __launch_bounds__(TT)
-    g_spmm_sw_kernel(const GSec* __restrict__ gs, int ns, int64_t nblocks, int from_x, const cplx* __restrict__ Wsrc,
-                     const double* __restrict__ f, const cplx* __restrict__ alpha, const cplx* __restrict__ beta,
-                     cplx* __restrict__ Vout, cplx* __restrict__ Wout) {
-  constexpr int R = TT / SW;
-  static_assert(SW % 8 == 0 && SPMM_RB % R == 0, "segments of whole entry batches, whole row groups");
-  const int64_t per_xcd = (int64_t)gridDim.x / 8;
-  const int64_t b = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (b >= nblocks) return;
-  const int si = find_sec(ns, b, [&](int i) { return gs[i].blk0; });
-  const GSec& S = gs[si];
-  const int lane = threadIdx.x;
-  const int sub = lane / SW, tt = lane % SW;
-  const int64_t nrb = (S.n + SPMM_RB - 1) / SPMM_RB;
-  const int64_t local = b - S.blk0;
-  const int64_t s0 = (local % nrb) * SPMM_RB;
-  const int nr = (int)min<int64_t>(SPMM_RB, S.n - s0);
-  const int t = (int)(local / nrb) * SW + tt;
-  const int m = S.m;
-  const bool active = t < m;
-  const cplx* __restrict__ src = from_x ? S.X : Wsrc + S.vo;
-  const int64_t lds = from_x ? S.ldx : m;
-  const int g = S.so + (active ? t : 0);
-  const int tl = active ? t : 0;  // inactive lanes read slot 0 (in range) and drop it
-  const double ft = active ? (f ? f[g] : 1.0) : 0.0;
-  const cplx al = active ? alpha[g] : cmake(0.0, 0.0);
-  const cplx be = active ? beta[g] : cmake(0.0, 0.0);
-  const int optr = (lane <= nr) ? S.outer[s0 + lane] : 0;
-  for (int r0 = 0; r0 < nr; r0 += R) {
-    // entry ranges of the R rows (uniform), this lane's row, the longest count
-    int e0 = 0, e1 = 0, cmax = 0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int a0 = __builtin_amdgcn_readlane(optr, min(r0 + q, nr));
-      const int a1 = __builtin_amdgcn_readlane(optr, min(r0 + q + 1, nr));
-      cmax = max(cmax, a1 - a0);
-      if (sub == q) {
-        e0 = a0;
-        e1 = a1;
-      }
-    }
-    const int r = r0 + sub;
-    const int cnt = e1 - e0;  // 0 past the last row (both pointers clamp to nr)
-    double sx = 0.0, sy = 0.0;
-    // pad entries gather the row's own line (read below anyway) with weight 0
-    const int own = (int)min<int64_t>(s0 + r, S.n - 1);
-    for (int base = 0; base < cmax; base += SW) {
-      const int here = cnt - base;  // entries of this lane's row in the window
-      const int ci = (tt < here) ? S.inner[e0 + base + tt] : own;
-      const cplx cv = (tt < here) ? S.vals[e0 + base + tt] : cmake(0.0, 0.0);
-      const int wcnt = min(SW, cmax - base);
-      for (int k0 = 0; k0 < wcnt; k0 += 8) {
-        cplx x[8];
-        // the eight gathers first, then each entry's value as it is summed
-        // (values pulled early would hold 32 more registers across the loads)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int k = k0 + i;  // < SW: k0 <= SW - 8 (both multiples of 8)
-          const int col = __shfl(ci, sub * SW + k);
-          x[i] = load_global(src + (int64_t)col * lds + tl);
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int k = k0 + i;
-          const bool real = active && k < here;
-          const double px = __shfl(cv.x, sub * SW + k);  // every lane takes part in the permute
-          const double py = __shfl(cv.y, sub * SW + k);
-          const double vx = real ? px : 0.0;
-          const double vy = real ? py : 0.0;
-          sx += vx * x[i].x + vy * x[i].y;
-          sy += vx * x[i].y - vy * x[i].x;
-        }
-      }
-    }
-    if (active && r < nr) {
-      const int64_t s = s0 + r;
-      const cplx x0 = load_global(src + s * lds + t);
-      const cplx x = cmake(x0.x * ft, x0.y * ft);
-      const cplx dg = cmake(al.x * S.D[s] + be.x, al.y * S.D[s] + be.y);
-      const cplx dx = cmul(dg, x);
-      if (Vout) {
-        __builtin_nontemporal_store(x.x, &Vout[S.vo + s * m + t].x);
-        __builtin_nontemporal_store(x.y, &Vout[S.vo + s * m + t].y);
-      }
-      __builtin_nontemporal_store(sx * ft + dx.x, &Wout[S.vo + s * m + t].x);
-      __builtin_nontemporal_store(sy * ft + dx.y, &Wout[S.vo + s * m + t].y);
-    }
-  }
+  if (apart) apart[b * TT + lane] = ap;
 }
 
 // ---------------------------------------------------------- delayed CGS2
@@ -1121,38 +1029,38 @@ struct SpmmProduct {
   void operator()(hipStream_t st, int from_x, const cplx* Wsrc, const double* f, const cplx* alpha,
                   const cplx* beta, cplx* Vout, cplx* Wout) const {
     const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
-    g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+    g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout, nullptr);
   }
 };
 
-// The same product by one of four kernels, chosen by timing: 0
-// g_spmm_kernel, 1 g_spmm_flat_kernel<4, true>, 2 g_spmm_sw_kernel<32>, 3
-// g_spmm_flat_kernel<8, false>; blocks counts ceil(n / rows) * ceil(m /
-// width) items per sector, as the sectors' blk0 do.  (16-slot segments and
-// 16-row streams were slower than these on the L = 8 trajectory sectors.)
-constexpr int SPMM_VARIANTS = 4;
-inline int spmm_width(int var) { return var == 2 ? 32 : TT; }
-inline int spmm_rows(int var) { return var == 3 ? 8 : SPMM_RB; }
+// The same product by one of three kernels, chosen by timing: 0
+// g_spmm_kernel, 1 g_spmm_flat_kernel<4, true>, 2 g_spmm_flat_kernel<8,
+// false>; blocks counts ceil(n / rows) * groups items per sector, as the
+// sectors' blk0 do.  apart (or NULL): the kernels' per-item sums of conj(x)
+// W, blocks x 64 values.  (Products over 32- and 16-slot row segments and
+// 16-row streams were slower on the L = 8 trajectory sectors.)
+constexpr int SPMM_VARIANTS = 3;
+inline int spmm_rows(int var) { return var == 2 ? 8 : SPMM_RB; }
 struct SpmmVarProduct {
   const GSec* gs;
   int ns;
   int64_t blocks;
   int var;
+  cplx* apart;
   void operator()(hipStream_t st, int from_x, const cplx* Wsrc, const double* f, const cplx* alpha,
                   const cplx* beta, cplx* Vout, cplx* Wout) const {
     const unsigned sgrid = (unsigned)((blocks + 7) / 8 * 8);
     switch (var) {
       case 1:
-        g_spmm_flat_kernel<4, true><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        g_spmm_flat_kernel<4, true>
+            <<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout, apart);
         break;
       case 2:
-        g_spmm_sw_kernel<32><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
-        break;
-      case 3:
-        g_spmm_flat_kernel<8, false><<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        g_spmm_flat_kernel<8, false>
+            <<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout, apart);
         break;
       default:
-        g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout);
+        g_spmm_kernel<<<sgrid, TT, 0, st>>>(gs, ns, blocks, from_x, Wsrc, f, alpha, beta, Vout, Wout, apart);
     }
   }
 };
@@ -1280,10 +1188,29 @@ __device__ inline cplx lz_chunk_sum(const GSec& S, const cplx* __restrict__ part
   return s;
 }
 
-// Lanczos step j after the pass over (u_j, W): alpha_j = inv_j <u_j, W> -
-// beta_j <v_j, v_{j-1}> -> Hess[j][j]; ca = alpha_j inv_j, cb = beta_j
-// inv_{j-1} (0 for j = 0 and once the space has ended).  One block per
-// 64-slot group.
+// Lanczos step j for slot g from s = <v_j, W> (= inv_j <u_j, W>): alpha_j =
+// s - beta_j <v_j, v_{j-1}> -> Hess[j][j]; ca = alpha_j inv_j, cb = beta_j
+// inv_{j-1} (0 for j = 0 and once the space has ended)
+__device__ inline void lz_alpha_finish(cplx s, int g, int Mtot, int j, int K, cplx* __restrict__ hess,
+                                       const LzState& lz) {
+  double iv = 1.0, cbv = 0.0;
+  cplx al = s;
+  if (j == 0) {
+    lz.inv[g] = 1.0;
+  } else {
+    iv = lz.inv[(int64_t)j * Mtot + g];
+    const double bj = lz.beta[(int64_t)j * Mtot + g];
+    const cplx ip = lz.ip[(int64_t)j * Mtot + g];
+    al = cmake(s.x - bj * ip.x, s.y - bj * ip.y);
+    cbv = iv > 0.0 ? bj * lz.inv[(int64_t)(j - 1) * Mtot + g] : 0.0;
+  }
+  hess[((int64_t)(j * (K + 1) + j)) * Mtot + g] = al;
+  lz.ca[g] = crmul(iv, al);
+  lz.cb[g] = cbv;
+}
+
+// Lanczos step j after the pass over (u_j, W): s = inv_j <u_j, W>, then
+// lz_alpha_finish.  One block per 64-slot group.
 __global__ void __launch_bounds__(RW * TT)
     g_lz_alpha_kernel(const GSec* __restrict__ gs, int ns, int Mtot, const cplx* __restrict__ part, int j, int K,
                       cplx* __restrict__ hess, LzState lz) {
@@ -1296,20 +1223,35 @@ __global__ void __launch_bounds__(RW * TT)
   const cplx s = lz_chunk_sum(S, part, 1, 0, t, wv, lane, sh);
   if (wv != 0 || t >= S.m) return;
   const int g = S.so + t;
-  double iv = 1.0, cbv = 0.0;
-  cplx al = s;
-  if (j == 0) {
-    lz.inv[g] = 1.0;
-  } else {
-    iv = lz.inv[(int64_t)j * Mtot + g];
-    const double bj = lz.beta[(int64_t)j * Mtot + g];
-    const cplx ip = lz.ip[(int64_t)j * Mtot + g];
-    al = cmake(s.x * iv - bj * ip.x, s.y * iv - bj * ip.y);
-    cbv = iv > 0.0 ? bj * lz.inv[(int64_t)(j - 1) * Mtot + g] : 0.0;
-  }
-  hess[((int64_t)(j * (K + 1) + j)) * Mtot + g] = al;
-  lz.ca[g] = crmul(iv, al);
-  lz.cb[g] = cbv;
+  const double iv = j == 0 ? 1.0 : lz.inv[(int64_t)j * Mtot + g];
+  lz_alpha_finish(crmul(iv, s), g, Mtot, j, K, hess, lz);
+}
+
+// The same from the product's per-item sums of conj(x) W (x = v_j, the
+// product's normalised source): s = the sum over the items of the slot's
+// 64-slot group, item order, RW waves as in lz_chunk_sum.  Items of `rows`
+// rows, group-major from the sector's blk0.  One block per 64-slot group.
+__global__ void __launch_bounds__(RW * TT)
+    g_lz_alpha_sp_kernel(const GSec* __restrict__ gs, int ns, int Mtot, const cplx* __restrict__ apart, int rows,
+                         int j, int K, cplx* __restrict__ hess, LzState lz) {
+  __shared__ cplx sh[RW][TT];
+  const int64_t gi = blockIdx.x;
+  const int si = find_sec(ns, gi, [&](int i) { return gs[i].grp0; });
+  const GSec& S = gs[si];
+  const int lane = threadIdx.x & (TT - 1), wv = threadIdx.x / TT;
+  const int gr = (int)(gi - S.grp0);
+  const int t = gr * TT + lane;
+  const int64_t nrb = (S.n + rows - 1) / rows;
+  const cplx* __restrict__ pb = apart + (S.blk0 + (int64_t)gr * nrb) * TT + lane;
+  cplx s = cmake(0.0, 0.0);
+#pragma unroll 4
+  for (int64_t rb = wv; rb < nrb; rb += RW) s = cadd(s, pb[rb * TT]);
+  sh[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || t >= S.m) return;
+  s = sh[0][lane];
+  for (int w = 1; w < RW; ++w) s = cadd(s, sh[w][lane]);
+  lz_alpha_finish(s, S.so + t, Mtot, j, K, hess, lz);
 }
 
 // after the update of step j: beta_{j+1} = |u_{j+1}|, inv_{j+1} (0 below
@@ -1349,10 +1291,14 @@ __global__ void g_lz_scale_kernel(int Mtot, int K, const double* __restrict__ in
 // y = expm(-i dt T_t) e_1 scaled for the unnormalised vectors, so that
 // mrhs_combine writes V y into X as after mrhs_arnoldi.  Hess must hold zeros
 // off the three diagonals (only those are written here).
+// apart (or NULL): the product writes per-item sums of conj(x) W there
+// (items of apart_rows rows), which give alpha_j without a pass over (u_j,
+// W); otherwise that pass runs.
 template <class Prod>
 inline void mrhs_lanczos_with(hipStream_t st, const GSec* d_gsec, int ns, int64_t tiles, int64_t ngroups,
                               int64_t VS, int Mtot, int K, double dt, const double* f, const cplx* alpha,
-                              const cplx* beta, const MrhsBufs& b, const LzState& lz, const Prod& prod) {
+                              const cplx* beta, const MrhsBufs& b, const LzState& lz, const Prod& prod,
+                              const cplx* apart = nullptr, int apart_rows = 0) {
   const unsigned tl = (unsigned)tiles;
   cplx* W = b.W;
   for (int j = 0; j < K; ++j) {
@@ -1360,8 +1306,13 @@ inline void mrhs_lanczos_with(hipStream_t st, const GSec* d_gsec, int ns, int64_
       prod(st, 1, nullptr, f, alpha, beta, b.V, W);
     else
       prod(st, 0, b.V + (int64_t)j * VS, lz.inv + (int64_t)j * Mtot, alpha, beta, nullptr, W);
-    g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, b.V + (int64_t)j * VS, VS, W, nullptr, Mtot, b.part);
-    g_lz_alpha_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, b.part, j, K, b.hess, lz);
+    if (apart) {
+      g_lz_alpha_sp_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, apart, apart_rows, j, K, b.hess,
+                                                                  lz);
+    } else {
+      g_pass_kernel<1, 0><<<tl, TT, 0, st>>>(d_gsec, ns, b.V + (int64_t)j * VS, VS, W, nullptr, Mtot, b.part);
+      g_lz_alpha_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, b.part, j, K, b.hess, lz);
+    }
     if (j + 1 < K) {
       g_lz_update_kernel<<<tl, TT, 0, st>>>(d_gsec, ns, b.V, VS, j, W, lz.ca, lz.cb, b.part);
       g_lz_beta_kernel<<<(unsigned)ngroups, RW * TT, 0, st>>>(d_gsec, ns, Mtot, b.part, j, K, b.hess, lz);

@@ -261,6 +261,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   cplx *d_V = nullptr, *d_W = nullptr, *d_W2 = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr;
   cplx *d_coef = nullptr, *d_staging = nullptr;
   LzState lz{};
+  cplx* d_apart = nullptr;  // the product's per-item alpha partials (Lanczos form)
   GSec* d_gsec = nullptr;
   JumpRec* d_jr = nullptr;
   int2 *d_pairs = nullptr, *d_moves = nullptr;
@@ -324,7 +325,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&lz.beta, sizeof(double) * (size_t)K * ntraj) ||
       !dalloc((void**)&lz.inv, sizeof(double) * (size_t)K * ntraj) ||
       !dalloc((void**)&lz.ip, sizeof(cplx) * (size_t)K * ntraj) || !dalloc((void**)&lz.ca, sizeof(cplx) * ntraj) ||
-      !dalloc((void**)&lz.cb, sizeof(cplx) * ntraj)) {
+      !dalloc((void**)&lz.cb, sizeof(cplx) * ntraj) ||
+      // product items over all sectors: sum of ceil(n / rows) ceil(m / 64) <= (nmax / 4 + 1) (ntraj / 64 + sectors)
+      (use_lz && !dalloc((void**)&d_apart, sizeof(cplx) * TT * (size_t)(nmax / SPMM_RB + 1) *
+                                               (size_t)((ntraj + TT - 1) / TT + NSMAX)))) {
     cleanup();
     return qd_set_error(ctx, QD_E_OOM, "trajectory workspace (%d trajectories, dim %lld, k %d)", ntraj,
                         (long long)nmax, k_rt);
@@ -414,8 +418,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       vo += S.n * S.m;
       co += (int64_t)g.nch * S.m;
       tiles += (int64_t)g.nch * g.groups;
-      blocks += (S.n + spmm_rows(spmm_var) - 1) / spmm_rows(spmm_var) *
-                ((S.m + spmm_width(spmm_var) - 1) / spmm_width(spmm_var));
+      blocks += (S.n + spmm_rows(spmm_var) - 1) / spmm_rows(spmm_var) * g.groups;
       ngroups += g.groups;
     }
     ns = (int)h_gsec.size();
@@ -448,9 +451,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     const int Mtot = ntraj;
     slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
                                                        d_scale);
-    const SpmmVarProduct prod{d_gsec, ns, blocks, spmm_var};
+    const SpmmVarProduct prod{d_gsec, ns, blocks, spmm_var, use_lz ? d_apart : nullptr};
     if (use_lz)
-      mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz, prod);
+      mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz, prod,
+                        d_apart, spmm_rows(spmm_var));
     else
       mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod);
     mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
@@ -512,7 +516,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     for (int var = 0; var < SPMM_VARIANTS; ++var) {
       spmm_var = var;
       build_groups();
-      const SpmmVarProduct prod{d_gsec, ns, blocks, var};
+      const SpmmVarProduct prod{d_gsec, ns, blocks, var, use_lz ? d_apart : nullptr};
       prod(st, 1, nullptr, d_scale, d_alpha, d_beta, d_V, d_W);
       hipEventRecord(ev0, st);
       for (int r = 0; r < 2; ++r) prod(st, 1, nullptr, d_scale, d_alpha, d_beta, d_V, d_W);
@@ -521,8 +525,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       float ms = 0.f;
       hipEventElapsedTime(&ms, ev0, ev1);
       if (tune_log)
-        std::fprintf(stderr, "[qd] trajectory product: kernel %d (%d rows x %d slots), %lld items over %d sectors, %.1f us\n",
-                     var, spmm_rows(var), spmm_width(var), (long long)blocks, ns, 1000.0 * ms / 2);
+        std::fprintf(stderr, "[qd] trajectory product: kernel %d (%d-row items), %lld items over %d sectors, %.1f us\n",
+                     var, spmm_rows(var), (long long)blocks, ns, 1000.0 * ms / 2);
       if (ms < best) {
         best = ms;
         best_var = var;
