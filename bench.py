@@ -124,7 +124,8 @@ def main():
                          "the union of the kernel intervals), print one JSON line and exit (no GPU)")
     ap.add_argument("--trace-window", default="",
                     help="with --trace-summary: only from the first to the last dispatch of a kernel whose name "
-                         "starts with this prefix, with the kernel time per kernel in that window")
+                         "starts with this prefix (FIRST..LAST: two prefixes), with the kernel time per kernel in "
+                         "that window")
     ap.add_argument("--cpu-full", action="store_true",
                     help="also time the CPU restatement on the full-size workloads: the XXZ N=24 Lanczos "
                          "ground state (k = 150) and one Hubbard 4x4 matrix-free product (minutes)")
@@ -361,7 +362,9 @@ def trace_summary(outdir, window=""):
     of the union of the kernel intervals, so kernels that overlap on several
     streams count once.  window: only the dispatches from the first to the
     last one whose kernel name starts with this prefix (one engine's run
-    inside a longer program), with the kernel time per kernel there."""
+    inside a longer program), with the kernel time per kernel there;
+    "FIRST..LAST": from the first dispatch whose name starts with FIRST to
+    the last one whose name starts with LAST."""
     import csv
     import gzip
     path = os.path.join(outdir, "run_kernel_trace.csv")
@@ -370,8 +373,11 @@ def trace_summary(outdir, window=""):
         rows = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kernel_short_name(r["Kernel_Name"]))
                       for r in csv.DictReader(fh))
     if window:
-        hit = [i for i, r in enumerate(rows) if r[2].startswith(window)]
-        rows = rows[hit[0]:hit[-1] + 1] if hit else []
+        first, _, last = window.partition("..")
+        last = last or first
+        i0 = next((i for i, r in enumerate(rows) if r[2].startswith(first)), None)
+        i1 = next((i for i in range(len(rows) - 1, -1, -1) if rows[i][2].startswith(last)), None)
+        rows = rows[i0:i1 + 1] if i0 is not None and i1 is not None and i1 >= i0 else []
     if not rows:
         return {"dispatches": 0, "span_s": 0.0, "busy_s": 0.0, "busy_frac": 0.0}
     busy, cs, ce = 0, rows[0][0], rows[0][1]

@@ -1243,9 +1243,15 @@ __global__ void __launch_bounds__(RW * TT)
   const int t = gr * TT + lane;
   const int64_t nrb = (S.n + rows - 1) / rows;
   const cplx* __restrict__ pb = apart + (S.blk0 + (int64_t)gr * nrb) * TT + lane;
+  // 8 loads in flight per wave, added in item order (rb, rb + RW, ...)
   cplx s = cmake(0.0, 0.0);
-#pragma unroll 4
-  for (int64_t rb = wv; rb < nrb; rb += RW) s = cadd(s, pb[rb * TT]);
+  for (int64_t rb = wv; rb < nrb; rb += 8 * RW) {
+    cplx v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = rb + RW * i < nrb ? pb[(rb + RW * i) * TT] : cmake(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s = cadd(s, v[i]);
+  }
   sh[wv][lane] = s;
   __syncthreads();
   if (wv != 0 || t >= S.m) return;

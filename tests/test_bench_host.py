@@ -86,3 +86,5 @@ def test_trace_summary_window_and_gzip(tmp_path):
     names = {k["kernel"]: (k["calls"], k["ms"]) for k in out["kernels"]}
     assert names == {"g_a<4, true>": (1, 0.1), "g_b": (1, 0.06), "other": (1, 0.02)}
     assert bench.trace_summary(str(tmp_path))["dispatches"] == 5
+    two = bench.trace_summary(str(tmp_path), "g_a..other")  # g_a at 100 us to the last `other` (400-450 us)
+    assert two["dispatches"] == 4 and abs(two["span_s"] - 350e-6) < 1e-12
