@@ -181,3 +181,24 @@ def test_traj_product_kernels_agree(ctx, var):
     assert got["losses"] > 50
     assert (got["log"] == ref["log"]).all()
     np.testing.assert_allclose(got["mean"], ref["mean"], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_traj_lanczos_form_smallest_krylov(ctx, k):
+    """kry_dim_rt = 2 and 3 (one and two Krylov vectors: no update pass, or
+    one): the Lanczos form matches the Arnoldi form and the oracle, over
+    sectors down to a single state (L = 4, losses)."""
+    L, nup, ndn = 4, 2, 2
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(36, 31)
+    ch = jump_channels(L, 0.5, 0.3)
+    ntraj, nsteps, dt = 64, 40, 0.05
+    lz = _with_krylov_env(None, lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 12,
+                                                 want_log=True))
+    ar = _with_krylov_env("arnoldi", lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k,
+                                                      12, want_log=True))
+    dsum, log = oracle.traj_run(ham, L, nup, ndn, psi0, ch, ntraj, nsteps, dt, k, 12)
+    assert lz["lanczos"] == 1 and lz["sectors"] > 3
+    assert (lz["log"] == ar["log"]).all() and (lz["log"] == log).all()
+    np.testing.assert_allclose(lz["mean"], ar["mean"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(lz["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
