@@ -1110,8 +1110,11 @@ inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t til
 // then reduces to the three-term recurrence: in exact arithmetic the
 // Hessenberg is tridiagonal, h_jj = alpha_j (Paige's form: v_j^H (H_eff v_j
 // - beta_j v_{j-1}), which includes c_t), h_{j+1,j} = h_{j,j+1} = beta_{j+1},
-// and the Krylov vectors are the same.  Per Krylov vector: the product, one
-// pass over (u_j, W) for alpha_j, one update pass; no pass over the basis.
+// and the Krylov vectors are the same.  It is the recurrence with which the
+// reference propagates Hermitian operators (Hermitian::LanczosSolver::evolve,
+// sparse_algorithm.hpp:261-285), here on H with c_t added to the diagonal.
+// Per Krylov vector: the product (which can also write the alpha sums), one
+// update pass; no pass over the basis.
 // The vectors stay unnormalised in the pool (u_0 = v_0, u_j = beta_j v_j),
 // so nothing is rewritten: the product takes 1/beta_j as its source factor,
 // the update and the final combination fold it into their coefficients.
