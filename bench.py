@@ -518,6 +518,7 @@ def trajectories(ctx, args):
            "run_s": round(r["run_s"], 3), "traj_steps_per_s": round(args.traj * nsteps / r["run_s"], 1),
            "jumps": r["jumps"], "losses": r["losses"], "sectors": r["sectors"],
            "krylov_form": "lanczos" if r["lanczos"] else "arnoldi",
+           "lanczos_fallback_steps": r["lanczos_fallback_steps"],
            "N_T": round(float(N.mean()), 5), "N_T_closed_form": round(expect, 5), "N_T_sigma": round(float(sigma), 5),
            "closed_form_z": round(abs(N.mean() - expect) / sigma, 3),
            "density_site0_up_T": round(float(r["mean"][-1][0]), 6),

@@ -422,6 +422,10 @@ typedef struct {
   int32_t lanczos; /* 1: the steps ran in the Lanczos form (every sector's
                     * sum_c L_c^+ L_c constant, H_eff = H + c_t there);
                     * 0: the Arnoldi form */
+  int32_t lanczos_fallback_steps; /* steps of the Lanczos form recomputed fully
+                                   * reorthogonalised because a trajectory's
+                                   * estimated loss of orthogonality (Simon's
+                                   * recurrence) passed the trigger */
 } qd_traj_stats;
 /* H: term program over 2*nsite bits conserving each spin's particle number.
  * d_psi0: start state in sector (nup0, ndn0) (device, ascending basis order).

@@ -44,7 +44,7 @@ class qd_jump(ctypes.Structure):
 
 class qd_traj_stats(ctypes.Structure):
     _fields_ = [("jumps", ctypes.c_int64), ("losses", ctypes.c_int64), ("sectors", ctypes.c_int32),
-                ("lanczos", ctypes.c_int32)]
+                ("lanczos", ctypes.c_int32), ("lanczos_fallback_steps", ctypes.c_int32)]
 
 
 QD_JUMP_DEPHASE = 0

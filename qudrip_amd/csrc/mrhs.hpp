@@ -1121,12 +1121,24 @@ inline void mrhs_arnoldi(hipStream_t st, const GSec* d_gsec, int ns, int64_t til
 // Finite-precision Lanczos vectors lose orthogonality as Ritz values
 // converge, but the projected exponential stays accurate (Druskin,
 // Greenbaum & Knizhnerman, SIAM J. Sci. Comput. 19 (1998) 38-54).
+// Semi-orthogonality is monitored per slot with Simon's estimate of the loss
+// of orthogonality (Simon, Math. Comp. 42 (1984) 115-142), the recurrence
+// the single-vector Lanczos with partial reorthogonalisation uses:
+//   beta_{j+1} w_{j+1,k} = beta_{k+1} w_{j,k+1} + (alpha_k - alpha_j) w_{j,k}
+//                          + beta_k w_{j,k-1} - beta_j w_{j-1,k} +- eps1 |T|,
+// w_{j+1,j} = eps1 (the recurrence's local orthogonality), w_{j+1,j+1} = 1,
+// eps1 = sqrt(n) eps / 2.  A slot whose max_k |w_{j+1,k}| passes `trigger`
+// sets *flag; the caller then recomputes the step fully reorthogonalised.
 struct LzState {
   double* beta;  // [j Mtot + slot] beta_j = |u_j|
   double* inv;   // 1 / beta_j (inv_0 = 1; 0 once the Krylov space has ended)
   cplx* ip;      // <v_j, v_{j-1}>
   cplx* ca;      // update coefficients of the current step: alpha_j / beta_j
   double* cb;    //                                          beta_j / beta_{j-1}
+  double* om;    // three rotating rows of the estimate: [(row (K + 1) + k) Mtot + slot]
+  double* tn;    // [slot] running estimate of |T|
+  int* flag;     // set when a slot's estimate passes trigger
+  double trigger;
 };
 
 // u_{j+1} = W - ca_t u_j - cb_t u_{j-1} (no u_{j-1} term for j = 0) into pool
@@ -1200,6 +1212,8 @@ __device__ inline void lz_alpha_finish(cplx s, int g, int Mtot, int j, int K, cp
   cplx al = s;
   if (j == 0) {
     lz.inv[g] = 1.0;
+    lz.om[g] = 1.0;  // w_{0,0} (row 0)
+    lz.tn[g] = 0.0;
   } else {
     iv = lz.inv[(int64_t)j * Mtot + g];
     const double bj = lz.beta[(int64_t)j * Mtot + g];
@@ -1287,6 +1301,31 @@ __global__ void __launch_bounds__(RW * TT)
   lz.ip[(int64_t)(j + 1) * Mtot + g] = cmake(c.x * iv * ivj, -c.y * iv * ivj);
   hess[((int64_t)(j * (K + 1) + j + 1)) * Mtot + g] = cmake(b, 0.0);
   hess[((int64_t)((j + 1) * (K + 1) + j)) * Mtot + g] = cmake(iv > 0.0 ? b : 0.0, 0.0);
+  if (iv == 0.0) return;  // the space has ended: no further vector to estimate
+  // Simon's estimate of row j + 1 from rows j and j - 1 (alpha_k - alpha_j:
+  // the real parts; the slot's shift c_t is imaginary and cancels)
+  const int64_t ld = (int64_t)(K + 1) * Mtot;
+  double* rn = lz.om + (int64_t)((j + 1) % 3) * ld + g;  // row j + 1
+  const double* r1 = lz.om + (int64_t)(j % 3) * ld + g;  // row j
+  const double* r2 = lz.om + (int64_t)((j + 2) % 3) * ld + g;  // row j - 1
+  const double aj = hess[((int64_t)(j * (K + 1) + j)) * Mtot + g].x;
+  const double bj = j > 0 ? lz.beta[(int64_t)j * Mtot + g] : 0.0;
+  const double T = fmax(lz.tn[g], fabs(aj) + bj + b);
+  lz.tn[g] = T;
+  const double eps1 = 0.5 * sqrt((double)S.n) * 2.220446049250313e-16;
+  double mx = 0.0;
+  for (int k = 0; k < j; ++k) {
+    const double ak = hess[((int64_t)(k * (K + 1) + k)) * Mtot + g].x;
+    const double bk1 = lz.beta[(int64_t)(k + 1) * Mtot + g];
+    double tt = bk1 * r1[(int64_t)(k + 1) * Mtot] + (ak - aj) * r1[(int64_t)k * Mtot] - bj * r2[(int64_t)k * Mtot];
+    if (k > 0) tt += lz.beta[(int64_t)k * Mtot + g] * r1[(int64_t)(k - 1) * Mtot];
+    const double v = (tt + copysign(eps1 * T, tt)) / b;
+    rn[(int64_t)k * Mtot] = v;
+    mx = fmax(mx, fabs(v));
+  }
+  rn[(int64_t)j * Mtot] = eps1;
+  rn[(int64_t)(j + 1) * Mtot] = 1.0;
+  if (mx > lz.trigger) atomicOr(lz.flag, 1);
 }
 
 // coef[j Mtot + slot] *= inv_j: coefficients of the unnormalised pool vectors

@@ -326,6 +326,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&lz.inv, sizeof(double) * (size_t)K * ntraj) ||
       !dalloc((void**)&lz.ip, sizeof(cplx) * (size_t)K * ntraj) || !dalloc((void**)&lz.ca, sizeof(cplx) * ntraj) ||
       !dalloc((void**)&lz.cb, sizeof(cplx) * ntraj) ||
+      !dalloc((void**)&lz.om, sizeof(double) * 3 * (size_t)(K + 1) * ntraj) ||
+      !dalloc((void**)&lz.tn, sizeof(double) * ntraj) || !dalloc((void**)&lz.flag, sizeof(cplx)) ||
       // product items over all sectors: sum of ceil(n / rows) ceil(m / 64) <= (nmax / 4 + 1) (ntraj / 64 + sectors)
       (use_lz && !dalloc((void**)&d_apart, sizeof(cplx) * TT * (size_t)(nmax / SPMM_RB + 1) *
                                                (size_t)((ntraj + TT - 1) / TT + NSMAX)))) {
@@ -337,6 +339,12 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   zero_kernel<<<g1((ntraj + 1) / 2, 256), 256, 0, st>>>((ntraj + 1) / 2, (cplx*)d_scale);  // ntraj + 1 doubles
   zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, d_alpha);
   zero_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, d_beta);
+  // the Lanczos form's trigger for recomputing a step fully reorthogonalised:
+  // a quarter of sqrt(eps / K), as the single-vector partial scheme
+  // (QD_TRAJ_OMEGA overrides it)
+  lz.trigger = 0.25 * std::sqrt(2.220446049250313e-16 / K);
+  if (const char* e = std::getenv("QD_TRAJ_OMEGA")) lz.trigger = std::atof(e);
+  int64_t lz_fallback = 0;
   if (use_lz) {
     // the Lanczos form writes only the three diagonals of each Hessenberg
     zero_kernel<<<g1((int64_t)K * (K + 1) * ntraj, 256), 256, 0, st>>>((int64_t)K * (K + 1) * ntraj, d_hess);
@@ -452,12 +460,28 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
                                                        d_scale);
     const SpmmVarProduct prod{d_gsec, ns, blocks, spmm_var, use_lz ? d_apart : nullptr};
-    if (use_lz)
+    const SpmmVarProduct prod_plain{d_gsec, ns, blocks, spmm_var, nullptr};
+    bool rezero = false;
+    if (use_lz) {
+      zero_kernel<<<1, 1, 0, st>>>(1, reinterpret_cast<cplx*>(lz.flag));
       mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz, prod,
                         d_apart, spmm_rows(spmm_var));
-    else
+      int hflag = 0;
+      QD_HIP(ctx, hipMemcpyAsync(&hflag, lz.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+      QD_HIP(ctx, hipStreamSynchronize(st));
+      if (hflag) {
+        // a slot's basis is no longer semi-orthogonal: this step again,
+        // fully reorthogonalised, from the same X (not yet overwritten)
+        ++lz_fallback;
+        mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod_plain);
+        rezero = true;
+      }
+    } else {
       mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod);
+    }
     mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
+    if (rezero)  // the Arnoldi step filled the Hessenbergs above the diagonals
+      zero_kernel<<<g1((int64_t)K * (K + 1) * ntraj, 256), 256, 0, st>>>((int64_t)K * (K + 1) * ntraj, d_hess);
     QD_CHECK_LAUNCH(ctx);
     return QD_OK;
   };
@@ -698,6 +722,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       for (auto& S : sec) used += S.basis ? 1 : 0;
       stats->sectors = used;
       stats->lanczos = use_lz ? 1 : 0;
+      stats->lanczos_fallback_steps = (int32_t)lz_fallback;
     }
   }
   hipEventDestroy(ev0);

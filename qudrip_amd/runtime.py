@@ -378,7 +378,8 @@ def traj_run(ctx, opsum, nsite, nup0, ndn0, psi0, channels, ntraj, nsteps, dt, k
                                   var.ctypes.data, log.ctypes.data if log is not None else None,
                                   ctypes.byref(st)), "traj_run")
     out = {"mean": mean.reshape(nsteps + 1, M), "var": var.reshape(nsteps + 1, M),
-           "jumps": st.jumps, "losses": st.losses, "sectors": st.sectors, "lanczos": st.lanczos}
+           "jumps": st.jumps, "losses": st.losses, "sectors": st.sectors, "lanczos": st.lanczos,
+           "lanczos_fallback_steps": st.lanczos_fallback_steps}
     if log is not None:
         out["log"] = log.reshape(ntraj, max(nsteps, 1))
     return out

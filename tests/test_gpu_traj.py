@@ -61,6 +61,7 @@ def test_traj_full_size_closed_form_and_oracle_prefix(ctx):
     ntraj, nsteps, dt, k = 512, 200, 0.05, 13
     print(f"traj L=8 ntraj={ntraj} nsteps={nsteps}", flush=True)
     g = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, ntraj, nsteps, dt, k, 5, want_log=True)
+    assert g["lanczos"] == 1 and g["lanczos_fallback_steps"] == 0
     is_loss = np.array([c[0] == 1 for c in ch] + [False])
     N = nup + ndn - is_loss[g["log"]].sum(axis=1)
     assert g["losses"] == (nup + ndn) * ntraj - N.sum()
@@ -115,6 +116,7 @@ def test_traj_lanczos_form_matches_arnoldi_form(ctx):
     lz = _with_krylov_env(None, lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True))
     ar = _with_krylov_env("arnoldi", lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True))
     assert lz["lanczos"] == 1 and ar["lanczos"] == 0
+    assert lz["lanczos_fallback_steps"] == 0  # every step's basis stayed semi-orthogonal
     assert lz["losses"] > 50 and lz["sectors"] > 6
     assert (lz["log"] == ar["log"]).all()
     np.testing.assert_allclose(lz["mean"], ar["mean"], rtol=0, atol=1e-12)
@@ -202,3 +204,29 @@ def test_traj_lanczos_form_smallest_krylov(ctx, k):
     assert (lz["log"] == ar["log"]).all() and (lz["log"] == log).all()
     np.testing.assert_allclose(lz["mean"], ar["mean"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(lz["mean"] * ntraj, dsum, rtol=0, atol=1e-9)
+
+
+def test_traj_lanczos_fallback_recomputes_fully_reorthogonalised(ctx):
+    """With the orthogonality trigger at 0 (QD_TRAJ_OMEGA) every step of the
+    Lanczos form is flagged and recomputed fully reorthogonalised from the
+    same state: the run then agrees with the Arnoldi form (same jumps,
+    densities to rounding) and reports every step as recomputed."""
+    import os
+    L, nup, ndn = 6, 3, 3
+    ham = _setup(L)
+    psi0 = oracle.seeded_state(400, 33)
+    ch = jump_channels(L, 0.3, 0.2)
+    args = (80, 40, 0.05, 13, 14)
+    old = os.environ.get("QD_TRAJ_OMEGA")
+    os.environ["QD_TRAJ_OMEGA"] = "0"
+    try:
+        fb = traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True)
+    finally:
+        if old is None:
+            os.environ.pop("QD_TRAJ_OMEGA", None)
+        else:
+            os.environ["QD_TRAJ_OMEGA"] = old
+    ar = _with_krylov_env("arnoldi", lambda: traj_run(ctx, ham, L, nup, ndn, _dev(psi0), ch, *args, want_log=True))
+    assert fb["lanczos"] == 1 and fb["lanczos_fallback_steps"] == 40
+    assert (fb["log"] == ar["log"]).all()
+    np.testing.assert_allclose(fb["mean"], ar["mean"], rtol=0, atol=1e-12)
