@@ -847,11 +847,14 @@ __global__ void __launch_bounds__(64)
   if (i < k) coef[i * m + t] = y;
 }
 
-// X[s,t] = sum_j V_j[s,t] coef[j Mtot + slot], all sectors (one thread per element of the pools)
+// X[s,t] = sum_j V_j[s,t] coef[j Mtot + slot], all sectors (one thread per
+// element of the pools); nothing is written when skip is given and nonzero
+// (a combination enqueued before the host has read a flag that voids it)
 __global__ void g_combine_kernel(const GSec* __restrict__ gs, int ns, int64_t total, const cplx* __restrict__ V,
-                                 int64_t VS, int k, const cplx* __restrict__ coef, int Mtot) {
+                                 int64_t VS, int k, const cplx* __restrict__ coef, int Mtot,
+                                 const int* __restrict__ skip) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
+  if (e >= total || (skip && *skip)) return;
   const int si = find_sec(ns, e, [&](int i) { return gs[i].vo; });
   const GSec& S = gs[si];
   const int64_t local = e - S.vo;
@@ -1370,8 +1373,9 @@ inline void mrhs_lanczos_with(hipStream_t st, const GSec* d_gsec, int ns, int64_
   g_lz_scale_kernel<<<g1((int64_t)K * Mtot, 256), 256, 0, st>>>(Mtot, K, lz.inv, b.coef);
 }
 
-inline void mrhs_combine(hipStream_t st, const GSec* d_gsec, int ns, int64_t VS, int Mtot, int K, const MrhsBufs& b) {
-  g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, b.V, VS, K, b.coef, Mtot);
+inline void mrhs_combine(hipStream_t st, const GSec* d_gsec, int ns, int64_t VS, int Mtot, int K, const MrhsBufs& b,
+                         const int* skip = nullptr) {
+  g_combine_kernel<<<g1(VS, 256), 256, 0, st>>>(d_gsec, ns, VS, b.V, VS, K, b.coef, Mtot, skip);
 }
 
 }  // namespace

@@ -105,7 +105,8 @@ __global__ void dens_accumulate_kernel(int ntraj, int M, const double* __restric
 // jump test per trajectory (hubbard_diss.cpp:450-463 generalised)
 __global__ void decide_kernel(int ntraj, int M, const double* __restrict__ dens_all, const int* __restrict__ jmode,
                               const double* __restrict__ jrate, int nj, uint64_t seed, int64_t step, double dt,
-                              int* __restrict__ action, double* __restrict__ gamma_traj) {
+                              int* __restrict__ action, double* __restrict__ gamma_traj,
+                              double* __restrict__ jscale) {
   const int tr = blockIdx.x * blockDim.x + threadIdx.x;
   if (tr >= ntraj) return;
   const double* d = dens_all + (int64_t)tr * (M + 1);
@@ -126,7 +127,20 @@ __global__ void decide_kernel(int ntraj, int M, const double* __restrict__ dens_
     }
   }
   action[tr] = ch;
+  // 1 / |L psi| of a jumper: |L psi|^2 = <n_m> |psi|^2 from the recorded densities
+  if (ch >= 0) jscale[tr] = 1.0 / sqrt(d[jmode[ch]] * d[M]);
 }
+
+// every sector's block and basis, indexed by sector id (the per-step jump
+// tables name sectors by this index, so each jump phase is one launch over
+// all sectors)
+struct SecTab {
+  int64_t n;
+  int ld;
+  cplx* X;
+  const uint64_t* states;
+  SectorDesc sec;
+};
 
 struct JumpRec {
   int slot;      // source slot
@@ -134,15 +148,21 @@ struct JumpRec {
   int loss;      // 0 dephasing, 1 loss
   int staging;   // staging vector index
   double scale;  // 1 / |L psi| (from the recorded densities and norm)
+  int src;       // source sector
+  int tgt;       // target sector of a loss (the source for dephasing)
 };
 
-// staged[j][target] = L psi_slot (normalised); one block per jump
-__global__ void jump_apply_kernel(int64_t n, int ld, const cplx* __restrict__ X, const uint64_t* __restrict__ states,
-                                  const JumpRec* __restrict__ jr, SectorDesc tup, SectorDesc tdn, int nsite,
-                                  int64_t stg_ld, cplx* __restrict__ staging) {
+// staged[j][target] = L psi_slot (normalised); grid (jumps of every sector, row chunks)
+__global__ void jump_apply_kernel(const SecTab* __restrict__ tab, const JumpRec* __restrict__ jr, int64_t stg_ld,
+                                  cplx* __restrict__ staging) {
   const JumpRec J = jr[blockIdx.x];
+  const SecTab& S = tab[J.src];
+  const int64_t n = S.n;
+  const int ld = S.ld;
+  const cplx* __restrict__ X = S.X;
+  const uint64_t* __restrict__ states = S.states;
   cplx* out = staging + (int64_t)J.staging * stg_ld;
-  for (int64_t s = threadIdx.x; s < n; s += blockDim.x) {
+  for (int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.y * blockDim.x) {
     const uint64_t x = states[s];
     if (!((x >> J.mode) & 1ull)) continue;
     cplx v = X[s * ld + J.slot];
@@ -152,25 +172,37 @@ __global__ void jump_apply_kernel(int64_t n, int ld, const cplx* __restrict__ X,
     } else {
       const uint64_t y = x & ~(1ull << J.mode);
       const double sg = (__popcll(x >> (J.mode + 1)) & 1) ? -1.0 : 1.0;
-      const uint64_t r = qd_rank_cns(J.mode < nsite ? tup : tdn, y);
+      const uint64_t r = qd_rank_cns(tab[J.tgt].sec, y);
       out[r] = cmake(sg * v.x, sg * v.y);
     }
   }
 }
 
-// X[s, dst_slot] = staging[j][s] for each (staging index, slot) pair
-__global__ void commit_kernel(int64_t n, int ld, cplx* __restrict__ X, const int2* __restrict__ pairs,
-                              int64_t stg_ld, const cplx* __restrict__ staging) {
-  const int2 p = pairs[blockIdx.x];
-  for (int64_t s = threadIdx.x; s < n; s += blockDim.x) X[s * ld + p.y] = staging[(int64_t)p.x * stg_ld + s];
+// X[s, slot] = staging[j][s] for each (staging index j, slot, sector) of the
+// step; grid (commits of every sector, row chunks)
+__global__ void commit_kernel(const SecTab* __restrict__ tab, const int4* __restrict__ pairs, int64_t stg_ld,
+                              const cplx* __restrict__ staging) {
+  const int4 p = pairs[blockIdx.x];
+  const SecTab& S = tab[p.z];
+  for (int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; s < S.n; s += (int64_t)gridDim.y * blockDim.x)
+    S.X[s * S.ld + p.y] = staging[(int64_t)p.x * stg_ld + s];
 }
 
-// column moves inside a block (swap-remove compaction): X[s, dst] = X[s, src]
-__global__ void move_cols_kernel(int64_t n, int ld, cplx* __restrict__ X, const int2* __restrict__ moves, int nmoves) {
+// column moves of the swap-remove compaction, in list order, X[s, dst] =
+// X[s, src]: list j = (sector, first move, move count); grid (row blocks, lists)
+__global__ void move_cols_kernel(const SecTab* __restrict__ tab, const int4* __restrict__ lists,
+                                 const int2* __restrict__ moves) {
+  const int4 L = lists[blockIdx.y];
+  const SecTab& S = tab[L.x];
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  for (int k = 0; k < nmoves; ++k) X[s * ld + moves[k].y] = X[s * ld + moves[k].x];
+  if (s >= S.n) return;
+  cplx* row = S.X + s * S.ld;
+  for (int k = 0; k < L.z; ++k) row[moves[L.y + k].y] = row[moves[L.y + k].x];
 }
+
+// row chunks of 256 for the per-jump kernels: a jump's column spread over up
+// to 64 workgroups instead of one walking all rows
+inline unsigned col_chunks(int64_t n) { return (unsigned)std::min<int64_t>(64, std::max<int64_t>(1, (n + 255) / 256)); }
 
 __global__ void zero_kernel(int64_t count, cplx* __restrict__ p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,11 +284,14 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   // chunk counts are at most min(QD_TRAJ_MAX_CHUNKS, ceil(n / 16)) per sector
   const int max_nch = (int)std::min<int64_t>(QD_TRAJ_MAX_CHUNKS, (nmax + 15) / 16);
   const int PMAX = std::max(2 * K + 2, M + 1);
+  // a step's commit table: at most one commit per trajectory, one move list
+  // per sector, one move per trajectory
+  const size_t pm_bytes = sizeof(int4) * (size_t)ntraj + sizeof(int4) * NSMAX + sizeof(int2) * (size_t)ntraj;
 
   // device buffers
   int *d_jmode = nullptr, *d_action = nullptr, *d_slot2traj = nullptr;
   double *d_jrate = nullptr, *d_dens_all = nullptr, *d_gtraj = nullptr, *d_dsum = nullptr, *d_dsq = nullptr;
-  double *d_scale = nullptr;
+  double *d_scale = nullptr, *d_jscale = nullptr;
   cplx *d_alpha = nullptr, *d_beta = nullptr, *d_red = nullptr, *d_coef2 = nullptr, *d_sc = nullptr;
   cplx *d_V = nullptr, *d_W = nullptr, *d_W2 = nullptr, *d_part = nullptr, *d_h = nullptr, *d_hess = nullptr;
   cplx *d_coef = nullptr, *d_staging = nullptr;
@@ -264,17 +299,48 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
   cplx* d_apart = nullptr;  // the product's per-item alpha partials (Lanczos form)
   GSec* d_gsec = nullptr;
   JumpRec* d_jr = nullptr;
-  int2 *d_pairs = nullptr, *d_moves = nullptr;
+  char* d_pm = nullptr;  // a step's commits, move lists and column moves
+  SecTab* d_sectab = nullptr;
   std::vector<double> dsum((size_t)(nsteps + 1) * M, 0.0), dsq((size_t)(nsteps + 1) * M, 0.0);
   int64_t n_jumps = 0, n_loss = 0;
   std::vector<void*> allocs;
+  std::vector<SecTab> h_sectab(NSMAX);  // uploaded before a jump phase after sectors were added
+  bool tab_dirty = false;
   auto dalloc = [&](void** p, size_t bytes) -> bool {
     if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) return false;
     allocs.push_back(*p);
     return true;
   };
+  // Pinned host regions for the per-step tables and readbacks.  Copies from
+  // pageable memory are staged by the runtime and hold the host until the
+  // stream reaches them, which left the GPU idle around every small launch of
+  // the jump phase.  An upload region is rewritten only after its previous
+  // upload has completed (its event).
+  struct PinRegion {
+    char* p = nullptr;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  char* pin_base = nullptr;
+  hipEvent_t ev_rb = nullptr, ev_flag = nullptr;  // a step's readback of the jump decisions; of the flag
+  PinRegion pin_gsec, pin_s2t, pin_jr, pin_pm, pin_tab;
+  int* h_action = nullptr;
+  double* h_jscale = nullptr;
+  int* h_flag = nullptr;
+  auto upload = [&](PinRegion& r, void* dst, const void* src, size_t bytes) {
+    if (r.pending) hipEventSynchronize(r.ev);
+    if (bytes) std::memcpy(r.p, src, bytes);
+    hipMemcpyAsync(dst, r.p, bytes, hipMemcpyHostToDevice, st);
+    hipEventRecord(r.ev, st);
+    r.pending = true;
+  };
   auto cleanup = [&]() {
     hipStreamSynchronize(st);
+    for (PinRegion* r : {&pin_gsec, &pin_s2t, &pin_jr, &pin_pm, &pin_tab})
+      if (r->ev) hipEventDestroy(r->ev);
+    if (ev_rb) hipEventDestroy(ev_rb);
+    if (ev_flag) hipEventDestroy(ev_flag);
+    if (pin_base) hipHostFree(pin_base);
     for (auto& S : sec) {
       if (S.csr) qd_csr_destroy(S.csr);
       if (S.basis) qd_basis_destroy(S.basis);
@@ -300,6 +366,8 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     if (!dalloc((void**)&S.D, sizeof(double) * S.n) || !dalloc((void**)&S.X, sizeof(cplx) * S.n * S.ld))
       return qd_set_error(ctx, QD_E_OOM, "sector (%d,%d) buffers", a, b);
     diag_rates_kernel<<<g1(S.n, 256), 256, 0, st>>>(S.n, S.basis->d_states, d_jmode, d_jrate, njumps, S.D);
+    h_sectab[sid(a, b)] = SecTab{S.n, S.ld, S.X, S.basis->d_states, S.basis->sec};
+    tab_dirty = true;
     return QD_OK;
   };
 
@@ -307,8 +375,10 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&d_jrate, sizeof(double) * std::max(1, njumps)) ||
       !dalloc((void**)&d_action, sizeof(int) * ntraj) || !dalloc((void**)&d_slot2traj, sizeof(int) * ntraj) ||
       !dalloc((void**)&d_dens_all, sizeof(double) * ntraj * (M + 1)) ||
-      !dalloc((void**)&d_gtraj, sizeof(double) * ntraj) || !dalloc((void**)&d_dsum, sizeof(double) * M) ||
-      !dalloc((void**)&d_dsq, sizeof(double) * M) || !dalloc((void**)&d_scale, sizeof(double) * (ntraj + 1)) ||
+      !dalloc((void**)&d_gtraj, sizeof(double) * ntraj) || !dalloc((void**)&d_jscale, sizeof(double) * ntraj) ||
+      !dalloc((void**)&d_dsum, sizeof(double) * (size_t)(nsteps + 1) * M) ||
+      !dalloc((void**)&d_dsq, sizeof(double) * (size_t)(nsteps + 1) * M) ||
+      !dalloc((void**)&d_scale, sizeof(double) * (ntraj + 1)) ||
       !dalloc((void**)&d_alpha, sizeof(cplx) * ntraj) || !dalloc((void**)&d_beta, sizeof(cplx) * ntraj) ||
       !dalloc((void**)&d_red, sizeof(cplx) * (size_t)(2 * K + 2) * ntraj) ||
       !dalloc((void**)&d_coef2, sizeof(cplx) * (size_t)(2 * K) * ntraj) ||
@@ -321,7 +391,7 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
       !dalloc((void**)&d_hess, sizeof(cplx) * (size_t)K * (K + 1) * ntraj) ||
       !dalloc((void**)&d_coef, sizeof(cplx) * (size_t)K * ntraj) ||
       !dalloc((void**)&d_gsec, sizeof(GSec) * NSMAX) || !dalloc((void**)&d_jr, sizeof(JumpRec) * ntraj) ||
-      !dalloc((void**)&d_pairs, sizeof(int2) * ntraj) || !dalloc((void**)&d_moves, sizeof(int2) * ntraj) ||
+      !dalloc((void**)&d_pm, pm_bytes) || !dalloc((void**)&d_sectab, sizeof(SecTab) * NSMAX) ||
       !dalloc((void**)&lz.beta, sizeof(double) * (size_t)K * ntraj) ||
       !dalloc((void**)&lz.inv, sizeof(double) * (size_t)K * ntraj) ||
       !dalloc((void**)&lz.ip, sizeof(cplx) * (size_t)K * ntraj) || !dalloc((void**)&lz.ca, sizeof(cplx) * ntraj) ||
@@ -334,6 +404,39 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     cleanup();
     return qd_set_error(ctx, QD_E_OOM, "trajectory workspace (%d trajectories, dim %lld, k %d)", ntraj,
                         (long long)nmax, k_rt);
+  }
+  {
+    auto up256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t b_gsec = up256(sizeof(GSec) * NSMAX), b_s2t = up256(sizeof(int) * ntraj),
+                 b_jr = up256(sizeof(JumpRec) * ntraj), b_pm = up256(pm_bytes), b_tab = up256(sizeof(SecTab) * NSMAX),
+                 b_act = up256(sizeof(int) * ntraj), b_js = up256(sizeof(double) * ntraj), b_flag = 256;
+    if (hipHostMalloc((void**)&pin_base, b_gsec + b_s2t + b_jr + b_pm + b_tab + b_act + b_js + b_flag) != hipSuccess) {
+      pin_base = nullptr;
+      cleanup();
+      return qd_set_error(ctx, QD_E_OOM, "trajectory host tables (%d trajectories)", ntraj);
+    }
+    char* q = pin_base;
+    pin_gsec.p = q;
+    q += b_gsec;
+    pin_s2t.p = q;
+    q += b_s2t;
+    pin_jr.p = q;
+    q += b_jr;
+    pin_pm.p = q;
+    q += b_pm;
+    pin_tab.p = q;
+    q += b_tab;
+    h_action = reinterpret_cast<int*>(q);
+    q += b_act;
+    h_jscale = reinterpret_cast<double*>(q);
+    q += b_js;
+    h_flag = reinterpret_cast<int*>(q);
+    for (hipEvent_t* e : {&pin_gsec.ev, &pin_s2t.ev, &pin_jr.ev, &pin_pm.ev, &pin_tab.ev, &ev_rb, &ev_flag})
+      if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+        *e = nullptr;
+        cleanup();
+        return qd_set_error(ctx, QD_E_HIP, "trajectory events");
+      }
   }
   // per-slot operands of the product, defined before the first tuning run reads them
   zero_kernel<<<g1((ntraj + 1) / 2, 256), 256, 0, st>>>((ntraj + 1) / 2, (cplx*)d_scale);  // ntraj + 1 doubles
@@ -367,17 +470,14 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     for (int t = 0; t < ntraj; ++t) S.slot2traj[t] = t;
     broadcast_kernel<<<g1(S.n * ntraj, 256), 256, 0, st>>>(S.n, ntraj, S.ld, (const cplx*)d_psi0, S.X);
   }
-  std::vector<int> action(ntraj);
-  std::vector<double> dens_host((size_t)ntraj * (M + 1));
-  // per-step host tables, reserved so that the async copies from them stay
-  // valid until the next synchronisation
+  // per-step host tables (uploaded through the pinned regions)
   std::vector<JumpRec> h_jr;
-  std::vector<int2> h_pairs, h_moves;
+  std::vector<int> pend_of(ntraj, -1);  // a jumping trajectory's index in the step's pending list
+  std::vector<char> h_pm;
   std::vector<GSec> h_gsec;
   std::vector<int> h_slot2traj;
   h_jr.reserve(ntraj);
-  h_pairs.reserve(ntraj);
-  h_moves.reserve(ntraj);
+  h_pm.reserve(pm_bytes);
   h_gsec.reserve(NSMAX);
   h_slot2traj.reserve(ntraj);
   int stg_cap = 0;
@@ -431,57 +531,68 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     }
     ns = (int)h_gsec.size();
     VS = vo;
-    hipMemcpyAsync(d_gsec, h_gsec.data(), sizeof(GSec) * ns, hipMemcpyHostToDevice, st);
-    hipMemcpyAsync(d_slot2traj, h_slot2traj.data(), sizeof(int) * ntraj, hipMemcpyHostToDevice, st);
+    upload(pin_gsec, d_gsec, h_gsec.data(), sizeof(GSec) * ns);
+    upload(pin_s2t, d_slot2traj, h_slot2traj.data(), sizeof(int) * h_slot2traj.size());
   };
 
-  // densities of every trajectory -> d_dens_all; sums over trajectories -> host (step >= 0)
+  // densities of every trajectory -> d_dens_all; sums over trajectories ->
+  // row `step` of d_dsum, d_dsq (read back once, after the last step)
   auto densities = [&](int step) -> int {
     g_dens_partial_kernel<<<(unsigned)tiles, TT, 0, st>>>(d_gsec, ns, M, (double*)d_part);
     g_dens_reduce_kernel<<<(unsigned)(ngroups * (M + 1)), RW * TT, 0, st>>>(d_gsec, ns, ngroups, M, (double*)d_part,
                                                                             d_slot2traj, d_dens_all);
-    dens_accumulate_kernel<<<M, 256, 0, st>>>(ntraj, M, d_dens_all, d_dsum, d_dsq);
-    double hs[2 * MAXM];
-    QD_HIP(ctx, hipMemcpyAsync(hs, d_dsum, sizeof(double) * M, hipMemcpyDeviceToHost, st));
-    QD_HIP(ctx, hipMemcpyAsync(hs + MAXM, d_dsq, sizeof(double) * M, hipMemcpyDeviceToHost, st));
-    QD_HIP(ctx, hipStreamSynchronize(st));
-    if (step >= 0)
-      for (int q = 0; q < M; ++q) {
-        dsum[(size_t)step * M + q] = hs[q];
-        dsq[(size_t)step * M + q] = hs[MAXM + q];
-      }
+    dens_accumulate_kernel<<<M, 256, 0, st>>>(ntraj, M, d_dens_all, d_dsum + (size_t)step * M,
+                                              d_dsq + (size_t)step * M);
+    QD_CHECK_LAUNCH(ctx);
     return QD_OK;
   };
 
-  // one Arnoldi step (delayed CGS2, K vectors) for every slot of every occupied sector, X updated in place
+  // one Arnoldi step (delayed CGS2, K vectors, or the Lanczos form) for every
+  // slot of every occupied sector, X updated in place.  evolve_begin enqueues
+  // the expansion, which only reads X; evolve_end enqueues the combination
+  // that overwrites X.  The jump phase (which reads the pre-step X) is
+  // enqueued in between, so the host's jump bookkeeping overlaps the
+  // expansion on the GPU.  In the Lanczos form the combination is enqueued
+  // before the host reads the orthogonality flag and writes nothing when the
+  // flag is set; the host then enqueues the fully reorthogonalised step and
+  // its combination.
   const MrhsBufs mb{d_V, d_W2, d_W, d_part, d_h, d_hess, d_red, d_coef2, d_sc, d_coef};
-  auto evolve_all = [&]() -> int {
+  auto evolve_begin = [&]() -> int {
     const int Mtot = ntraj;
     slot_params_kernel<<<g1(Mtot, 256), 256, 0, st>>>(Mtot, M, d_slot2traj, d_gtraj, d_dens_all, d_alpha, d_beta,
                                                        d_scale);
     const SpmmVarProduct prod{d_gsec, ns, blocks, spmm_var, use_lz ? d_apart : nullptr};
-    const SpmmVarProduct prod_plain{d_gsec, ns, blocks, spmm_var, nullptr};
-    bool rezero = false;
     if (use_lz) {
       zero_kernel<<<1, 1, 0, st>>>(1, reinterpret_cast<cplx*>(lz.flag));
       mrhs_lanczos_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, lz, prod,
                         d_apart, spmm_rows(spmm_var));
-      int hflag = 0;
-      QD_HIP(ctx, hipMemcpyAsync(&hflag, lz.flag, sizeof(int), hipMemcpyDeviceToHost, st));
-      QD_HIP(ctx, hipStreamSynchronize(st));
-      if (hflag) {
-        // a slot's basis is no longer semi-orthogonal: this step again,
-        // fully reorthogonalised, from the same X (not yet overwritten)
-        ++lz_fallback;
-        mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod_plain);
-        rezero = true;
-      }
+      QD_HIP(ctx, hipMemcpyAsync(h_flag, lz.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+      QD_HIP(ctx, hipEventRecord(ev_flag, st));
     } else {
       mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod);
     }
-    mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
-    if (rezero)  // the Arnoldi step filled the Hessenbergs above the diagonals
+    QD_CHECK_LAUNCH(ctx);
+    return QD_OK;
+  };
+  auto evolve_end = [&]() -> int {
+    const int Mtot = ntraj;
+    if (!use_lz) {
+      mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
+      QD_CHECK_LAUNCH(ctx);
+      return QD_OK;
+    }
+    mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb, lz.flag);
+    QD_HIP(ctx, hipEventSynchronize(ev_flag));
+    if (*h_flag) {
+      // a slot's basis is no longer semi-orthogonal: the combination above
+      // wrote nothing; this step again, fully reorthogonalised, from the same X
+      ++lz_fallback;
+      const SpmmVarProduct prod_plain{d_gsec, ns, blocks, spmm_var, nullptr};
+      mrhs_arnoldi_with(st, d_gsec, ns, tiles, ngroups, VS, Mtot, K, dt, d_scale, d_alpha, d_beta, mb, prod_plain);
+      mrhs_combine(st, d_gsec, ns, VS, Mtot, K, mb);
+      // the Arnoldi step filled the Hessenbergs above the diagonals
       zero_kernel<<<g1((int64_t)K * (K + 1) * ntraj, 256), 256, 0, st>>>((int64_t)K * (K + 1) * ntraj, d_hess);
+    }
     QD_CHECK_LAUNCH(ctx);
     return QD_OK;
   };
@@ -571,26 +682,33 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
     if (rc) break;
     if (step == nsteps) break;
     decide_kernel<<<g1(ntraj, 256), 256, 0, st>>>(ntraj, M, d_dens_all, d_jmode, d_jrate, njumps, seed, step, dt,
-                                                  d_action, d_gtraj);
-    hipMemcpyAsync(action.data(), d_action, sizeof(int) * ntraj, hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(dens_host.data(), d_dens_all, sizeof(double) * dens_host.size(), hipMemcpyDeviceToHost, st);
-    if (hipStreamSynchronize(st) != hipSuccess) {
+                                                  d_action, d_gtraj, d_jscale);
+    hipMemcpyAsync(h_action, d_action, sizeof(int) * ntraj, hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(h_jscale, d_jscale, sizeof(double) * ntraj, hipMemcpyDeviceToHost, st);
+    hipEventRecord(ev_rb, st);
+    // the expansion of every occupied sector (jumpers too; their columns are
+    // replaced below) goes to the GPU before the host waits for the jump
+    // decisions
+    if ((rc = evolve_begin())) break;
+    if (hipEventSynchronize(ev_rb) != hipSuccess) {
       rc = qd_set_error(ctx, QD_E_HIP, "trajectory step %d", step);
       break;
     }
+    const int* action = h_action;
     if (h_jump_log)
       for (int t = 0; t < ntraj; ++t) h_jump_log[(size_t)t * nsteps + step] = action[t];
-    // jumps from the pre-step states into staging
+    // jumps from the pre-step states into staging: the records of every
+    // sector (target sectors built on first use), then one upload, one
+    // clearing of the staging vectors and one launch
     struct Pending {
       int traj, a, b, sidx, src_slot;
       int loss;
     };
     std::vector<Pending> pend;
     h_jr.clear();
-    h_pairs.clear();
-    h_moves.clear();
     for (auto& S : sec) {
       if (!S.basis || S.m == 0) continue;
+      const int src = sid(S.a, S.b);
       const size_t jr0 = h_jr.size();
       for (int slot = 0; slot < S.m; ++slot) {
         int tr = S.slot2traj[slot];
@@ -598,8 +716,6 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
         if (ch < 0) continue;
         int mode = jm[ch];
         int loss = jk[ch] == QD_JUMP_LOSS;
-        double nd = dens_host[(size_t)tr * (M + 1) + mode] * dens_host[(size_t)tr * (M + 1) + M];
-        JumpRec r{slot, mode, loss, (int)pend.size(), 1.0 / std::sqrt(nd)};
         int na = S.a, nb = S.b;
         if (loss) {
           if (mode < nsite)
@@ -607,64 +723,59 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
           else
             --nb;
         }
+        pend_of[tr] = (int)pend.size();
+        h_jr.push_back(JumpRec{slot, mode, loss, (int)pend.size(), h_jscale[tr], src, loss ? sid(na, nb) : src});
         pend.push_back({tr, na, nb, (int)pend.size(), slot, loss});
-        h_jr.push_back(r);
       }
-      const int njr = (int)(h_jr.size() - jr0);
-      if (njr == 0) continue;
-      // staging capacity grows on demand
-      if (!d_staging || (int)pend.size() > stg_cap) {
-        int ncap = std::max<int>((int)pend.size() * 2, 64);
-        cplx* nsg = nullptr;
-        if (hipMalloc(&nsg, sizeof(cplx) * (size_t)ncap * nmax) != hipSuccess) {
+      // the sectors a loss out of S reaches, built on the first jump out of S
+      if (h_jr.size() > jr0) {
+        if (S.a > 0 && (rc = ensure_sector(S.a - 1, S.b))) break;
+        if (S.b > 0 && (rc = ensure_sector(S.a, S.b - 1))) break;
+      }
+    }
+    if (rc) break;
+    if (!pend.empty()) {
+      // staging capacity grows on demand (nothing in it outlives a step)
+      if ((int)pend.size() > stg_cap) {
+        if (d_staging) hipFree(d_staging);
+        d_staging = nullptr;
+        stg_cap = 0;
+        const int ncap = std::min(ntraj, std::max<int>((int)pend.size() * 2, 64));
+        if (hipMalloc(&d_staging, sizeof(cplx) * (size_t)ncap * nmax) != hipSuccess) {
+          d_staging = nullptr;
           rc = qd_set_error(ctx, QD_E_OOM, "jump staging");
           break;
         }
-        if (d_staging) {
-          hipMemcpyAsync(nsg, d_staging, sizeof(cplx) * (size_t)stg_cap * nmax, hipMemcpyDeviceToDevice, st);
-          hipStreamSynchronize(st);
-          hipFree(d_staging);
-        }
-        d_staging = nsg;
         stg_cap = ncap;
       }
-      // target sector descriptors for losses out of S
-      SectorDesc tup{}, tdn{};
-      if (S.a > 0) {
-        if ((rc = ensure_sector(S.a - 1, S.b))) break;
-        tup = sec[sid(S.a - 1, S.b)].basis->sec;
+      if (tab_dirty) {
+        upload(pin_tab, d_sectab, h_sectab.data(), sizeof(SecTab) * h_sectab.size());
+        tab_dirty = false;
       }
-      if (S.b > 0) {
-        if ((rc = ensure_sector(S.a, S.b - 1))) break;
-        tdn = sec[sid(S.a, S.b - 1)].basis->sec;
-      }
-      const int first = (int)pend.size() - njr;
-      zero_kernel<<<g1((int64_t)njr * nmax, 256), 256, 0, st>>>((int64_t)njr * nmax, d_staging + (int64_t)first * nmax);
-      hipMemcpyAsync(d_jr + jr0, h_jr.data() + jr0, sizeof(JumpRec) * njr, hipMemcpyHostToDevice, st);
-      jump_apply_kernel<<<(unsigned)njr, 256, 0, st>>>(S.n, S.ld, S.X, S.basis->d_states, d_jr + jr0, tup, tdn, nsite,
-                                                       nmax, d_staging);
+      zero_kernel<<<g1((int64_t)pend.size() * nmax, 256), 256, 0, st>>>((int64_t)pend.size() * nmax, d_staging);
+      upload(pin_jr, d_jr, h_jr.data(), sizeof(JumpRec) * h_jr.size());
+      jump_apply_kernel<<<dim3((unsigned)pend.size(), col_chunks(nmax)), 256, 0, st>>>(d_sectab, d_jr, nmax,
+                                                                                         d_staging);
     }
-    if (rc) break;
-    // evolve every occupied sector (jumpers too; their columns are replaced below)
-    if ((rc = evolve_all())) break;
-    // commit jumps: dephasing in place; losses move to the target sector
+    if ((rc = evolve_end())) break;
+    // commit jumps: dephasing in place; losses move to the target sector.
+    // The host tables first, then one upload (commits, move lists, moves) and
+    // two launches: all commits before all moves
     if (!pend.empty()) {
       n_jumps += (int64_t)pend.size();
       std::vector<std::vector<int>> removed(sec.size());
-      std::vector<std::vector<int2>> commits(sec.size());
+      std::vector<int4> h_commit;
       for (auto& S : sec) {
         if (!S.basis || S.m == 0) continue;
+        const int si = sid(S.a, S.b);
         for (int slot = 0; slot < S.m; ++slot) {
-          int tr = S.slot2traj[slot];
+          const int tr = S.slot2traj[slot];
           if (action[tr] < 0) continue;
-          for (const Pending& p : pend)
-            if (p.traj == tr) {
-              if (!p.loss)
-                commits[sid(S.a, S.b)].push_back(make_int2(p.sidx, slot));
-              else
-                removed[sid(S.a, S.b)].push_back(slot);
-              break;
-            }
+          const Pending& p = pend[pend_of[tr]];
+          if (!p.loss)
+            h_commit.push_back(make_int4(p.sidx, slot, si, 0));
+          else
+            removed[si].push_back(slot);
         }
       }
       // losses: append to targets
@@ -675,38 +786,52 @@ extern "C" int qd_traj_run(qd_ctx* ctx, int nsite, const qd_term* terms, int nte
         int slot = T.m++;
         T.slot2traj.resize(std::max<size_t>(T.slot2traj.size(), (size_t)T.m));
         T.slot2traj[slot] = p.traj;
-        commits[sid(p.a, p.b)].push_back(make_int2(p.sidx, slot));
-      }
-      for (size_t i = 0; i < sec.size(); ++i) {
-        Sector& S = sec[i];
-        if (!S.basis || commits[i].empty()) continue;
-        const size_t p0 = h_pairs.size();
-        h_pairs.insert(h_pairs.end(), commits[i].begin(), commits[i].end());
-        hipMemcpyAsync(d_pairs + p0, h_pairs.data() + p0, sizeof(int2) * commits[i].size(), hipMemcpyHostToDevice, st);
-        commit_kernel<<<(unsigned)commits[i].size(), 256, 0, st>>>(S.n, S.ld, S.X, d_pairs + p0, nmax, d_staging);
+        h_commit.push_back(make_int4(p.sidx, slot, sid(p.a, p.b), 0));
       }
       // compaction of the source sectors (swap-remove, highest removed slot first)
+      std::vector<int4> h_lists;
+      std::vector<int2> h_mv;
       for (size_t i = 0; i < sec.size(); ++i) {
         Sector& S = sec[i];
         if (!S.basis || removed[i].empty()) continue;
         std::sort(removed[i].begin(), removed[i].end(), std::greater<int>());
-        const size_t m0 = h_moves.size();
+        const int m0 = (int)h_mv.size();
         for (int slot : removed[i]) {
           int last = S.m - 1;
           if (slot != last) {
-            h_moves.push_back(make_int2(last, slot));
+            h_mv.push_back(make_int2(last, slot));
             S.slot2traj[slot] = S.slot2traj[last];
           }
           --S.m;
         }
-        const int nmv = (int)(h_moves.size() - m0);
-        if (nmv > 0) {
-          hipMemcpyAsync(d_moves + m0, h_moves.data() + m0, sizeof(int2) * nmv, hipMemcpyHostToDevice, st);
-          move_cols_kernel<<<g1(S.n, 256), 256, 0, st>>>(S.n, S.ld, S.X, d_moves + m0, nmv);
-        }
+        const int nmv = (int)h_mv.size() - m0;
+        if (nmv > 0) h_lists.push_back(make_int4((int)i, m0, nmv, 0));
       }
+      for (const Pending& p : pend) pend_of[p.traj] = -1;
+      // one buffer: commits (int4), move lists (int4), moves (int2)
+      const size_t b_c = sizeof(int4) * h_commit.size(), b_l = sizeof(int4) * h_lists.size(),
+                   b_m = sizeof(int2) * h_mv.size();
+      h_pm.resize(b_c + b_l + b_m);
+      if (b_c) std::memcpy(h_pm.data(), h_commit.data(), b_c);
+      if (b_l) std::memcpy(h_pm.data() + b_c, h_lists.data(), b_l);
+      if (b_m) std::memcpy(h_pm.data() + b_c + b_l, h_mv.data(), b_m);
+      upload(pin_pm, d_pm, h_pm.data(), h_pm.size());
+      if (!h_commit.empty())
+        commit_kernel<<<dim3((unsigned)h_commit.size(), col_chunks(nmax)), 256, 0, st>>>(
+            d_sectab, reinterpret_cast<const int4*>(d_pm), nmax, d_staging);
+      if (!h_lists.empty())
+        move_cols_kernel<<<dim3((unsigned)((nmax + 255) / 256), (unsigned)h_lists.size()), 256, 0, st>>>(
+            d_sectab, reinterpret_cast<const int4*>(d_pm + b_c), reinterpret_cast<const int2*>(d_pm + b_c + b_l));
     }
     QD_CHECK_LAUNCH(ctx);
+  }
+  if (rc == QD_OK) {
+    // the per-step sums over trajectories, kept on the device during the run
+    hipError_t e = hipMemcpyAsync(dsum.data(), d_dsum, sizeof(double) * dsum.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dsq.data(), d_dsq, sizeof(double) * dsq.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = qd_set_error(ctx, QD_E_HIP, "trajectory density sums: %s", hipGetErrorString(e));
   }
   if (rc == QD_OK) {
     for (int step = 0; step <= nsteps; ++step)
