@@ -10,12 +10,13 @@ from qudrip_amd import workloads as WL
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("matfree", [True, False])
-def test_hubbard_quench_matches_dense(ctx, matfree):
-    """3x2 periodic Hubbard, 2 up / 2 down: GS by Lanczos, quench U 4 -> 2, Arnoldi dt=0.05."""
+@pytest.mark.parametrize("matfree,rt_method", [(True, 1), (False, 1), (True, 2)])
+def test_hubbard_quench_matches_dense(ctx, matfree, rt_method):
+    """3x2 periodic Hubbard, 2 up / 2 down: GS by Lanczos, quench U 4 -> 2, Arnoldi dt=0.05
+    (real-time basis fully reorthogonalised, or by the partial scheme)."""
     Lx, Ly, nup, ndn = 3, 2, 2, 2
     res = WL.hubbard_quench(ctx, Lx, Ly, nup, ndn, U0=4.0, U1=2.0, k_gs=60, k_rt=20, dt=0.05, steps=20,
-                            matfree=matfree, two_vector_gs=True)
+                            matfree=matfree, two_vector_gs=True, rt_method=rt_method)
     nsite = Lx * Ly
     bonds = Q.hubbard_bonds_square(Lx, Ly)
     idx, h0 = Q.hubbard(nsite, bonds, 1.0, 4.0)
@@ -38,6 +39,22 @@ def test_hubbard_quench_matches_dense(ctx, matfree):
     assert np.max(np.abs(np.array(res["docc"]) - np.array(docc))) < 1e-9
     assert abs(res["energy_end"] - res["energy_start"]) < 1e-9
     assert abs(res["norm_end"] - 1.0) < 1e-9
+
+
+def test_quench_partial_reorth_matches_full(ctx):
+    """3x4 periodic Hubbard (3,3), n = 48,400, k_rt = 30: the real-time basis
+    kept semi-orthogonal by partial reorthogonalisation gives the same double
+    occupancy at every step and the same final state as the fully
+    reorthogonalised basis, to rounding."""
+    import torch
+    kw = dict(U0=4.0, U1=2.0, k_gs=100, k_rt=30, dt=0.05, steps=10, matfree=True)
+    full = WL.hubbard_quench(ctx, 3, 4, 3, 3, **kw)
+    part = WL.hubbard_quench(ctx, 3, 4, 3, 3, rt_method=2, **kw)
+    assert part["n"] == 48400 and part["rt_method"] == 2
+    assert np.max(np.abs(np.array(part["docc"]) - np.array(full["docc"]))) < 1e-12
+    d = float(torch.linalg.vector_norm(part["psi"] - full["psi"]) / torch.linalg.vector_norm(full["psi"]))
+    assert d < 1e-11
+    assert abs(part["energy_end"] - part["energy_start"]) < 1e-10
 
 
 def test_hubbard_4x4_free_fermions_closed_form(ctx):
