@@ -606,7 +606,8 @@ def serving(ctx, args):
     workers, ground states stopped when the Ritz-residual bound r^2/gap or the
     move of E0 over 4 Krylov vectors is <= 1e-12 |E0|, matrix-free real-time
     operator with the Krylov dimension set per step by Saad's estimate).  The
-    ground states use partial reorthogonalisation by default.  Rows, to
+    Krylov bases (ground states and real-time steps) use partial
+    reorthogonalisation by default.  Rows, to
     separate what serving buys from what the per-request algorithm buys:
       served             -- the stream, args.serve_workers workers;
       served_1_worker    -- the stream, one worker;

@@ -501,10 +501,12 @@ int qd_server_submit(qd_server* s, const qd_request* req);
  * of them, on the model's stored matrix; arrivals join at the next step).
  * 1 (default): each request alone on its worker. */
 int qd_server_set_batch(qd_server* s, int slots);
-/* Orthogonalisation scheme of ground-state requests submitted from now on
+/* Orthogonalisation scheme of the Krylov bases of requests submitted from
+ * now on, ground states and (unbatched) real-time steps alike
  * (qd_krylov_set_method): 2 (default) = local window with partial
  * reorthogonalisation, 1 = full delayed CGS2 (the scheme of
- * qd_krylov_ground_state), 0 = CGS2.  No reference counterpart. */
+ * qd_krylov_ground_state and qd_krylov_evolve), 0 = CGS2.  No reference
+ * counterpart. */
 int qd_server_set_gs_method(qd_server* s, int method);
 /* Cancels every request with this tag: a queued one is answered at once
  * (status QD_E_CANCELLED, worker -1); a running one stops at its next

@@ -308,7 +308,7 @@ int run_ground_state(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request
   return done(QD_OK);
 }
 
-int run_evolve(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request& r, qd_result* out) {
+int run_evolve(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request& r, int method, qd_result* out) {
   const int64_t n = M.n;
   cplx *diag = nullptr, *psi = nullptr, *nxt = nullptr;
   qd_linop* op = nullptr;
@@ -334,7 +334,10 @@ int run_evolve(qd_server* S, qd_ctx* ctx, const Model& M, const qd_request& r, q
   scale_diag_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, r.coeff, M.d_diagD, diag);
   if ((rc = qd_linop_with_diagonal(ctx, M.structured ? M.opK : M.opA, (const double*)diag, &op))) return done(rc);
   if ((rc = qd_seeded_state(ctx, n, r.seed, (double*)psi))) return done(rc);
-  if ((rc = qd_krylov_create(ctx, n, r.k, 0, &kr))) return done(rc);
+  // the basis scheme of the request (A + c D is Hermitian, so the partial
+  // scheme applies to the real-time basis as to a ground state's)
+  if ((rc = qd_krylov_create(ctx, n, r.k, method >= 2 ? 1 : 0, &kr))) return done(rc);
+  if (method != 1 && (rc = qd_krylov_set_method(kr, method))) return done(rc);
   double h0[2] = {0.0, 0.0}, n0 = 0.0;
   if ((rc = qd_expect(ctx, op, (const double*)psi, h0))) return done(rc);
   if ((rc = qd_norm2(ctx, n, (const double*)psi, &n0))) return done(rc);
@@ -751,7 +754,7 @@ void worker_main(qd_server* S, int wi) {
     res.worker = wi;
     const Model& M = *job.model;
     res.status = job.req.kind == QD_REQ_GROUND_STATE ? run_ground_state(S, ctx, M, job.req, job.gs_method, &res)
-                                                     : run_evolve(S, ctx, M, job.req, &res);
+                                                     : run_evolve(S, ctx, M, job.req, job.gs_method, &res);
     res.t_end = S->now();
     {
       std::lock_guard<std::mutex> lk(S->mu);
